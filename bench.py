@@ -1,0 +1,149 @@
+#!/usr/bin/env python3
+"""Benchmark of the KD-tree KNN query on one MI355X.
+
+Default workload (``plain_gpu``): d=3 fp32, 4,000,000 reference points U[0,1]^3, 16,000,000
+queries (half U[-0.1,1.1]^3, half from a 32-component Gaussian mixture, sigma 0.02), k=16.
+A step is one ``TorchKDTree.query`` over all 16M queries (leaf presort + traversal + outputs:
+squared distances, structured int32 indices and original int64 indices), with the tree and the
+queries already resident in HBM.  The tree build and upload are outside the timed region, as
+in the reference's benchmark (README.md:75).
+
+Prints one JSON line.  ``roofline`` refers to the traversal kernel: its average duration is
+measured with HIP events on the launch stream during the timed steps; ``achieved`` divides
+the algorithmic bytes of one launch by it.  ``cpu_baseline`` times the oracle (the C++
+restatement of the reference CPU search, OpenMP over queries) on a bounded sample of the same
+queries and the same tree on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="plain_gpu", choices=["plain_gpu"])
+    ap.add_argument("--levels", default="auto", help="tree depth: 'auto', 'default' or an int")
+    ap.add_argument("--tie", default="reference", choices=["reference", "lower_index"])
+    ap.add_argument("--presort", default="auto", choices=["auto", "on", "off"])
+    ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="queries timed on the CPU (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    return ap.parse_args()
+
+
+def main():
+    args = _parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (queries shard with no exchange; "
+              "see the sharding note in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import workloads
+    import torch_kdtree_amd as tk
+    from torch_kdtree_amd import torch_knn
+
+    dev = torch.device("cuda", 0)
+    w = workloads.plain_gpu()
+    ref, qry, k = w["ref"], w["queries"], w["k"]
+    n, d = ref.shape
+    nq = qry.shape[0]
+    levels = {"auto": "auto", "default": None}.get(args.levels, None if args.levels == "default" else args.levels)
+    if isinstance(levels, str) and levels != "auto":
+        levels = int(levels)
+    t0 = time.perf_counter()
+    tree = tk.build_kd_tree(torch.from_numpy(ref), device=dev, levels=levels, tie_break=args.tie,
+                            presort=args.presort)
+    build_s = time.perf_counter() - t0
+
+    q = torch.from_numpy(qry).to(dev)
+    res_d = torch.empty((nq, k), dtype=torch.float32, device=dev)
+    res_i = torch.empty((nq, k), dtype=torch.int32, device=dev)
+    for _ in range(args.warmup):
+        tree.query(q, nr_nns_searches=k, result_dists=res_d, result_idx=res_i)
+    torch.cuda.synchronize()
+
+    torch_knn.profile_reset()
+    torch_knn.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tree.query(q, nr_nns_searches=k, result_dists=res_d, result_idx=res_i)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    torch_knn.profile_enable(False)
+    trav_ms, trav_n = torch_knn.profile_read("traverse")
+    pre_ms, pre_n = torch_knn.profile_read("presort")
+    torch_knn.profile_reset()
+
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = nq * args.steps / elapsed
+
+    # Algorithmic bytes of one traversal launch: every query read once (d*4 B) and its k results
+    # written once (4 B distance + 4 B structured index + 8 B original index), plus the tree read
+    # once (records 16 B/point, split values 4 B/node, leaf offsets 4 B/leaf).
+    rec_bytes = 16
+    tree_bytes = n * rec_bytes + ((1 << tree.levels) - 1) * 4 + ((1 << tree.levels) + 1) * 4
+    bytes_per_query = d * 4 + k * (4 + 4 + 8)
+    launch_bytes = nq * bytes_per_query + tree_bytes
+    trav_avg_ms = trav_ms / max(trav_n, 1)
+    achieved = launch_bytes / (trav_avg_ms * 1e-3) / 1e9
+
+    cpu = None
+    if args.cpu_sample > 0:
+        import oracle
+
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        half = w["halves"][0]
+        s = min(args.cpu_sample // 2, half)
+        sample = np.ascontiguousarray(np.concatenate([qry[:s], qry[half:half + s]]))
+        ot = oracle.tree_from_product(tree)
+        ot.search(sample[:1024], k, nthreads=threads)
+        t0 = time.perf_counter()
+        ot.search(sample, k, nthreads=threads)
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": sample.shape[0] / cpu_s, "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": "first %d queries of each half of the 16M-query set (%d total), same tree (levels=%d), k=%d; "
+                         "oracle = C++ restatement of KDTreeKNNSearch, OpenMP over queries, %d threads; %.2f s wall"
+                         % (s, sample.shape[0], tree.levels, k, threads, cpu_s)}
+
+    line = {
+        "metric": "KNN queries/s at d=3,k=16 fp32 (+fwd+bwd/s); batched vs per-cloud, same GPU",
+        "value": value,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (numpy PCG64 seed 1): refs U[0,1]^3; queries half U[-0.1,1.1]^3, half 32-GMM sigma 0.02",
+        "config": {"workload": "plain_gpu: d=3 fp32, N=4,000,000 refs, Q=16,000,000 queries, k=16",
+                   "levels": tree.levels, "tie_break": args.tie, "presort": args.presort,
+                   "tree_build_s": round(build_s, 3),
+                   "presort_ms_per_step": pre_ms / max(pre_n, 1), "traverse_ms_per_step": trav_avg_ms},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "knn_tree_kernel<float,3,16,false>",
+                     "algorithmic_bytes_per_launch": launch_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,138 @@
+/*
+ * torch_kdtree_amd — C ABI of the MI355X (gfx950) KD-tree KNN operator.
+ *
+ * Drop-in boundary for the hot path of thomgrand/torch_kdtree:
+ *     build_kd_tree(points).query(q, nr_nns_searches=k)
+ * The reference binds this path to Python through pybind11 (module `torch_knn`,
+ * src/interface.cpp); each entry point below names the reference interface it replaces.
+ * This library's own pybind11 module `torch_knn` is a thin layer over these functions.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Point arrays are row-major [n, dims] of float
+ *     (TK_F32) or double (TK_F64); KNN outputs are row-major [n_queries, k].
+ *   - "structured" indices are positions in the tree's reordered point array (what the
+ *     reference's C++ layer writes, src/kdtree.cpp:191-192); "original" indices are rows of
+ *     the input point array (what the reference's Python layer returns,
+ *     torch_kdtree/nn_distance.py:109).
+ *   - GPU entry points take device pointers and a hipStream_t passed as void* (NULL = the
+ *     legacy default stream, the stream the reference launches on).  They are asynchronous
+ *     with respect to the host, like the reference's launches (src/kdtree_g.cu:425,444).
+ *   - Every function returns 0 on success and nonzero on failure; tk_last_error() then
+ *     returns the message (thread-local).  No function calls exit() (the reference does,
+ *     src/cutils.cuh:34-43).
+ *   - `workspace` arguments may be NULL: the library then takes stream-ordered scratch
+ *     memory (hipMallocAsync/hipFreeAsync) on the given stream.  The *_workspace functions
+ *     return the byte count a caller-provided workspace must have.
+ */
+#ifndef TORCH_KDTREE_AMD_H
+#define TORCH_KDTREE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tk_tree tk_tree; /* opaque: host tree (+ its GPU copy once uploaded) */
+
+enum { TK_F32 = 0, TK_F64 = 1 };
+
+/* Exact-distance tie policies of the GPU query.
+ * TK_TIE_REFERENCE: the reference CPU search's order (ties keep the near-first depth-first
+ *   scan order, src/kdtree.cpp:16-110 with src/nndistance.hpp:41-85); output is identical to
+ *   KDTreeKNNSearch on the same tree.
+ * TK_TIE_LOWER_INDEX: the k smallest (distance, original index) pairs. */
+enum { TK_TIE_REFERENCE = 0, TK_TIE_LOWER_INDEX = 1 };
+
+/* Query presort (groups queries by the leaf they fall into before the traversal). */
+enum { TK_PRESORT_AUTO = -1, TK_PRESORT_OFF = 0, TK_PRESORT_ON = 1 };
+
+/* Thread-local message of the last failure on this thread. */
+const char* tk_last_error(void);
+
+/* Library version string, "1.0" (torch_kdtree/__init__.py:4, CMakeLists.txt:15-16,151). */
+const char* tk_version(void);
+
+/* Replaces check_for_gpu() (src/interface.cpp:105-112): 1 when built with GPU support. */
+int tk_gpu_available(void);
+
+/* Number of visible GPUs (0 when none). */
+int tk_device_count(int* count);
+
+/* Replaces the KDTree<T,dims,use_gpu> constructor's host part (src/interface.cpp:25-36)
+ * and createKDTree (src/kdtree.hpp:494-534).  Builds the tree on the host with the
+ * reference's layout.  dims in {1,2,3,8}; 1 <= levels <= 24; n_threads <= 0 = all cores.
+ * Fails with the reference's "Ran out of points" message when n_points < 2^(levels-1). */
+int tk_tree_build(const void* points, int64_t n_points, int dims, int dtype, int levels, int n_threads,
+                  tk_tree** out);
+
+/* Replaces copyPartitionToGPU (src/kdtree_g.cu:63-98), called from the constructor at
+ * src/interface.cpp:38-41.  Copies the tree to GPU `device` (-1 = current device). */
+int tk_tree_upload(tk_tree* tree, int device);
+
+/* Replaces ~KDTree (src/interface.cpp:88-94) and freePartitionFromGPU
+ * (src/kdtree_g.cu:140-157).  Frees host and device copies; NULL is ignored. */
+void tk_tree_free(tk_tree* tree);
+
+/* Tree facts: out[0]=dims, out[1]=dtype, out[2]=levels, out[3]=n_points, out[4]=device
+ * (-1 when not uploaded). */
+int tk_tree_info(const tk_tree* tree, int32_t out[5]);
+
+/* Replaces KDTree::get_shuffled_inds (src/interface.cpp:45-50): out[n_points] receives the
+ * original index of each structured position. */
+int tk_tree_get_shuffled_inds(const tk_tree* tree, int32_t* out);
+
+/* Replaces KDTree::get_structured_points (src/interface.cpp:52-58): out[n_points*dims]. */
+int tk_tree_get_structured_points(const tk_tree* tree, void* out);
+
+/* Node and leaf arrays of the tree (the reference's PartitionInfo, src/kdtree.hpp:165-206):
+ * axis[2^levels-1], split[2^levels-1] (dtype), leaf_off[2^levels+1].  Any may be NULL. */
+int tk_tree_get_nodes(const tk_tree* tree, int32_t* axis, void* split, int32_t* leaf_off);
+
+/* Replaces KDTree::query / query_recast for CPU trees (src/interface.cpp:60-86) ->
+ * KDTreeKNNSearch (src/kdtree.cpp:164-194).  Host pointers.  dist[nq*k], idx[nq*k]
+ * structured; orig_idx[nq*k] (may be NULL) original indices. */
+int tk_knn_query_cpu(const tk_tree* tree, const void* queries, int64_t n_queries, int k, void* dist, int32_t* idx,
+                     int64_t* orig_idx, int n_threads);
+
+/* Replaces KDTree::query / query_recast for GPU trees (src/interface.cpp:60-86) ->
+ * KDTreeKNNGPUSearch (src/kdtree_g.cu:409-457).  Device pointers.  Writes squared
+ * distances and structured int32 indices like the reference; orig_idx (may be NULL) also
+ * receives the original int64 indices, which the reference computes afterwards in torch
+ * (nn_distance.py:109).  1 <= k <= min(n_points, 128) (src/kdtree_g.cu:421-422). */
+int tk_knn_query_gpu(const tk_tree* tree, const void* queries, int64_t n_queries, int k, void* dist, int32_t* idx,
+                     int64_t* orig_idx, int tie, int presort, void* workspace, size_t workspace_bytes, void* stream);
+int tk_knn_query_gpu_workspace(const tk_tree* tree, int64_t n_queries, int k, int tie, int presort, size_t* bytes);
+
+/* Replaces NmDistanceKernelLauncher[Dynamic] (src/nndistance_g.cu:142-220), which the
+ * reference compiles but never binds.  Exhaustive KNN over ref[n_ref*dims] in original
+ * order; ties go to the lower index.  idx receives original int32 indices.  k <= 32. */
+int tk_knn_bruteforce_gpu(const void* ref, int32_t n_ref, const void* queries, int64_t n_queries, int dims, int dtype,
+                          int k, void* dist, int32_t* idx, void* workspace, size_t workspace_bytes, void* stream);
+int tk_knn_bruteforce_gpu_workspace(int32_t n_ref, int64_t n_queries, int dims, int dtype, int k, size_t* bytes);
+
+/* Replaces the autograd recomputation of torch_kdtree/nn_distance.py:111-112.  Given the
+ * original-order queries[nq*dims], ref[n_ref*dims], neighbour indices idx[nq*k] (original,
+ * int64) and the upstream gradient grad_dist[nq*k] of the squared distances, writes
+ * grad_queries[nq*dims] and grad_ref[n_ref*dims] (either may be NULL).  grad_ref is
+ * deterministic: no atomics, a fixed summation order. */
+int tk_knn_backward_gpu(const void* queries, const void* ref, int64_t n_queries, int32_t n_ref, int dims, int dtype,
+                        int k, const int64_t* idx, const void* grad_dist, void* grad_queries, void* grad_ref,
+                        void* workspace, size_t workspace_bytes, void* stream);
+int tk_knn_backward_gpu_workspace(int64_t n_queries, int32_t n_ref, int k, size_t* bytes);
+
+/* Kernel timing with HIP events (no counterpart in the reference, whose only timing is
+ * commented out, src/kdtree_g.cu:435-456).  When enabled, every GPU entry point brackets its
+ * kernels with events on its stream, per stage: "presort", "traverse", "bruteforce",
+ * "backward".  tk_profile_read waits for the recorded events and returns the summed
+ * milliseconds and the number of bracketed calls of that stage. */
+int tk_profile_enable(int on);
+int tk_profile_reset(void);
+int tk_profile_read(const char* stage, double* total_ms, int64_t* calls);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TORCH_KDTREE_AMD_H */

@@ -1,0 +1,152 @@
+"""GPU parity: the HIP traversal against the oracle (C++ restatement of the reference CPU path).
+
+Reference policy: distances and indices must equal ``KDTreeKNNSearch`` (src/kdtree.cpp:164-194)
+on the same tree, bit for bit, including the order of exact-distance ties.
+Lower-index policy: distances and indices must equal the brute-force (distance, index) order.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import workloads
+import torch_kdtree_amd as tk
+from tests._util import assert_same, gpu_query, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(rng, n, q, d, dtype, scale=1e3):
+    return ((rng.standard_normal((n, d)) * scale).astype(dtype), (rng.standard_normal((q, d)) * scale).astype(dtype))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [1, 2, 3, 8])
+@pytest.mark.parametrize("k", [1, 5, 16, 33])
+def test_reference_policy_matches_oracle(cuda, dtype, d, k):
+    rng = np.random.default_rng(100 + d * 7 + k)
+    ref, qry = _data(rng, 20000, 3000, d, dtype)
+    for levels in (None, 12):
+        tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels=levels)
+        dist, sidx, oidx = gpu_query(tree, qry, k, cuda)
+        od, oi = oracle_for(tree).search(qry, k)
+        assert_same(dist, od, "distances")
+        assert_same(sidx, oi, "structured indices")
+        assert_same(oidx, tree.kdtree.get_shuffled_inds()[oi].astype(np.int64), "original indices")
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [1, 3, 8])
+@pytest.mark.parametrize("k", [1, 8, 32, 100])
+def test_lower_index_policy_matches_bruteforce(cuda, dtype, d, k):
+    rng = np.random.default_rng(200 + d * 11 + k)
+    ref, qry = _data(rng, 6000, 1500, d, dtype)
+    tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels=9, tie_break="lower_index")
+    dist, sidx, oidx = gpu_query(tree, qry, k, cuda)
+    bd, bi = oracle.bruteforce(ref, qry, k)
+    assert_same(dist, bd, "distances")
+    assert_same(oidx, bi.astype(np.int64), "original indices")
+    assert_same(tree.kdtree.get_shuffled_inds()[sidx], bi, "structured -> original")
+
+
+@pytest.mark.parametrize("policy", ["reference", "lower_index"])
+def test_exact_ties_on_integer_grid(cuda, policy):
+    """Points and queries on a small integer lattice: nearly every distance is tied."""
+    rng = np.random.default_rng(7)
+    ref = rng.integers(0, 6, (5000, 3)).astype(np.float32)
+    qry = rng.integers(0, 6, (2000, 3)).astype(np.float32) + np.float32(0.5) * rng.integers(0, 2, (2000, 3))
+    tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels=8, tie_break=policy)
+    for k in (1, 7, 20, 64):
+        dist, sidx, oidx = gpu_query(tree, qry, k, cuda)
+        if policy == "reference":
+            od, oi = oracle_for(tree).search(qry, k)
+            assert_same(dist, od, "distances")
+            assert_same(sidx, oi, "structured indices")
+        else:
+            bd, bi = oracle.bruteforce(ref, qry, k)
+            assert_same(dist, bd, "distances")
+            assert_same(oidx, bi.astype(np.int64), "original indices")
+
+
+def test_presort_does_not_change_results(cuda):
+    rng = np.random.default_rng(8)
+    ref, qry = _data(rng, 50000, 20000, 3, np.float32)
+    out = {}
+    for mode in ("on", "off"):
+        tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels=13, presort=mode)
+        out[mode] = gpu_query(tree, qry, 16, cuda)
+    for a, b in zip(out["on"], out["off"]):
+        assert_same(a, b, "presort on vs off")
+
+
+def test_small_and_degenerate_shapes(cuda):
+    rng = np.random.default_rng(9)
+    ref = rng.standard_normal((5, 3)).astype(np.float32)
+    tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda)
+    assert tree.levels == 1
+    qry = rng.standard_normal((33, 3)).astype(np.float32)
+    for k in (1, 5):
+        dist, sidx, _ = gpu_query(tree, qry, k, cuda)
+        od, oi = oracle_for(tree).search(qry, k)
+        assert_same(dist, od, "distances")
+        assert_same(sidx, oi, "indices")
+    with pytest.raises(RuntimeError):
+        tree.query(torch.from_numpy(qry).to(cuda), nr_nns_searches=6)
+    empty = torch.empty((0, 3), dtype=torch.float32, device=cuda)
+    d, i = tree.query(empty, nr_nns_searches=2)
+    assert d.shape == (0, 2) and i.shape == (0, 2)
+    one = torch.from_numpy(qry[:1]).to(cuda)
+    d, i = tree.query(one, nr_nns_searches=3)
+    od, oi = oracle_for(tree).search(qry[:1], 3)
+    assert_same(d.cpu().numpy(), od, "single query")
+
+
+def test_k_equals_n_and_minimum_points_per_level(cuda):
+    rng = np.random.default_rng(10)
+    ref = rng.standard_normal((64, 2)).astype(np.float64)
+    qry = rng.standard_normal((100, 2)).astype(np.float64)
+    tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels=7)  # 2^(7-1) = 64 points: leaves of 0/1
+    for k in (1, 64):
+        dist, sidx, _ = gpu_query(tree, qry, k, cuda)
+        od, oi = oracle_for(tree).search(qry, k)
+        assert_same(dist, od, "distances")
+        assert_same(sidx, oi, "indices")
+
+
+def test_gpu_k_limit(cuda):
+    ref = np.random.default_rng(11).standard_normal((1000, 3)).astype(np.float32)
+    tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda)
+    with pytest.raises(RuntimeError):
+        tree.query(torch.from_numpy(ref[:10]).to(cuda), nr_nns_searches=129)
+
+
+def test_plain_cpu_workload_exact(cuda):
+    """Workload 1 (100k refs, 10k queries, k=5, seed 0) at the reference's default depth."""
+    w = workloads.plain_cpu()
+    tree = tk.build_kd_tree(torch.from_numpy(w["ref"]), device=cuda)
+    assert tree.levels == 13
+    dist, sidx, oidx = gpu_query(tree, w["queries"], w["k"], cuda)
+    od, oi = oracle.OracleTree(w["ref"], 13).search(w["queries"], w["k"])
+    assert_same(dist, od, "distances")
+    assert_same(sidx, oi, "structured indices")
+
+
+def test_plain_gpu_workload_sample_exact(cuda):
+    """Workload 2 at 1/16 scale (250k refs, 1M queries, k=16), checked in full against the oracle."""
+    w = workloads.plain_gpu(n=250_000, q=1_000_000)
+    for levels in (None, "auto"):
+        tree = tk.build_kd_tree(torch.from_numpy(w["ref"]), device=cuda, levels=levels)
+        dist, sidx, _ = gpu_query(tree, w["queries"], 16, cuda)
+        od, oi = oracle_for(tree).search(w["queries"], 16)
+        assert_same(dist, od, "distances")
+        assert_same(sidx, oi, "structured indices")
+
+
+def test_repeatable(cuda):
+    rng = np.random.default_rng(12)
+    ref, qry = _data(rng, 100000, 100000, 3, np.float32)
+    tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels="auto")
+    a = gpu_query(tree, qry, 16, cuda)
+    b = gpu_query(tree, qry, 16, cuda)
+    for x, y in zip(a, b):
+        assert_same(x, y, "second run")

@@ -1,0 +1,171 @@
+// Device-side building blocks shared by the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace tk {
+
+#define TK_HIP_CHECK(expr)                                                                          \
+    do {                                                                                            \
+        hipError_t _e = (expr);                                                                     \
+        if (_e != hipSuccess)                                                                       \
+            throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(_e) + " at " + \
+                                     __FILE__ + ":" + std::to_string(__LINE__));                    \
+    } while (0)
+
+// ---------------------------------------------------------------------------------------------
+// Point records in HBM.  A leaf is a contiguous run of records in structured order; each record
+// holds the point's coordinates followed by its original index, padded so that one record is
+// one (or a few) 16-byte loads: fp32 d=3 is exactly one float4 (x, y, z, index bits).
+// ---------------------------------------------------------------------------------------------
+template <typename T, int D>
+struct RecLayout {
+    static constexpr int kRaw = D * (int)sizeof(T) + 4;
+    static constexpr int kBytes = kRaw <= 8 ? 8 : ((kRaw + 15) / 16) * 16;
+    static constexpr int kWords = kBytes / 4;
+    static constexpr int kIdxWord = D * (int)sizeof(T) / 4;
+};
+
+template <typename T, int D>
+struct Rec {
+    T c[D];
+    int32_t o;
+};
+
+template <typename T, int D>
+__device__ __forceinline__ Rec<T, D> load_rec(const unsigned char* __restrict__ base, int32_t s) {
+    using L = RecLayout<T, D>;
+    uint32_t w[L::kWords];
+    if constexpr (L::kBytes % 16 == 0) {
+        const uint4* p = reinterpret_cast<const uint4*>(base + (size_t)s * L::kBytes);
+#pragma unroll
+        for (int i = 0; i < L::kBytes / 16; ++i) {
+            const uint4 v = p[i];
+            w[4 * i + 0] = v.x;
+            w[4 * i + 1] = v.y;
+            w[4 * i + 2] = v.z;
+            w[4 * i + 3] = v.w;
+        }
+    } else {
+        const uint2 v = reinterpret_cast<const uint2*>(base + (size_t)s * L::kBytes)[0];
+        w[0] = v.x;
+        w[1] = v.y;
+    }
+    Rec<T, D> r;
+    __builtin_memcpy(r.c, w, sizeof(T) * D);
+    r.o = (int32_t)w[L::kIdxWord];
+    return r;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Arithmetic.  The reference computes every distance and bound in T with separate rounding of
+// each subtraction, product and sum (src/kdtree.hpp:546-614).  All kernels are compiled with
+// -ffp-contract=off, so a*b+c below is a rounded product followed by a rounded sum.
+// ---------------------------------------------------------------------------------------------
+template <typename T, int D>
+__device__ __forceinline__ T sq_dist(const T (&q)[D], const T (&r)[D]) {
+    T s = T(0);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        const T v = q[i] - r[i];
+        s = s + v * v;
+    }
+    return s;
+}
+
+// Reference pruning bound, src/kdtree.hpp:606-614: v = pp - q, v[a] += m - pp[a], sum of squares.
+template <typename T, int D>
+__device__ __forceinline__ T proj_bound(const T (&q)[D], const T (&pp)[D], int a, T m) {
+    T s = T(0);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        T v = pp[i] - q[i];
+        if (i == a) v = v + (m - pp[i]);
+        s = s + v * v;
+    }
+    return s;
+}
+
+// Bound used by the lower-index tie policy: v[a] = m - q[a] rounded once, so that by the
+// monotonicity of rounding it never exceeds the computed distance of any point beyond the
+// split (the reference's two-rounding form above can exceed it by an ulp).
+template <typename T, int D>
+__device__ __forceinline__ T clean_bound(const T (&q)[D], const T (&pp)[D], int a, T m) {
+    T s = T(0);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        const T v = (i == a) ? (m - q[i]) : (pp[i] - q[i]);
+        s = s + v * v;
+    }
+    return s;
+}
+
+template <typename T>
+__device__ __forceinline__ T pos_inf();
+template <>
+__device__ __forceinline__ float pos_inf<float>() { return __builtin_huge_valf(); }
+template <>
+__device__ __forceinline__ double pos_inf<double>() { return __builtin_huge_val(); }
+
+// ---------------------------------------------------------------------------------------------
+// k-best list in VGPRs.  K is the compile-time capacity (a power of two >= k); the first K-k
+// slots hold -inf and never move, so the live list is slots K-k .. K-1 and the worst entry is
+// always slot K-1.  Fully unrolled, branch-free compare-and-shift.
+//
+// kLower == false (reference policy, src/nndistance.hpp:41-85): insert iff d < worst, at the
+//   upper-bound position (after every equal distance already present).
+// kLower == true (lower-index policy): the list is ordered by (d, original index).
+// ---------------------------------------------------------------------------------------------
+template <typename T, int K, bool kLower>
+struct KBest {
+    T d[K];
+    int32_t s[K];                 // structured index
+    int32_t o[kLower ? K : 1];    // original index (lower-index policy only)
+
+    __device__ __forceinline__ void init(int k) {
+        const int pad = K - k;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            d[j] = j < pad ? -pos_inf<T>() : pos_inf<T>();
+            s[j] = 0;
+            if constexpr (kLower) o[j] = j < pad ? -1 : 0x7fffffff;
+        }
+    }
+
+    __device__ __forceinline__ T worst() const { return d[K - 1]; }
+
+    __device__ __forceinline__ bool accepts(T dist, int32_t orig) const {
+        if constexpr (kLower)
+            return dist < d[K - 1] || (dist == d[K - 1] && orig < o[K - 1]);
+        else
+            return dist < d[K - 1];
+    }
+
+    __device__ __forceinline__ bool after(int j, T dist, int32_t orig) const {
+        if constexpr (kLower)
+            return d[j] > dist || (d[j] == dist && o[j] > orig);
+        else
+            return d[j] > dist;
+    }
+
+    __device__ __forceinline__ void insert(T dist, int32_t sidx, int32_t orig) {
+#pragma unroll
+        for (int j = K - 1; j > 0; --j) {
+            const bool shift = after(j - 1, dist, orig);
+            const bool here = after(j, dist, orig);
+            d[j] = shift ? d[j - 1] : (here ? dist : d[j]);
+            s[j] = shift ? s[j - 1] : (here ? sidx : s[j]);
+            if constexpr (kLower) o[j] = shift ? o[j - 1] : (here ? orig : o[j]);
+        }
+        const bool here0 = after(0, dist, orig);
+        d[0] = here0 ? dist : d[0];
+        s[0] = here0 ? sidx : s[0];
+        if constexpr (kLower) o[0] = here0 ? orig : o[0];
+    }
+};
+
+}  // namespace tk

@@ -1,0 +1,179 @@
+// CPU KNN search for trees built on the host device ("cpu" trees, the reference's
+// KDTreeCPU{d}D[F] classes, src/interface.cpp:60-77 -> src/kdtree.cpp:164-194).
+//
+// Same semantics as the reference's iterative search (src/kdtree.cpp:16-110):
+//   * near-first depth-first traversal; the near child of a node with split (a, m) is the
+//     left one iff q[a] < m;
+//   * a projected point pp starts at q.  When the far child of an internal node is tested,
+//     pp[a] = m first; the far child is entered iff bound(q, pp, a, m) < worst, and pp[a]
+//     goes back to q[a] when the node is left.  At a leaf-parent the near leaf is scanned,
+//     then the far leaf iff bound(q, pp, a, m) < worst with pp as it stands;
+//   * bound = sum_i v_i^2 with v = pp - q and v[a] += m - pp[a] (src/kdtree.hpp:606-614);
+//   * candidates are scanned in structured order, rejected when d >= worst, otherwise
+//     inserted at the upper-bound position (src/nndistance.hpp:41-85).
+// The reference keeps a per-node tag array; here the state is the node index and the
+// child we came back from, which is all the tag array encodes on the active path.
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include <omp.h>
+
+#include "tree.hpp"
+
+namespace tk {
+namespace {
+
+template <typename T, int D>
+inline T sq_dist(const T* q, const T* r) {
+    T s = T(0);
+    for (int i = 0; i < D; ++i) {
+        const T v = q[i] - r[i];
+        s = s + v * v;
+    }
+    return s;
+}
+
+template <typename T, int D>
+inline T proj_bound(const T* q, const T* pp, int a, T m) {
+    T s = T(0);
+    for (int i = 0; i < D; ++i) {
+        T v = pp[i] - q[i];
+        if (i == a) v = v + (m - pp[i]);
+        s = s + v * v;
+    }
+    return s;
+}
+
+template <typename T, int D>
+struct Searcher {
+    const Tree& t;
+    const T* split;
+    const T* pts;
+    int k;
+
+    void scan(int32_t leaf, const T* q, T* bd, int32_t* bi) const {
+        const int32_t beg = t.leaf_off[leaf], end = t.leaf_off[leaf + 1];
+        for (int32_t s = beg; s < end; ++s) {
+            const T d = sq_dist<T, D>(q, pts + (int64_t)s * D);
+            if (!(d < bd[k - 1])) continue;
+            int lo = 0, cnt = k;  // upper bound: first position with bd[pos] > d
+            while (cnt > 0) {
+                const int step = cnt / 2;
+                if (d >= bd[lo + step]) {
+                    lo += step + 1;
+                    cnt -= step + 1;
+                } else {
+                    cnt = step;
+                }
+            }
+            std::memmove(bd + lo + 1, bd + lo, sizeof(T) * (size_t)(k - 1 - lo));
+            std::memmove(bi + lo + 1, bi + lo, sizeof(int32_t) * (size_t)(k - 1 - lo));
+            bd[lo] = d;
+            bi[lo] = s;
+        }
+    }
+
+    void one(const T* q, T* bd, int32_t* bi) const {
+        T pp[D];
+        for (int i = 0; i < D; ++i) pp[i] = q[i];
+        const int L = t.levels;
+        const int32_t nn = t.n_nodes;
+        int32_t node = 0;
+        int level = 0;
+        auto descend = [&]() {
+            while (level < L - 1) {
+                const int a = level % D;
+                node = 2 * node + (q[a] < split[node] ? 1 : 2);
+                ++level;
+            }
+        };
+        descend();
+        for (;;) {
+            // at a leaf-parent: near leaf, then the far leaf if the bound allows it
+            {
+                const int a = level % D;
+                const T m = split[node];
+                const bool lower = q[a] < m;
+                const int32_t left_leaf = 2 * node + 1 - nn;
+                scan(lower ? left_leaf : left_leaf + 1, q, bd, bi);
+                if (proj_bound<T, D>(q, pp, a, m) < bd[k - 1]) scan(lower ? left_leaf + 1 : left_leaf, q, bd, bi);
+                if (level == 0) return;
+                pp[a] = q[a];
+            }
+            // climb until a far child is worth entering
+            bool entered = false;
+            while (!entered) {
+                const int32_t child = node;
+                node = (node - 1) >> 1;
+                --level;
+                const int a = level % D;
+                const T m = split[node];
+                const bool lower = q[a] < m;
+                const int32_t near_child = 2 * node + (lower ? 1 : 2);
+                if (child == near_child) {
+                    pp[a] = m;
+                    if (proj_bound<T, D>(q, pp, a, m) < bd[k - 1]) {
+                        node = 2 * node + (lower ? 2 : 1);
+                        ++level;
+                        entered = true;
+                        break;
+                    }
+                }
+                pp[a] = q[a];
+                if (level == 0) return;
+            }
+            descend();
+        }
+    }
+};
+
+template <typename T, int D>
+void query_typed(const Tree& t, const T* q, int64_t nq, int k, T* dist, int32_t* idx, int64_t* orig,
+                 int n_threads) {
+    Searcher<T, D> s{t, t.split_ptr<T>(), t.points_ptr<T>(), k};
+    if (n_threads <= 0) n_threads = omp_get_max_threads();
+#pragma omp parallel num_threads(n_threads)
+    {
+        std::vector<T> bd(k);
+        std::vector<int32_t> bi(k);
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t i = 0; i < nq; ++i) {
+            std::fill(bd.begin(), bd.end(), std::numeric_limits<T>::infinity());
+            std::fill(bi.begin(), bi.end(), 0);
+            s.one(q + i * D, bd.data(), bi.data());
+            std::memcpy(dist + i * k, bd.data(), sizeof(T) * (size_t)k);
+            std::memcpy(idx + i * k, bi.data(), sizeof(int32_t) * (size_t)k);
+            if (orig)
+                for (int j = 0; j < k; ++j) orig[i * k + j] = t.shuffled[bi[j]];
+        }
+    }
+}
+
+}  // namespace
+
+void query_cpu(const Tree& t, const void* queries, int64_t n_queries, int k, void* dist, int32_t* idx,
+               int64_t* orig_idx, int n_threads) {
+    if (k < 1) throw std::runtime_error("nr_nns_searches must be >= 1");
+    if (k > t.n_points) throw std::runtime_error("You requested more nearest neighbors than there are in the KD-Tree");
+#define TK_Q(TT, DD) \
+    query_typed<TT, DD>(t, static_cast<const TT*>(queries), n_queries, k, static_cast<TT*>(dist), idx, orig_idx, n_threads)
+    if (t.dtype == kF32) {
+        switch (t.dims) {
+            case 1: TK_Q(float, 1); break;
+            case 2: TK_Q(float, 2); break;
+            case 3: TK_Q(float, 3); break;
+            case 8: TK_Q(float, 8); break;
+        }
+    } else {
+        switch (t.dims) {
+            case 1: TK_Q(double, 1); break;
+            case 2: TK_Q(double, 2); break;
+            case 3: TK_Q(double, 3); break;
+            case 8: TK_Q(double, 8); break;
+        }
+    }
+#undef TK_Q
+}
+
+}  // namespace tk

@@ -1,0 +1,364 @@
+// GPU tree upload, the leaf-bucket presort (scan + scatter) and the query driver.
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "knn_gpu.hpp"
+#include "knn_tree.hpp"
+
+namespace tk {
+
+// ------------------------------------------------------------------------------------------
+// Small helpers
+// ------------------------------------------------------------------------------------------
+StreamScratch::StreamScratch(size_t bytes, hipStream_t s) : stream(s) {
+    if (bytes) TK_HIP_CHECK(hipMallocAsync(&ptr, bytes, s));
+}
+StreamScratch::~StreamScratch() {
+    if (ptr) (void)hipFreeAsync(ptr, stream);
+}
+
+DeviceGuard::DeviceGuard(int device) {
+    TK_HIP_CHECK(hipGetDevice(&prev));
+    if (device >= 0 && device != prev) TK_HIP_CHECK(hipSetDevice(device));
+}
+DeviceGuard::~DeviceGuard() {
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
+}
+
+namespace {
+struct ProfEntry {
+    std::string stage;
+    hipEvent_t start, stop;
+};
+std::mutex g_prof_mu;
+std::vector<ProfEntry> g_prof;
+std::atomic<bool> g_prof_on{false};
+}  // namespace
+
+void prof_enable(bool on) { g_prof_on = on; }
+
+void prof_reset() {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (auto& e : g_prof) {
+        (void)hipEventDestroy(e.start);
+        (void)hipEventDestroy(e.stop);
+    }
+    g_prof.clear();
+}
+
+bool prof_read(const char* stage, double* total_ms, int64_t* launches) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    double ms = 0;
+    int64_t n = 0;
+    for (auto& e : g_prof) {
+        if (e.stage != stage) continue;
+        TK_HIP_CHECK(hipEventSynchronize(e.stop));
+        float m = 0;
+        TK_HIP_CHECK(hipEventElapsedTime(&m, e.start, e.stop));
+        ms += m;
+        ++n;
+    }
+    *total_ms = ms;
+    *launches = n;
+    return n > 0;
+}
+
+ProfScope::ProfScope(const char* st, hipStream_t s) {
+    if (!g_prof_on) return;
+    stage = st;
+    stream = s;
+    TK_HIP_CHECK(hipEventCreate(&start));
+    TK_HIP_CHECK(hipEventRecord(start, s));
+}
+
+ProfScope::~ProfScope() {
+    if (!start) return;
+    hipEvent_t stop = nullptr;
+    if (hipEventCreate(&stop) != hipSuccess || hipEventRecord(stop, stream) != hipSuccess) {
+        (void)hipEventDestroy(start);
+        if (stop) (void)hipEventDestroy(stop);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof.push_back(ProfEntry{stage, start, stop});
+}
+
+int rec_bytes_for(int dtype, int dims) {
+#define TK_RB(TT)                                  \
+    switch (dims) {                                \
+        case 1: return RecLayout<TT, 1>::kBytes;   \
+        case 2: return RecLayout<TT, 2>::kBytes;   \
+        case 3: return RecLayout<TT, 3>::kBytes;   \
+        case 8: return RecLayout<TT, 8>::kBytes;   \
+    }
+    if (dtype == kF32) {
+        TK_RB(float)
+    } else {
+        TK_RB(double)
+    }
+#undef TK_RB
+    throw std::runtime_error("unsupported dims");
+}
+
+void free_device_tree(Tree& t) noexcept {
+    DeviceTree* d = t.dev;
+    if (!d) return;
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    if (d->device >= 0) (void)hipSetDevice(d->device);
+    (void)hipFree(d->split);
+    (void)hipFree(d->recs);
+    (void)hipFree(d->leaf_off);
+    (void)hipFree(d->shuffled);
+    if (prev >= 0) (void)hipSetDevice(prev);
+    delete d;
+    t.dev = nullptr;
+}
+
+void upload_tree(Tree& t, int device) {
+    free_device_tree(t);
+    if (device < 0) TK_HIP_CHECK(hipGetDevice(&device));
+    DeviceGuard g(device);
+    std::unique_ptr<DeviceTree> d(new DeviceTree());
+    d->device = device;
+    d->rec_bytes = rec_bytes_for(t.dtype, t.dims);
+    const size_t ts = dtype_size(t.dtype);
+    const size_t coord_bytes = ts * (size_t)t.dims;
+    const size_t n = (size_t)t.n_points;
+    std::vector<unsigned char> recs(n * (size_t)d->rec_bytes, 0);
+    for (size_t s = 0; s < n; ++s) {
+        unsigned char* r = &recs[s * (size_t)d->rec_bytes];
+        std::memcpy(r, &t.points[s * coord_bytes], coord_bytes);
+        std::memcpy(r + coord_bytes, &t.shuffled[s], 4);
+    }
+    try {
+        TK_HIP_CHECK(hipMalloc(&d->split, (size_t)t.n_nodes * ts));
+        TK_HIP_CHECK(hipMalloc((void**)&d->recs, recs.size()));
+        TK_HIP_CHECK(hipMalloc((void**)&d->leaf_off, sizeof(int32_t) * t.leaf_off.size()));
+        TK_HIP_CHECK(hipMalloc((void**)&d->shuffled, sizeof(int32_t) * n));
+        TK_HIP_CHECK(hipMemcpy(d->split, t.split.data(), (size_t)t.n_nodes * ts, hipMemcpyHostToDevice));
+        TK_HIP_CHECK(hipMemcpy(d->recs, recs.data(), recs.size(), hipMemcpyHostToDevice));
+        TK_HIP_CHECK(hipMemcpy(d->leaf_off, t.leaf_off.data(), sizeof(int32_t) * t.leaf_off.size(),
+                               hipMemcpyHostToDevice));
+        TK_HIP_CHECK(hipMemcpy(d->shuffled, t.shuffled.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    } catch (...) {
+        (void)hipFree(d->split);
+        (void)hipFree(d->recs);
+        (void)hipFree(d->leaf_off);
+        (void)hipFree(d->shuffled);
+        throw;
+    }
+    t.dev = d.release();
+}
+
+// ------------------------------------------------------------------------------------------
+// Scan / scatter kernels for the presort
+// ------------------------------------------------------------------------------------------
+__global__ void fill_i32_kernel(int32_t* __restrict__ p, int64_t n, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ int wave_inclusive_scan(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(v, off, 64);
+        if (lane >= off) v += y;
+    }
+    return v;
+}
+
+// Exclusive scan over a block of kScanBlock threads; `total` receives the block sum.
+__device__ __forceinline__ int block_exclusive_scan(int v, int* wsum, int& total) {
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    constexpr int kWaves = kScanBlock / 64;
+    const int inc = wave_inclusive_scan(v);
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+        const int ws = lane < kWaves ? wsum[lane] : 0;
+        const int wi = wave_inclusive_scan(ws);
+        if (lane < kWaves) wsum[lane] = wi - ws;
+        if (lane == kWaves - 1) wsum[kWaves] = wi;
+    }
+    __syncthreads();
+    total = wsum[kWaves];
+    const int r = inc - v + wsum[wid];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_block_sums_kernel(const int32_t* __restrict__ in, int32_t n,
+                                                                      int32_t* __restrict__ sums) {
+    __shared__ int wsum[kScanBlock / 64 + 1];
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanItems;
+    int v = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j)
+        if (base + j < n) v += in[base + j];
+    int total;
+    (void)block_exclusive_scan(v, wsum, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_sums_kernel(int32_t* __restrict__ sums, int32_t nsums) {
+    __shared__ int wsum[kScanBlock / 64 + 1];
+    const int base = threadIdx.x * kScanItems;
+    int x[kScanItems];
+    int v = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        x[j] = base + j < nsums ? sums[base + j] : 0;
+        v += x[j];
+    }
+    int total;
+    int run = block_exclusive_scan(v, wsum, total);
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        if (base + j < nsums) sums[base + j] = run;
+        run += x[j];
+    }
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_apply_kernel(const int32_t* __restrict__ in, int32_t n,
+                                                                 const int32_t* __restrict__ sums,
+                                                                 int32_t* __restrict__ out) {
+    __shared__ int wsum[kScanBlock / 64 + 1];
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanItems;
+    int x[kScanItems];
+    int v = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        x[j] = base + j < n ? in[base + j] : 0;
+        v += x[j];
+    }
+    int total;
+    int run = block_exclusive_scan(v, wsum, total) + sums[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        if (base + j < n) out[base + j] = run;
+        run += x[j];
+    }
+}
+
+__global__ void bucket_scatter_kernel(int64_t nq, const int32_t* __restrict__ bucket, const int32_t* __restrict__ rank,
+                                      const int32_t* __restrict__ offsets, int32_t* __restrict__ order) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nq) return;
+    order[offsets[bucket[i]] + rank[i]] = (int32_t)i;
+}
+
+// ------------------------------------------------------------------------------------------
+// Query driver
+// ------------------------------------------------------------------------------------------
+namespace {
+
+constexpr int64_t kPresortMinQueries = 4096;
+constexpr int kMaxBucketBits = 22;
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+bool presort_enabled(const Tree& t, int64_t nq, const QueryOpts& o) {
+    if (o.presort == kPresortOff) return false;
+    if (o.presort == kPresortOn) return nq > 0;
+    return nq >= kPresortMinQueries && t.levels > 1;
+}
+
+int bucket_shift(const Tree& t) { return std::max(0, t.levels - kMaxBucketBits); }
+
+struct PresortLayout {
+    size_t bucket, rank, order, counts, offsets, sums, total;
+    int32_t nb, nsums;
+};
+
+PresortLayout presort_layout(const Tree& t, int64_t nq) {
+    PresortLayout l{};
+    l.nb = t.n_leaves >> bucket_shift(t);
+    l.nsums = (l.nb + kScanChunk - 1) / kScanChunk;
+    size_t off = 0;
+    l.bucket = off; off += align256(sizeof(int32_t) * (size_t)nq);
+    l.rank = off; off += align256(sizeof(int32_t) * (size_t)nq);
+    l.order = off; off += align256(sizeof(int32_t) * (size_t)nq);
+    l.counts = off; off += align256(sizeof(int32_t) * (size_t)l.nb);
+    l.offsets = off; off += align256(sizeof(int32_t) * (size_t)l.nb);
+    l.sums = off; off += align256(sizeof(int32_t) * (size_t)l.nsums);
+    l.total = off;
+    return l;
+}
+
+void check_launch() { TK_HIP_CHECK(hipGetLastError()); }
+
+}  // namespace
+
+size_t query_gpu_workspace(const Tree& t, int64_t nq, const QueryOpts& o) {
+    if (!presort_enabled(t, nq, o)) return 0;
+    return presort_layout(t, nq).total;
+}
+
+void query_gpu(const Tree& t, const void* queries, int64_t nq, int k, void* dist, int32_t* idx, int64_t* orig,
+               const QueryOpts& o, hipStream_t stream, void* workspace, size_t workspace_bytes) {
+    if (!t.dev) throw std::runtime_error("The tree has no GPU copy (it was built for the CPU)");
+    if (k < 1) throw std::runtime_error("nr_nns_searches must be >= 1");
+    if (k > t.n_points) throw std::runtime_error("You requested more nearest neighbors than there are in the KD-Tree");
+    if (k > kMaxGpuK)
+        throw std::runtime_error("nr_nns_searches > " + std::to_string(kMaxGpuK) + " is not supported on the GPU");
+    if (o.tie != kTieReference && o.tie != kTieLowerIndex) throw std::runtime_error("unknown tie policy");
+    if (nq < 0) throw std::runtime_error("negative query count");
+    if (nq == 0) return;
+    if (nq >= (int64_t(1) << 31)) throw std::runtime_error("Only 32 bit signed query indexing implemented");
+    DeviceGuard g(t.dev->device);
+    const int32_t* order = nullptr;
+    std::unique_ptr<StreamScratch> own;
+    if (presort_enabled(t, nq, o)) {
+        ProfScope ps("presort", stream);
+        const PresortLayout l = presort_layout(t, nq);
+        unsigned char* ws = static_cast<unsigned char*>(workspace);
+        if (!ws) {
+            own.reset(new StreamScratch(l.total, stream));
+            ws = static_cast<unsigned char*>(own->ptr);
+        } else if (workspace_bytes < l.total) {
+            throw std::runtime_error("query workspace too small");
+        }
+        int32_t* bucket = reinterpret_cast<int32_t*>(ws + l.bucket);
+        int32_t* rank = reinterpret_cast<int32_t*>(ws + l.rank);
+        int32_t* ord = reinterpret_cast<int32_t*>(ws + l.order);
+        int32_t* counts = reinterpret_cast<int32_t*>(ws + l.counts);
+        int32_t* offsets = reinterpret_cast<int32_t*>(ws + l.offsets);
+        int32_t* sums = reinterpret_cast<int32_t*>(ws + l.sums);
+        hipLaunchKernelGGL(fill_i32_kernel, dim3((l.nb + 255) / 256), dim3(256), 0, stream, counts, (int64_t)l.nb, 0);
+        check_launch();
+        if (t.dtype == kF32)
+            launch_leaf_bucket_f32(t, static_cast<const float*>(queries), nq, bucket_shift(t), bucket, rank, counts,
+                                   stream);
+        else
+            launch_leaf_bucket_f64(t, static_cast<const double*>(queries), nq, bucket_shift(t), bucket, rank, counts,
+                                   stream);
+        hipLaunchKernelGGL(scan_block_sums_kernel, dim3(l.nsums), dim3(kScanBlock), 0, stream, counts, l.nb, sums);
+        check_launch();
+        hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, stream, sums, l.nsums);
+        check_launch();
+        hipLaunchKernelGGL(scan_apply_kernel, dim3(l.nsums), dim3(kScanBlock), 0, stream, counts, l.nb, sums, offsets);
+        check_launch();
+        hipLaunchKernelGGL(bucket_scatter_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, stream, nq, bucket,
+                           rank, offsets, ord);
+        check_launch();
+        order = ord;
+    }
+    ProfScope pt("traverse", stream);
+    if (t.dtype == kF32)
+        launch_tree_search_f32(t, static_cast<const float*>(queries), nq, order, k, static_cast<float*>(dist), idx, orig,
+                               o.tie, stream);
+    else
+        launch_tree_search_f64(t, static_cast<const double*>(queries), nq, order, k, static_cast<double*>(dist), idx,
+                               orig, o.tie, stream);
+}
+
+}  // namespace tk

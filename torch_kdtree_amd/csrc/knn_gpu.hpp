@@ -1,0 +1,98 @@
+// Host-side interface of the GPU subsystem (internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "tree.hpp"
+
+namespace tk {
+
+// Device copy of a tree (the reference's copyPartitionToGPU, src/kdtree_g.cu:63-98, with a
+// different layout: split values in heap order, one record per point, leaf offsets).
+struct DeviceTree {
+    int device = -1;
+    void* split = nullptr;            // T[n_nodes]
+    unsigned char* recs = nullptr;    // records, RecLayout<T, dims>::kBytes each
+    int32_t* leaf_off = nullptr;      // [n_leaves + 1]
+    int32_t* shuffled = nullptr;      // [n_points]
+    int rec_bytes = 0;
+};
+
+int rec_bytes_for(int dtype, int dims);
+void upload_tree(Tree& t, int device);
+void free_device_tree(Tree& t) noexcept;
+
+enum TiePolicy : int { kTieReference = 0, kTieLowerIndex = 1 };
+enum PresortMode : int { kPresortAuto = -1, kPresortOff = 0, kPresortOn = 1 };
+
+// Largest k served on the GPU (the reference's limit, src/kdtree_g.cu:283,421-422).
+constexpr int kMaxGpuK = 128;
+
+struct QueryOpts {
+    int tie = kTieReference;
+    int presort = kPresortAuto;
+};
+
+// Workspace (bytes) the query needs for the presort; 0 when no presort runs.
+size_t query_gpu_workspace(const Tree& t, int64_t nq, const QueryOpts& o);
+
+// dist [nq*k] T, idx [nq*k] int32 structured indices, orig [nq*k] int64 original indices
+// (may be null).  `workspace` may be null: then it is allocated stream-ordered.
+void query_gpu(const Tree& t, const void* queries, int64_t nq, int k, void* dist, int32_t* idx, int64_t* orig,
+               const QueryOpts& o, hipStream_t stream, void* workspace, size_t workspace_bytes);
+
+// Per-dtype kernel launchers (knn_tree_f32.hip / knn_tree_f64.hip).
+void launch_tree_search_f32(const Tree& t, const float* q, int64_t nq, const int32_t* order, int k, float* dist,
+                            int32_t* idx, int64_t* orig, int tie, hipStream_t s);
+void launch_tree_search_f64(const Tree& t, const double* q, int64_t nq, const int32_t* order, int k, double* dist,
+                            int32_t* idx, int64_t* orig, int tie, hipStream_t s);
+void launch_leaf_bucket_f32(const Tree& t, const float* q, int64_t nq, int shift, int32_t* bucket, int32_t* rank,
+                            int32_t* counts, hipStream_t s);
+void launch_leaf_bucket_f64(const Tree& t, const double* q, int64_t nq, int shift, int32_t* bucket, int32_t* rank,
+                            int32_t* counts, hipStream_t s);
+
+// Brute force (bruteforce.hip).
+void bruteforce_gpu(const void* ref, int32_t n, const void* queries, int64_t nq, int dims, int dtype, int k,
+                    void* dist, int32_t* idx, hipStream_t stream, void* workspace, size_t workspace_bytes);
+size_t bruteforce_gpu_workspace(int32_t n, int64_t nq, int dims, int dtype, int k);
+
+// Backward (backward.hip).
+void knn_backward_gpu(const void* queries, const void* ref, int64_t nq, int32_t n, int dims, int dtype, int k,
+                      const int64_t* idx, const void* grad_dist, void* grad_q, void* grad_ref, hipStream_t stream,
+                      void* workspace, size_t workspace_bytes);
+size_t knn_backward_workspace(int64_t nq, int32_t n, int k);
+
+// RAII stream-ordered scratch allocation used when the caller passes no workspace.
+struct StreamScratch {
+    void* ptr = nullptr;
+    hipStream_t stream = nullptr;
+    StreamScratch(size_t bytes, hipStream_t s);
+    ~StreamScratch();
+    StreamScratch(const StreamScratch&) = delete;
+    StreamScratch& operator=(const StreamScratch&) = delete;
+};
+
+// Optional HIP-event timing of the kernels a query launches (off by default).  When on, each
+// stage ("presort", "traverse", "bruteforce", "backward") is bracketed by two events on the
+// launch stream; prof_read synchronises those events and returns the summed duration.
+void prof_enable(bool on);
+void prof_reset();
+bool prof_read(const char* stage, double* total_ms, int64_t* launches);
+struct ProfScope {
+    const char* stage = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t start = nullptr;
+    ProfScope(const char* stage, hipStream_t s);
+    ~ProfScope();
+};
+
+// Makes the tree's device current for the lifetime of the guard.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int device);
+    ~DeviceGuard();
+};
+
+}  // namespace tk

@@ -1,0 +1,16 @@
+// float64 instantiations of the traversal and presort kernels.
+#include "knn_tree_launch.hpp"
+
+namespace tk {
+
+void launch_tree_search_f64(const Tree& t, const double* q, int64_t nq, const int32_t* order, int k, double* dist,
+                            int32_t* idx, int64_t* orig, int tie, hipStream_t s) {
+    launch_tree_search_t<double>(t, q, nq, order, k, dist, idx, orig, tie, s);
+}
+
+void launch_leaf_bucket_f64(const Tree& t, const double* q, int64_t nq, int shift, int32_t* bucket, int32_t* rank,
+                            int32_t* counts, hipStream_t s) {
+    launch_leaf_bucket_t<double>(t, q, nq, shift, bucket, rank, counts, s);
+}
+
+}  // namespace tk

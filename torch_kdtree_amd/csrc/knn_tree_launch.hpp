@@ -1,0 +1,90 @@
+// Launch helpers for the traversal kernels, instantiated once per dtype
+// (knn_tree_f32.hip, knn_tree_f64.hip) so the two compile in parallel.
+#pragma once
+
+#include "knn_gpu.hpp"
+#include "knn_tree.hpp"
+
+namespace tk {
+
+template <typename T>
+TreeView<T> tree_view(const Tree& t) {
+    return TreeView<T>{static_cast<const T*>(t.dev->split), t.dev->recs, t.dev->leaf_off, t.dev->shuffled, t.levels,
+                       t.n_nodes};
+}
+
+template <typename T, int D, bool kLower>
+void launch_tree_td(const TreeView<T>& tv, const T* q, int64_t nq, const int32_t* order, int k, T* dist, int32_t* idx,
+                    int64_t* orig, hipStream_t s) {
+    const dim3 grid((unsigned)((nq + kTreeBlock - 1) / kTreeBlock));
+    const dim3 block(kTreeBlock);
+    int kb = 1;
+    while (kb < k) kb <<= 1;
+#define TK_LAUNCH_K(KK)                                                                                        \
+    case KK:                                                                                                   \
+        hipLaunchKernelGGL((knn_tree_kernel<T, D, KK, kLower>), grid, block, 0, s, tv, q, nq, order, k, dist, \
+                           idx, orig);                                                                         \
+        break;
+    switch (kb) {
+        TK_LAUNCH_K(1)
+        TK_LAUNCH_K(2)
+        TK_LAUNCH_K(4)
+        TK_LAUNCH_K(8)
+        TK_LAUNCH_K(16)
+        TK_LAUNCH_K(32)
+        default: {
+            const size_t shmem = lds_list_bytes(k, sizeof(T), kLower);
+            const void* fn = reinterpret_cast<const void*>(&knn_tree_lds_kernel<T, D, kLower>);
+            if (shmem > 65536)
+                TK_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+            hipLaunchKernelGGL((knn_tree_lds_kernel<T, D, kLower>), dim3((unsigned)((nq + kLdsBlock - 1) / kLdsBlock)),
+                               dim3(kLdsBlock), shmem, s, tv, q, nq, order, k, dist, idx, orig);
+        }
+    }
+#undef TK_LAUNCH_K
+    TK_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_tree_search_t(const Tree& t, const T* q, int64_t nq, const int32_t* order, int k, T* dist, int32_t* idx,
+                          int64_t* orig, int tie, hipStream_t s) {
+    const TreeView<T> tv = tree_view<T>(t);
+#define TK_DIMS(DD)                                                            \
+    case DD:                                                                   \
+        if (tie == kTieLowerIndex)                                             \
+            launch_tree_td<T, DD, true>(tv, q, nq, order, k, dist, idx, orig, s); \
+        else                                                                   \
+            launch_tree_td<T, DD, false>(tv, q, nq, order, k, dist, idx, orig, s); \
+        break;
+    switch (t.dims) {
+        TK_DIMS(1)
+        TK_DIMS(2)
+        TK_DIMS(3)
+        TK_DIMS(8)
+        default: throw std::runtime_error("unsupported dims");
+    }
+#undef TK_DIMS
+}
+
+template <typename T>
+void launch_leaf_bucket_t(const Tree& t, const T* q, int64_t nq, int shift, int32_t* bucket, int32_t* rank,
+                          int32_t* counts, hipStream_t s) {
+    const TreeView<T> tv = tree_view<T>(t);
+    const dim3 grid((unsigned)((nq + kPresortBlock - 1) / kPresortBlock));
+#define TK_DIMS(DD)                                                                                              \
+    case DD:                                                                                                     \
+        hipLaunchKernelGGL((leaf_bucket_kernel<T, DD>), grid, dim3(kPresortBlock), 0, s, tv, q, nq, shift, bucket, \
+                           rank, counts);                                                                        \
+        break;
+    switch (t.dims) {
+        TK_DIMS(1)
+        TK_DIMS(2)
+        TK_DIMS(3)
+        TK_DIMS(8)
+        default: throw std::runtime_error("unsupported dims");
+    }
+#undef TK_DIMS
+    TK_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace tk

@@ -1,0 +1,215 @@
+// pybind11 module `torch_knn`: the reference's binding surface (src/interface.cpp:13-150)
+// rebuilt over this library's C ABI (include/torch_kdtree_amd.h).
+//
+// Classes KDTree{CPU,GPU}{1,2,3,8}D (float64) and KDTree{CPU,GPU}{1,2,3,8}DF (float32), with
+// the reference's methods:
+//   __init__(points_ref, levels)            src/interface.cpp:25-43
+//   get_shuffled_inds() -> int32[N]          src/interface.cpp:45-50
+//   get_structured_points() -> T[N, d]       src/interface.cpp:52-58
+//   query(points_query_ptr, nr_query_points, nr_nns_searches, dist_arr_ptr, knn_idx_ptr)
+//                                            src/interface.cpp:79-86
+// plus keyword-only extensions that default to the reference's behaviour: the constructor's
+// `device`, and query's `orig_idx_ptr`, `tie`, `presort`, `stream`, `n_threads`.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/torch_kdtree_amd.h"
+
+namespace py = pybind11;
+
+namespace {
+
+void check(int rc) {
+    if (rc != 0) throw std::runtime_error(tk_last_error());
+}
+
+template <typename T>
+constexpr int dtype_code() {
+    return std::is_same<T, float>::value ? TK_F32 : TK_F64;
+}
+
+template <typename T, int D, bool kGpu>
+struct KDTree {
+    tk_tree* tree = nullptr;
+    int levels = 0;
+
+    KDTree(const py::array_t<T, py::array::c_style | py::array::forcecast>& points_ref, int levels_, int device) {
+        if (points_ref.ndim() != 2 || points_ref.shape(1) != D)
+            throw std::runtime_error("points_ref must have shape (N, " + std::to_string(D) + ")");
+        const T* p = points_ref.data();
+        const int64_t n = points_ref.shape(0);
+        levels = levels_;
+        int rc;
+        {
+            py::gil_scoped_release release;
+            rc = tk_tree_build(p, n, D, dtype_code<T>(), levels_, 0, &tree);
+            if (rc == 0 && kGpu) rc = tk_tree_upload(tree, device);
+        }
+        if (rc != 0) {
+            const std::string msg = tk_last_error();
+            tk_tree_free(tree);
+            tree = nullptr;
+            throw std::runtime_error(msg);
+        }
+    }
+    ~KDTree() { tk_tree_free(tree); }
+
+    int32_t n_points() const {
+        int32_t info[5];
+        check(tk_tree_info(tree, info));
+        return info[3];
+    }
+
+    py::array_t<int32_t> get_shuffled_inds() const {
+        py::array_t<int32_t> out(n_points());
+        check(tk_tree_get_shuffled_inds(tree, out.mutable_data()));
+        return out;
+    }
+
+    py::array_t<T> get_structured_points() const {
+        py::array_t<T> out(std::vector<py::ssize_t>{n_points(), D});
+        check(tk_tree_get_structured_points(tree, out.mutable_data()));
+        return out;
+    }
+
+    py::tuple get_nodes() const {
+        int32_t info[5];
+        check(tk_tree_info(tree, info));
+        const int L = info[2];
+        const py::ssize_t nodes = (py::ssize_t(1) << L) - 1;
+        py::array_t<int32_t> axis(nodes);
+        py::array_t<T> split(nodes);
+        py::array_t<int32_t> leaf_off((py::ssize_t(1) << L) + 1);
+        check(tk_tree_get_nodes(tree, axis.mutable_data(), split.mutable_data(), leaf_off.mutable_data()));
+        return py::make_tuple(axis, split, leaf_off);
+    }
+
+    void query(size_t points_query_ptr, size_t nr_query_points, int32_t nr_nns_searches, size_t dist_arr_ptr,
+               size_t knn_idx_ptr, size_t orig_idx_ptr, int tie, int presort, size_t stream, int n_threads) {
+        int rc;
+        {
+            py::gil_scoped_release release;
+            if (kGpu)
+                rc = tk_knn_query_gpu(tree, reinterpret_cast<const void*>(points_query_ptr), (int64_t)nr_query_points,
+                                      nr_nns_searches, reinterpret_cast<void*>(dist_arr_ptr),
+                                      reinterpret_cast<int32_t*>(knn_idx_ptr), reinterpret_cast<int64_t*>(orig_idx_ptr),
+                                      tie, presort, nullptr, 0, reinterpret_cast<void*>(stream));
+            else
+                rc = tk_knn_query_cpu(tree, reinterpret_cast<const void*>(points_query_ptr), (int64_t)nr_query_points,
+                                      nr_nns_searches, reinterpret_cast<void*>(dist_arr_ptr),
+                                      reinterpret_cast<int32_t*>(knn_idx_ptr), reinterpret_cast<int64_t*>(orig_idx_ptr),
+                                      n_threads);
+        }
+        check(rc);
+    }
+};
+
+template <typename T, int D, bool kGpu>
+void bind_tree(py::module_& m, const char* name) {
+    using C = KDTree<T, D, kGpu>;
+    py::class_<C, std::shared_ptr<C>>(m, name)
+        .def(py::init<const py::array_t<T, py::array::c_style | py::array::forcecast>&, int, int>(),
+             py::arg("points_ref"), py::arg("levels"), py::kw_only(), py::arg("device") = -1)
+        .def("get_shuffled_inds", &C::get_shuffled_inds,
+             "Returns the shuffled indices to translate from local to global indices")
+        .def("get_structured_points", &C::get_structured_points,
+             "Returns the ordered points how they are used in the KD-Tree")
+        .def("get_nodes", &C::get_nodes, "Returns (axis[int32], split[T], leaf_offsets[int32]) of the tree")
+        .def("query", &C::query, py::arg("points_query_ptr"), py::arg("nr_query_points"), py::arg("nr_nns_searches"),
+             py::arg("dist_arr_ptr"), py::arg("knn_idx_ptr"), py::kw_only(), py::arg("orig_idx_ptr") = 0,
+             py::arg("tie") = 0, py::arg("presort") = -1, py::arg("stream") = 0, py::arg("n_threads") = 0,
+             "Queries the KNN from the KD-Tree and puts the results in the array pointed to by dist_arr_ptr and "
+             "knn_idx_ptr")
+        .def_readonly("levels", &C::levels);
+}
+
+int dtype_from_str(const std::string& s) {
+    if (s == "float32") return TK_F32;
+    if (s == "float64") return TK_F64;
+    throw std::runtime_error("unsupported dtype " + s);
+}
+
+}  // namespace
+
+PYBIND11_MODULE(torch_knn, m) {
+    m.doc() = "KD-tree KNN for AMD Instinct MI355X (gfx950): pybind11 layer over the torch_kdtree_amd C ABI";
+
+    bind_tree<float, 3, false>(m, "KDTreeCPU3DF");
+    bind_tree<double, 3, false>(m, "KDTreeCPU3D");
+    bind_tree<float, 3, true>(m, "KDTreeGPU3DF");
+    bind_tree<double, 3, true>(m, "KDTreeGPU3D");
+    bind_tree<float, 2, false>(m, "KDTreeCPU2DF");
+    bind_tree<double, 2, false>(m, "KDTreeCPU2D");
+    bind_tree<float, 2, true>(m, "KDTreeGPU2DF");
+    bind_tree<double, 2, true>(m, "KDTreeGPU2D");
+    bind_tree<float, 1, false>(m, "KDTreeCPU1DF");
+    bind_tree<double, 1, false>(m, "KDTreeCPU1D");
+    bind_tree<float, 1, true>(m, "KDTreeGPU1DF");
+    bind_tree<double, 1, true>(m, "KDTreeGPU1D");
+    bind_tree<float, 8, false>(m, "KDTreeCPU8DF");
+    bind_tree<double, 8, false>(m, "KDTreeCPU8D");
+    bind_tree<float, 8, true>(m, "KDTreeGPU8DF");
+    bind_tree<double, 8, true>(m, "KDTreeGPU8D");
+
+    m.def("check_for_gpu", [] { return tk_gpu_available() != 0; }, "Check if the library was compiled with GPU support");
+    m.def("device_count", [] {
+        int c = 0;
+        check(tk_device_count(&c));
+        return c;
+    });
+
+    m.def(
+        "knn_bruteforce",
+        [](size_t ref_ptr, int32_t n_ref, size_t q_ptr, int64_t n_q, int dims, const std::string& dtype, int k,
+           size_t dist_ptr, size_t idx_ptr, size_t stream) {
+            int rc;
+            {
+                py::gil_scoped_release release;
+                rc = tk_knn_bruteforce_gpu(reinterpret_cast<const void*>(ref_ptr), n_ref,
+                                           reinterpret_cast<const void*>(q_ptr), n_q, dims, dtype_from_str(dtype), k,
+                                           reinterpret_cast<void*>(dist_ptr), reinterpret_cast<int32_t*>(idx_ptr),
+                                           nullptr, 0, reinterpret_cast<void*>(stream));
+            }
+            check(rc);
+        },
+        py::arg("ref_ptr"), py::arg("n_ref"), py::arg("queries_ptr"), py::arg("n_queries"), py::arg("dims"),
+        py::arg("dtype"), py::arg("k"), py::arg("dist_ptr"), py::arg("idx_ptr"), py::arg("stream") = 0);
+
+    m.def(
+        "knn_backward",
+        [](size_t q_ptr, size_t ref_ptr, int64_t n_q, int32_t n_ref, int dims, const std::string& dtype, int k,
+           size_t idx_ptr, size_t grad_ptr, size_t grad_q_ptr, size_t grad_ref_ptr, size_t stream) {
+            int rc;
+            {
+                py::gil_scoped_release release;
+                rc = tk_knn_backward_gpu(reinterpret_cast<const void*>(q_ptr), reinterpret_cast<const void*>(ref_ptr),
+                                         n_q, n_ref, dims, dtype_from_str(dtype), k,
+                                         reinterpret_cast<const int64_t*>(idx_ptr),
+                                         reinterpret_cast<const void*>(grad_ptr), reinterpret_cast<void*>(grad_q_ptr),
+                                         reinterpret_cast<void*>(grad_ref_ptr), nullptr, 0,
+                                         reinterpret_cast<void*>(stream));
+            }
+            check(rc);
+        },
+        py::arg("queries_ptr"), py::arg("ref_ptr"), py::arg("n_queries"), py::arg("n_ref"), py::arg("dims"),
+        py::arg("dtype"), py::arg("k"), py::arg("idx_ptr"), py::arg("grad_dist_ptr"), py::arg("grad_queries_ptr"),
+        py::arg("grad_ref_ptr"), py::arg("stream") = 0);
+
+    m.def("profile_enable", [](bool on) { check(tk_profile_enable(on ? 1 : 0)); }, py::arg("on"));
+    m.def("profile_reset", [] { check(tk_profile_reset()); });
+    m.def(
+        "profile_read",
+        [](const std::string& stage) {
+            double ms = 0;
+            int64_t calls = 0;
+            check(tk_profile_read(stage.c_str(), &ms, &calls));
+            return py::make_tuple(ms, calls);
+        },
+        py::arg("stage"), "Summed kernel milliseconds and call count of a stage (waits for its events)");
+
+    m.attr("__version__") = tk_version();
+}

@@ -1,0 +1,64 @@
+// Host-side KD-tree object shared by the CPU build, the CPU search and the GPU upload.
+//
+// Layout (identical to the reference's flat layout, src/kdtree.hpp:126-206):
+//   * nodes in heap order: node i has children 2i+1 and 2i+2; a tree of `levels` levels has
+//     2^levels - 1 nodes and 2^levels leaves (src/kdtree.hpp:73-93);
+//   * the split axis of a node at depth l is l mod dims (root axis 0, src/kdtree.hpp:473-477),
+//     its split value is the median of src/kdtree.hpp:392-401;
+//   * leaf j of leaf-parent p is 2p+1-(2^levels-1) (left) and +1 (right), src/kdtree.hpp:300-303;
+//   * leaves are contiguous runs of the structured point array; leaf_off[j] .. leaf_off[j+1].
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace tk {
+
+enum DType : int { kF32 = 0, kF64 = 1 };
+
+inline size_t dtype_size(int dtype) { return dtype == kF64 ? 8 : 4; }
+
+inline int32_t total_nodes(int levels) { return (int32_t)((1u << levels) - 1u); }
+
+struct DeviceTree;  // defined by the GPU translation unit
+
+struct Tree {
+    int dtype = kF32;
+    int dims = 0;
+    int levels = 0;
+    int32_t n_points = 0;
+    int32_t n_nodes = 0;
+    int32_t n_leaves = 0;
+    std::vector<uint8_t> axis;          // [n_nodes]
+    std::vector<unsigned char> split;   // [n_nodes] values of type T
+    std::vector<int32_t> leaf_off;      // [n_leaves + 1]
+    std::vector<unsigned char> points;  // [n_points * dims] values of type T, structured order
+    std::vector<int32_t> shuffled;      // [n_points] original index of each structured position
+    DeviceTree* dev = nullptr;          // GPU copy, owned
+
+    template <typename T> T* split_ptr() { return reinterpret_cast<T*>(split.data()); }
+    template <typename T> const T* split_ptr() const { return reinterpret_cast<const T*>(split.data()); }
+    template <typename T> T* points_ptr() { return reinterpret_cast<T*>(points.data()); }
+    template <typename T> const T* points_ptr() const { return reinterpret_cast<const T*>(points.data()); }
+};
+
+// Largest tree depth accepted by the build (the reference caps `levels` at 13 in Python,
+// nn_distance.py:155, because of its GPU kernel's LDS tag array; this build has no tag array).
+constexpr int kMaxLevels = 24;
+
+// Dimensions this library is instantiated for (the reference: 1, 2, 3; README.md:83-95
+// describes adding more; 8 is added here).
+inline bool dims_supported(int d) { return d == 1 || d == 2 || d == 3 || d == 8; }
+
+// Builds the tree on the host.  Throws std::runtime_error with the reference's messages.
+Tree* build_tree(const void* points, int64_t n_points, int dims, int levels, int dtype, int n_threads);
+
+// CPU KNN search with the reference's semantics (iterative near-first DFS, upper-bound
+// insertion; src/kdtree.cpp:16-194).
+void query_cpu(const Tree& t, const void* queries, int64_t n_queries, int k, void* dist, int32_t* idx,
+               int64_t* orig_idx, int n_threads);
+
+}  // namespace tk

@@ -1,0 +1,266 @@
+"""Python surface of the MI355X KD-tree: ``build_kd_tree`` and ``TorchKDTree.query``.
+
+Mirrors ``torch_kdtree/nn_distance.py`` of thomgrand/torch_kdtree (same names, argument
+meaning, return values and error behaviour) over this package's own ``torch_knn`` module.
+
+Differences, all opt-in or fixes of reference defects:
+
+* GPU queries write the original ``int64`` indices from the kernel (the reference gathers
+  ``shuffled_ind[result_idx.long()]`` afterwards in torch, ``nn_distance.py:109``).
+* With autograd, the GPU path returns the kernel's own squared distances through a
+  ``torch.autograd.Function`` whose backward is a HIP kernel; the reference recomputes the
+  distances with torch ops (``nn_distance.py:111-112``).  CPU trees keep that recomputation.
+* ``levels`` may go up to 24 (reference: 13, ``nn_distance.py:155``); ``levels=None`` keeps
+  the reference's default (``nn_distance.py:146-147``) and ``levels="auto"`` picks a leaf
+  size tuned for the GPU.
+* ``tie_break`` ("reference" or "lower_index") selects how exact distance ties are ordered on
+  the GPU; "reference" (default) reproduces the reference CPU search exactly.
+* Non-contiguous ``result_dists``/``result_idx`` buffers receive the results (the reference
+  silently writes to a contiguous copy, ``nn_distance.py:93-97``).
+"""
+from __future__ import annotations
+
+from typing import Union
+
+import numpy as np
+import torch
+
+from . import torch_knn
+from . import autotune
+
+gpu_available = torch_knn.check_for_gpu()
+if not gpu_available:
+    print("The library was not successfully compiled using CUDA. Only the CPU version will be available.")
+
+_transl_torch_device = {"cpu": "CPU", "cuda": "GPU"}
+_TIE_POLICIES = {"reference": 0, "lower_index": 1}
+_PRESORT = {"auto": -1, "off": 0, "on": 1, None: -1, False: 0, True: 1}
+MAX_LEVELS = 24
+
+
+def default_levels(nr_points: int) -> int:
+    """The reference's default depth, ``nn_distance.py:146-147``."""
+    return int(np.maximum(1, np.minimum(13, int(np.log(int(nr_points)) / np.log(2)) - 3)))
+
+
+def _dtype_name(dtype: torch.dtype) -> str:
+    return {torch.float32: "float32", torch.float64: "float64"}[dtype]
+
+
+def _stream_handle(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+class _KnnSquaredDistances(torch.autograd.Function):
+    """Squared KNN distances with a HIP backward (``tk_knn_backward_gpu``).
+
+    forward returns the distances the query kernel computed; backward recomputes the
+    neighbour differences from the coordinates and returns
+    dL/dq_i = sum_j 2 g_ij (q_i - r_idx_ij) and dL/dr_m = -sum_{idx_ij = m} 2 g_ij (q_i - r_m),
+    the gradient of ``nn_distance.py:112`` of the reference.
+    """
+
+    @staticmethod
+    def forward(ctx, points_query, points_ref, inds, dists):
+        ctx.save_for_backward(points_query, points_ref, inds)
+        return dists.clone()
+
+    @staticmethod
+    def backward(ctx, grad):
+        points_query, points_ref, inds = ctx.saved_tensors
+        need_q, need_r = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
+        if not (need_q or need_r):
+            return None, None, None, None
+        q = points_query.detach().contiguous()
+        r = points_ref.detach().contiguous()
+        g = grad.contiguous()
+        inds = inds.contiguous()
+        dq = torch.empty_like(q) if need_q else None
+        dr = torch.empty_like(r) if need_r else None
+        torch_knn.knn_backward(
+            q.data_ptr(), r.data_ptr(), q.shape[0], r.shape[0], q.shape[1], _dtype_name(q.dtype), inds.shape[1],
+            inds.data_ptr(), g.data_ptr(), dq.data_ptr() if need_q else 0, dr.data_ptr() if need_r else 0,
+            _stream_handle(q.device))
+        return dq, dr, None, None
+
+
+class TorchKDTree:
+    def __init__(self, points_ref: torch.Tensor, device: torch.device, levels: int, squared_distances: bool,
+                 tie_break: str = "reference", presort="auto"):
+        """Builds the KDTree. See :ref:`build_kd_tree` for more details."""
+        assert device.type in ['cpu', 'cuda']
+        assert points_ref.shape[0] < 2**31, "Only 32 bit signed indexing implemented"
+
+        self.dtype = points_ref.dtype
+        self.dims = points_ref.shape[-1]
+        self.nr_ref_points = points_ref.shape[0]
+        precision = {torch.float32: "F", torch.float64: ""}.get(self.dtype)
+        kdtree_str = ("KDTree" + _transl_torch_device[device.type] + "%dD" % (self.dims) + precision
+                      if precision is not None else None)
+
+        try:
+            if kdtree_str is None:
+                raise AttributeError("unsupported dtype %s" % self.dtype)
+            tree_cls = getattr(torch_knn, kdtree_str)
+        except AttributeError as err:
+            raise RuntimeError("Could not find the KD-Tree for your specified options. This probably means the library "
+                               "was not compiled for the specified dimensionality, precision or the targeted device. "
+                               "Original error:", err)
+
+        kwargs = {}
+        if device.type == "cuda":
+            kwargs["device"] = device.index if device.index is not None else torch.cuda.current_device()
+            device = torch.device("cuda", kwargs["device"])
+        self.kdtree = tree_cls(points_ref.detach().cpu().numpy(), levels, **kwargs)
+        self.levels = levels
+
+        self.structured_points = torch.from_numpy(self.kdtree.get_structured_points())
+        self.shuffled_ind = torch.from_numpy(self.kdtree.get_shuffled_inds()).long()
+        self.use_gpu = use_gpu = (device.type == 'cuda')
+        self.device = device
+        self.dtype_idx = torch.int32  # Restriction in the compiled library
+        self.ref_requires_grad = points_ref.requires_grad
+        self.points_ref_bak = points_ref
+        self.squared_distances = squared_distances
+        if tie_break not in _TIE_POLICIES:
+            raise ValueError("tie_break must be one of %s" % sorted(_TIE_POLICIES))
+        self.tie_break = tie_break
+        if presort not in _PRESORT:
+            raise ValueError("presort must be 'auto', 'on' or 'off'")
+        self.presort = presort
+
+        if self.use_gpu:
+            self.structured_points = self.structured_points.to(self.device)
+            self.shuffled_ind = self.shuffled_ind.to(self.device)
+
+    def query(self, points_query: torch.Tensor, nr_nns_searches: int = 1,
+              result_dists: torch.Tensor = None, result_idx: torch.Tensor = None):
+        """Searches the specified KD-Tree for KNN of the given points
+
+        Parameters
+        ----------
+        points_query : torch.Tensor of float or double precision
+            Points for which the KNNs will be computed
+        nr_nns_searches : int, optional
+            How many closest nearest neighbors will be queried (=k), by default 1
+        result_dists : torch.Tensor of float or double precision, optional
+            Target array that will hold the resulting distance. If not specified, this will be dynamically created.
+        result_idx : torch.Tensor of dtype_idx type, optional
+            Target array that will hold the resulting KNN indices (structured order, as in the reference).
+            If not specified, this will be dynamically created.
+
+        Returns
+        -------
+        tuple
+            * dists (tensor of float or double precision) : Quadratic distance of KD-Tree points to the queried points
+            * inds (tensor of int64) : Indices (rows of ``points_ref``) of the K closest neighbors
+
+        Raises
+        ------
+        RuntimeError
+            If more neighbours are requested than the tree holds, or the query fails.
+        """
+        if nr_nns_searches > self.nr_ref_points:
+            raise RuntimeError("You requested more nearest neighbors than there are in the KD-Tree")
+
+        points_query = points_query.to(self.device)
+        nq = points_query.shape[0]
+
+        if result_dists is None:
+            result_dists = torch.empty(size=[nq, nr_nns_searches], dtype=self.dtype, device=self.device)
+        if result_idx is None:
+            result_idx = torch.empty(size=[nq, nr_nns_searches], dtype=self.dtype_idx, device=self.device)
+
+        assert list(result_dists.shape) == [nq, nr_nns_searches]
+        assert result_dists.dtype == self.dtype
+        assert list(result_idx.shape) == [nq, nr_nns_searches]
+        assert result_idx.dtype == self.dtype_idx
+        assert points_query.dtype == self.dtype
+
+        user_dists, user_idx = result_dists, result_idx
+        if not result_dists.is_contiguous():
+            result_dists = result_dists.contiguous()
+        if not result_idx.is_contiguous():
+            result_idx = result_idx.contiguous()
+        if not points_query.is_contiguous():
+            points_query = points_query.contiguous()
+
+        inds = torch.empty(size=[nq, nr_nns_searches], dtype=torch.int64, device=self.device)
+        if self.use_gpu:
+            self.kdtree.query(points_query.data_ptr(), nq, nr_nns_searches, result_dists.data_ptr(),
+                              result_idx.data_ptr(), orig_idx_ptr=inds.data_ptr(),
+                              tie=_TIE_POLICIES[self.tie_break], presort=_PRESORT[self.presort],
+                              stream=_stream_handle(self.device))
+        else:
+            self.kdtree.query(points_query.data_ptr(), nq, nr_nns_searches, result_dists.data_ptr(),
+                              result_idx.data_ptr(), orig_idx_ptr=inds.data_ptr())
+
+        if user_dists is not result_dists:
+            user_dists.copy_(result_dists)
+        if user_idx is not result_idx:
+            user_idx.copy_(result_idx)
+
+        dists = result_dists
+        if (points_query.requires_grad or self.ref_requires_grad) and torch.is_grad_enabled():
+            points_ref = self.points_ref_bak.to(self.device)
+            if self.use_gpu:
+                dists = _KnnSquaredDistances.apply(points_query, points_ref, inds, result_dists)
+            else:
+                dists = torch.sum((points_query[:, None] - points_ref[inds])**2, dim=-1)
+
+        if not self.squared_distances:
+            dists = torch.sqrt(dists)
+
+        return dists, inds
+
+
+def build_kd_tree(points_ref: Union[torch.Tensor, np.ndarray], device: torch.device = None,
+                  squared_distances=True, levels: int = None, tie_break: str = "reference", presort="auto"):
+    """Builds the KD-Tree for subsequent queries.
+
+    The tree is always built on the CPU and then transferred to the GPU if necessary.
+
+    Parameters
+    ----------
+    points_ref : torch.Tensor or np.ndarray
+        Points from which to build the KD-Tree, shape (N, d), d in {1, 2, 3, 8}
+    device : torch.device
+        Target torch device of the KD-Tree; ``points_ref.device`` if not specified.
+    squared_distances : bool
+        If true, the squared euclidean distances will be returned, by default True.
+    levels : int, "auto" or None
+        Levels of the KD-Tree (1 to 24).  None picks the reference's default; "auto" picks a leaf
+        size tuned for the GPU path.
+    tie_break : str
+        "reference" (default): exact distance ties in the reference CPU search's order.
+        "lower_index": ties ordered by the lower original index.
+    presort : "auto", "on" or "off"
+        Whether GPU queries are grouped by leaf before the traversal.
+
+    Returns
+    -------
+    TorchKDTree
+    """
+    if issubclass(type(points_ref), np.ndarray):
+        points_ref = torch.from_numpy(points_ref)
+
+    if device is None:
+        device = points_ref.device
+
+    if issubclass(type(device), str):
+        device = torch.device(device)
+
+    if levels is None:
+        levels = default_levels(points_ref.shape[0])
+    elif isinstance(levels, str):
+        if levels != "auto":
+            raise ValueError("levels must be an int, None or 'auto'")
+        levels = autotune.pick_levels(int(points_ref.shape[0]), int(points_ref.shape[-1]), device)
+
+    assert levels >= 1 and levels <= MAX_LEVELS
+    assert issubclass(type(points_ref), torch.Tensor)
+    assert device.type != 'cuda' or gpu_available, \
+        "You requested the KD-Tree on the GPU, but the library was compiled with CPU support only"
+    assert device.type in ['cuda', 'cpu']
+
+    return TorchKDTree(points_ref, device, levels, squared_distances, tie_break=tie_break, presort=presort)
