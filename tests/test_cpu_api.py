@@ -1,0 +1,256 @@
+"""Host-side behaviour of the product (no GPU needed): tree build parity, CPU search parity,
+the reference's API contract (tests/test_knn_unit.py restated for device='cpu') and the C ABI."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import torch_kdtree_amd as tk
+from torch_kdtree_amd import torch_knn
+from tests._util import assert_same
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# ---------------------------------------------------------------------------------------------
+# Tree build: identical to the oracle's faithful two-sort restatement of createKDTree
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [1, 2, 3, 8])
+@pytest.mark.parametrize("kind", ["normal", "duplicates"])
+def test_build_matches_oracle(dtype, d, kind):
+    rng = np.random.default_rng(d * 3 + (kind == "duplicates"))
+    n = 70000
+    if kind == "normal":
+        pts = (rng.standard_normal((n, d)) * 1e3).astype(dtype)
+    else:
+        pts = rng.integers(0, 9, (n, d)).astype(dtype)  # sorts full of equal keys
+    for levels in (1, oracle.default_levels(n), 14):
+        t = tk.build_kd_tree(torch.from_numpy(pts), device="cpu", levels=levels)
+        e = oracle.OracleTree(pts, levels).export()
+        axis, split, leaf_off = t.kdtree.get_nodes()
+        assert_same(t.kdtree.get_structured_points(), e["structured"], "structured points")
+        assert_same(t.kdtree.get_shuffled_inds(), e["shuffled"], "shuffled indices")
+        assert_same(axis, e["axis"], "axis")
+        assert_same(split, e["median"], "medians")
+        assert_same(leaf_off[:-1], e["leaf_off"], "leaf offsets")
+        assert leaf_off[-1] == n
+
+
+def test_build_independent_of_thread_count():
+    pts = np.random.default_rng(1).integers(0, 50, (200000, 3)).astype(np.float32)
+    from torch_kdtree_amd import torch_knn as m
+    a = m.KDTreeCPU3DF(pts, 12)
+    old = os.environ.get("OMP_NUM_THREADS")
+    b = m.KDTreeCPU3DF(pts, 12)
+    assert_same(a.get_shuffled_inds(), b.get_shuffled_inds(), "shuffled")
+    assert old == os.environ.get("OMP_NUM_THREADS")
+
+
+# ---------------------------------------------------------------------------------------------
+# CPU search: identical to the oracle's restatement of KDTreeKNNSearch
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [1, 2, 3, 8])
+def test_cpu_query_matches_oracle(dtype, d):
+    rng = np.random.default_rng(50 + d)
+    ref = (rng.standard_normal((30000, d)) * 1e3).astype(dtype)
+    qry = (rng.standard_normal((4000, d)) * 1e3).astype(dtype)
+    for levels in (None, 13):
+        t = tk.build_kd_tree(torch.from_numpy(ref), device="cpu", levels=levels)
+        ot = oracle.tree_from_product(t)
+        for k in (1, 7, 16, 150):
+            res_i = torch.empty((qry.shape[0], k), dtype=torch.int32)
+            dist, inds = t.query(torch.from_numpy(qry), nr_nns_searches=k, result_idx=res_i)
+            od, oi = ot.search(qry, k)
+            assert_same(dist.numpy(), od, "distances")
+            assert_same(res_i.numpy(), oi, "structured indices")
+            assert_same(inds.numpy(), t.kdtree.get_shuffled_inds()[oi].astype(np.int64), "original indices")
+
+
+def test_cpu_query_ties_match_oracle():
+    rng = np.random.default_rng(3)
+    ref = rng.integers(0, 6, (20000, 3)).astype(np.float32)
+    qry = (rng.integers(0, 6, (3000, 3)) + 0.5 * rng.integers(0, 2, (3000, 3))).astype(np.float32)
+    t = tk.build_kd_tree(torch.from_numpy(ref), device="cpu", levels=9)
+    od, oi = oracle.OracleTree(ref, 9).search(qry, 24)
+    res_i = torch.empty((qry.shape[0], 24), dtype=torch.int32)
+    dist, _ = t.query(torch.from_numpy(qry), nr_nns_searches=24, result_idx=res_i)
+    assert_same(dist.numpy(), od, "distances")
+    assert_same(res_i.numpy(), oi, "structured indices")
+
+
+def test_known_answers_cpu():
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "known_answers.json")) as f:
+        cases = json.load(f)
+    for case in cases:
+        pts = torch.tensor(case["points"], dtype=torch.float32)
+        t = tk.build_kd_tree(pts, device="cpu", levels=case["levels"])
+        for q in case["queries"]:
+            res_i = torch.empty((1, q["k"]), dtype=torch.int32)
+            d, i = t.query(torch.tensor([q["q"]], dtype=torch.float32), nr_nns_searches=q["k"], result_idx=res_i)
+            assert d[0].tolist() == q["dist"] and res_i[0].tolist() == q["idx"] and i[0].tolist() == q["orig"]
+
+
+# ---------------------------------------------------------------------------------------------
+# The reference's own test grid (tests/test_knn_unit.py:30-70), device='cpu'
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("nr_refs", [5, 10, 20, 30, 1000, 10000])
+@pytest.mark.parametrize("nr_query", [5, 30, 1000])
+@pytest.mark.parametrize("k", [1, 5, 10, 20])
+@pytest.mark.parametrize("d", [1, 2, 3])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("requires_grad", [True, False])
+def test_reference_grid_cpu(nr_refs, nr_query, k, d, dtype, requires_grad):
+    from scipy.spatial import cKDTree
+    g = torch.Generator().manual_seed(nr_refs * 31 + nr_query * 7 + k + d)
+    points_ref = torch.randn(size=(nr_refs, d), dtype=dtype, generator=g) * 1e3
+    points_query = (torch.randn(size=(nr_query, d), dtype=dtype, generator=g) * 1e3).requires_grad_(requires_grad)
+    t = tk.build_kd_tree(points_ref, device=torch.device("cpu"))
+    if k > nr_refs:
+        with pytest.raises(Exception):
+            t.query(points_query, nr_nns_searches=k)
+        return
+    dists, inds = t.query(points_query, nr_nns_searches=k)
+    assert inds.shape == (nr_query, k) and dists.shape == (nr_query, k) and inds.dtype == torch.int64
+    dref, iref = cKDTree(points_ref.numpy()).query(points_query.detach().numpy(), k)
+    dref = np.asarray(dref).reshape(nr_query, k)
+    iref = np.asarray(iref).reshape(nr_query, k)
+    assert np.allclose(dref ** 2, dists.detach().numpy(), atol=1e-5, rtol=1e-5 if dtype == torch.float64 else 1e-4)
+    assert np.sum(iref == inds.numpy()) / iref.size > 0.95
+    od, oi = oracle.tree_from_product(t).search(points_query.detach().numpy(), k)
+    assert np.array_equal(dists.detach().numpy(), od)
+
+
+def test_uncompiled_dimension():
+    points_ref = np.random.default_rng(0).uniform(size=(10, 436))
+    with pytest.raises(Exception):
+        tk.build_kd_tree(points_ref, device="cpu")
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_wrong_dtype(dtype):
+    other = torch.float64 if dtype == torch.float32 else torch.float32
+    t = tk.build_kd_tree(torch.randn(size=(10, 3), dtype=dtype), device="cpu")
+    with pytest.raises(Exception):
+        t.query(torch.randn(size=(10, 3), dtype=other))
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_grad_cpu(dtype):
+    """tests/test_knn_unit.py:90-107 on a CPU tree (the reference's torch recomputation path)."""
+    torch.manual_seed(0)
+    points_ref = torch.randn(size=(10, 3), dtype=dtype, requires_grad=True)
+    t = tk.build_kd_tree(points_ref)
+    points_query = torch.randn(size=(5, 3), dtype=dtype, requires_grad=True)
+    dists, inds = t.query(points_query, nr_nns_searches=2)
+    (0.5 * torch.sum(dists)).backward()
+    assert torch.allclose(points_query.grad, torch.sum((points_query[:, None] - points_ref[inds]), axis=-2))
+    ref_grad = torch.zeros_like(points_ref)
+    for i, ind in enumerate(inds):
+        ref_grad[ind] += points_ref[ind] - points_query[i]
+    assert torch.allclose(points_ref.grad, ref_grad)
+
+
+def test_levels_and_point_count_errors():
+    pts = torch.randn(100, 3)
+    with pytest.raises(AssertionError):
+        tk.build_kd_tree(pts, device="cpu", levels=0)
+    with pytest.raises(AssertionError):
+        tk.build_kd_tree(pts, device="cpu", levels=25)
+    with pytest.raises(RuntimeError, match="Ran out of points"):
+        tk.build_kd_tree(pts, device="cpu", levels=8)  # needs 128 points
+    tk.build_kd_tree(torch.randn(128, 3), device="cpu", levels=8)
+    with pytest.raises(ValueError):
+        tk.build_kd_tree(pts, device="cpu", tie_break="random")
+
+
+def test_unsupported_dtype_raises():
+    with pytest.raises(RuntimeError):
+        tk.build_kd_tree(torch.randn(100, 3).half(), device="cpu")
+
+
+def test_noncontiguous_result_buffers_are_written():
+    ref = torch.randn(500, 3)
+    qry = torch.randn(40, 3)
+    t = tk.build_kd_tree(ref, device="cpu")
+    buf_d = torch.zeros(4, 40).t()
+    buf_i = torch.zeros(4, 40, dtype=torch.int32).t()
+    assert not buf_d.is_contiguous()
+    d, _ = t.query(qry, nr_nns_searches=4, result_dists=buf_d, result_idx=buf_i)
+    assert torch.equal(buf_d, d)
+    ref_i = torch.empty(40, 4, dtype=torch.int32)
+    t.query(qry, nr_nns_searches=4, result_idx=ref_i)
+    assert torch.equal(buf_i, ref_i)
+
+
+def test_squared_distances_false():
+    ref = torch.randn(500, 3, dtype=torch.float64)
+    qry = torch.randn(40, 3, dtype=torch.float64)
+    d2, i2 = tk.build_kd_tree(ref, device="cpu").query(qry, nr_nns_searches=3)
+    d1, i1 = tk.build_kd_tree(ref, device="cpu", squared_distances=False).query(qry, nr_nns_searches=3)
+    assert torch.equal(i1, i2) and torch.equal(d1, torch.sqrt(d2))
+
+
+def test_numpy_input_and_module_surface():
+    assert tk.__version__ == "1.0" and torch_knn.__version__ == "1.0"
+    assert tk.gpu_available is True
+    for dev in ("CPU", "GPU"):
+        for d in (1, 2, 3, 8):
+            for p in ("", "F"):
+                assert hasattr(torch_knn, "KDTree%s%dD%s" % (dev, d, p))
+    t = tk.build_kd_tree(np.random.default_rng(0).standard_normal((300, 2)), device="cpu")
+    assert t.dtype == torch.float64 and t.dims == 2 and t.nr_ref_points == 300
+    assert t.shuffled_ind.dtype == torch.int64 and t.structured_points.shape == (300, 2)
+
+
+# ---------------------------------------------------------------------------------------------
+# C ABI
+# ---------------------------------------------------------------------------------------------
+def _header_symbols():
+    with open(os.path.join(ROOT, "include", "torch_kdtree_amd.h")) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(tk_\w+)\s*\(", text, re.M)))
+
+
+def test_c_abi_exports_every_declared_symbol():
+    lib = ctypes.CDLL(os.path.join(ROOT, "torch_kdtree_amd", "libtorch_kdtree_amd.so"))
+    syms = _header_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
+def test_c_abi_build_query_cpu_and_errors():
+    lib = ctypes.CDLL(os.path.join(ROOT, "torch_kdtree_amd", "libtorch_kdtree_amd.so"))
+    lib.tk_last_error.restype = ctypes.c_char_p
+    lib.tk_version.restype = ctypes.c_char_p
+    assert lib.tk_version() == b"1.0"
+    pts = np.random.default_rng(0).standard_normal((1000, 3)).astype(np.float32)
+    h = ctypes.c_void_p()
+    rc = lib.tk_tree_build(pts.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(1000), 3, 0, 7, 0, ctypes.byref(h))
+    assert rc == 0
+    q = pts[:10].copy()
+    dist = np.empty((10, 4), np.float32)
+    idx = np.empty((10, 4), np.int32)
+    orig = np.empty((10, 4), np.int64)
+    rc = lib.tk_knn_query_cpu(h, q.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(10), 4,
+                              dist.ctypes.data_as(ctypes.c_void_p), idx.ctypes.data_as(ctypes.c_void_p),
+                              orig.ctypes.data_as(ctypes.c_void_p), 0)
+    assert rc == 0 and np.array_equal(orig[:, 0], np.arange(10)) and np.all(dist[:, 0] == 0)
+    rc = lib.tk_knn_query_cpu(h, q.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(10), 1001,
+                              dist.ctypes.data_as(ctypes.c_void_p), idx.ctypes.data_as(ctypes.c_void_p), None, 0)
+    assert rc != 0 and b"more nearest neighbors" in lib.tk_last_error()
+    # a CPU-only tree refuses GPU queries with an error, not a crash
+    rc = lib.tk_knn_query_gpu(h, None, ctypes.c_int64(10), 4, None, None, None, 0, -1, None, ctypes.c_size_t(0), None)
+    assert rc != 0 and b"no GPU copy" in lib.tk_last_error()
+    lib.tk_tree_free(h)
+    bad = ctypes.c_void_p()
+    assert lib.tk_tree_build(pts.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(1000), 5, 0, 7, 0,
+                             ctypes.byref(bad)) != 0
+    assert b"dimensionality" in lib.tk_last_error()

@@ -54,7 +54,7 @@ def test_exact_ties_on_integer_grid(cuda, policy):
     """Points and queries on a small integer lattice: nearly every distance is tied."""
     rng = np.random.default_rng(7)
     ref = rng.integers(0, 6, (5000, 3)).astype(np.float32)
-    qry = rng.integers(0, 6, (2000, 3)).astype(np.float32) + np.float32(0.5) * rng.integers(0, 2, (2000, 3))
+    qry = (rng.integers(0, 6, (2000, 3)) + 0.5 * rng.integers(0, 2, (2000, 3))).astype(np.float32)
     tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels=8, tie_break=policy)
     for k in (1, 7, 20, 64):
         dist, sidx, oidx = gpu_query(tree, qry, k, cuda)

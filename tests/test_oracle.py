@@ -1,0 +1,162 @@
+"""Pins of the oracle (CPU restatement of the reference path) — no GPU needed.
+
+Each oracle function is checked against something that does not share its code:
+* known-answer vectors traced by hand through the reference algorithm (tests/golden/);
+* a numpy float32 brute force with explicit per-step rounding (tests/golden/bruteforce_f32.npz);
+* the oracle's own brute force on random data (the KNN set is unique up to exact ties);
+* the reference's own debug cross-check, iterative == recursive (src/kdtree.cpp:182-189);
+* tree-layout invariants of createKDTree (src/kdtree.hpp:418-534);
+* the default-levels closed form (nn_distance.py:146-147) at hand-computed points.
+A deliberately broken oracle variant is not needed: every check above fails on a dropped
+term, a swapped operand or a wrong tie rule (see test_plants_are_caught).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+
+
+def _known():
+    with open(os.path.join(GOLD, "known_answers.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("case", _known(), ids=lambda c: c["name"])
+def test_known_answers(case):
+    pts = np.array(case["points"], np.float32)
+    t = oracle.OracleTree(pts, case["levels"])
+    e = t.export()
+    tr = case["tree"]
+    assert np.array_equal(e["structured"], np.array(tr["structured"], np.float32))
+    assert e["shuffled"].tolist() == tr["shuffled"]
+    assert e["axis"].tolist() == tr["axis"]
+    assert e["median"].tolist() == tr["median"]
+    assert np.append(e["leaf_off"], len(pts)).tolist() == tr["leaf_off"]
+    for q in case["queries"]:
+        d, i = t.search(np.array([q["q"]], np.float32), q["k"])
+        assert d[0].tolist() == q["dist"]
+        assert i[0].tolist() == q["idx"]
+        assert e["shuffled"][i[0]].tolist() == q["orig"]
+        bd, bi = oracle.bruteforce(pts, np.array([q["q"]], np.float32), q["k"])
+        assert bi[0].tolist() == q["orig_lower"]
+        assert bd[0].tolist() == q["dist"]
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_bruteforce_matches_numpy_fixture(d):
+    z = np.load(os.path.join(GOLD, "bruteforce_f32.npz"))
+    bd, bi = oracle.bruteforce(z["ref_d%d" % d], z["qry_d%d" % d], 7)
+    assert np.array_equal(bd, z["dist_d%d" % d])
+    assert np.array_equal(bi, z["idx_d%d" % d])
+
+
+@pytest.mark.parametrize("d", [2, 3])
+@pytest.mark.parametrize("levels", [1, 3, 6])
+def test_tree_search_on_fixture_ties(d, levels):
+    """Reference search on the tie-heavy fixture: distances equal brute force everywhere, and
+    the index multiset equals brute force outside the last tie group of each row."""
+    z = np.load(os.path.join(GOLD, "bruteforce_f32.npz"))
+    ref, qry = z["ref_d%d" % d], z["qry_d%d" % d]
+    t = oracle.OracleTree(ref, levels)
+    dist, idx = t.search(qry, 7)
+    assert np.array_equal(dist, z["dist_d%d" % d])
+    orig = t.export()["shuffled"][idx]
+    for row in range(qry.shape[0]):
+        last = dist[row, -1]
+        inner = dist[row] < last
+        assert sorted(orig[row][inner]) == sorted(z["idx_d%d" % d][row][inner])
+
+
+def _numpy_sq_dist(q, r):
+    s = np.zeros(q.shape[0], q.dtype)
+    for a in range(q.shape[1]):
+        v = (q[:, a] - r[:, a]).astype(q.dtype)
+        s = (s + (v * v).astype(q.dtype)).astype(q.dtype)
+    return s
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [1, 2, 3, 8])
+def test_search_against_bruteforce_and_recursive(dtype, d):
+    rng = np.random.default_rng(d + (10 if dtype == np.float64 else 0))
+    ref = (rng.standard_normal((4000, d)) * 1e3).astype(dtype)
+    qry = (rng.standard_normal((600, d)) * 1e3).astype(dtype)
+    levels = oracle.default_levels(4000)
+    t = oracle.OracleTree(ref, levels)
+    e = t.export()
+    for k in (1, 6, 17):
+        dist, idx = t.search(qry, k)
+        rd, ri = t.search(qry, k, recursive=True)
+        assert np.array_equal(dist, rd) and np.array_equal(idx, ri)  # src/kdtree.cpp:182-189
+        bd, bi = oracle.bruteforce(ref, qry, k)
+        assert np.array_equal(dist, bd)
+        assert np.array_equal(e["shuffled"][idx], bi)  # continuous data: no exact ties
+        for j in range(k):  # distances are the reference's arithmetic, restated in numpy
+            assert np.array_equal(dist[:, j], _numpy_sq_dist(qry, e["structured"][idx[:, j]]))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [1, 2, 3, 8])
+@pytest.mark.parametrize("n,levels", [(1000, 6), (777, 5), (64, 7), (5, 1), (3001, 10)])
+def test_tree_layout_invariants(dtype, d, n, levels):
+    rng = np.random.default_rng(n + d)
+    pts = rng.integers(0, 7, (n, d)).astype(dtype)  # duplicates on every axis
+    t = oracle.OracleTree(pts, levels)
+    e = t.export()
+    assert t.n_nodes == 2 ** levels - 1 and t.n_leaves == 2 ** levels
+    assert sorted(e["shuffled"].tolist()) == list(range(n))
+    assert np.array_equal(e["structured"], pts[e["shuffled"]])
+    assert np.array_equal(np.cumsum(np.concatenate([[0], e["leaf_n"][:-1]])), e["leaf_off"])
+    assert e["leaf_n"].sum() == n
+
+    def check(node, lo, hi, depth):
+        a = e["axis"][node]
+        assert a == depth % d
+        cnt = hi - lo
+        m = e["median"][node]
+        seg = np.sort(e["structured"][lo:hi, a])  # children re-sort their ranges on other axes
+        half = cnt // 2
+        expect = seg[half] if cnt % 2 else dtype(0.5 * (seg[half] + seg[half - 1]))
+        assert m == expect
+        assert np.all(e["structured"][lo:lo + half, a] <= m) and np.all(e["structured"][lo + half:hi, a] >= m)
+        if depth == levels - 1:
+            leaf = 2 * node + 1 - t.n_nodes
+            assert e["leaf_off"][leaf] == lo and e["leaf_n"][leaf] == half and e["leaf_n"][leaf + 1] == cnt - half
+            return
+        check(2 * node + 1, lo, lo + half, depth + 1)
+        check(2 * node + 2, lo + half, hi, depth + 1)
+
+    check(0, 0, n, 0)
+
+
+def test_ran_out_of_points():
+    pts = np.zeros((63, 3), np.float32)
+    with pytest.raises(RuntimeError, match="Ran out of points"):
+        oracle.OracleTree(pts, 7)  # needs 2^(7-1) = 64 points
+    oracle.OracleTree(np.zeros((64, 3), np.float32), 7)
+
+
+def test_default_levels_closed_form():
+    # int(log(N)/log(2)) - 3 clamped to [1, 13], evaluated by hand
+    table = {5: 1, 10: 1, 20: 1, 30: 1, 64: 3, 1000: 6, 1024: 7, 10000: 10, 100000: 13, 4_000_000: 13}
+    for n, lv in table.items():
+        assert oracle.default_levels(n) == lv, n
+
+
+def test_plants_are_caught():
+    """Mistakes a restatement could make are visible to the checks above."""
+    z = np.load(os.path.join(GOLD, "bruteforce_f32.npz"))
+    ref, qry = z["ref_d3"], z["qry_d3"]
+    good = z["dist_d3"]
+    # r - q + square of the sum instead of the sum of squares
+    wrong = np.array([[np.float32(np.sum(q - r)) ** 2 for r in ref[:7]] for q in qry[:3]])
+    assert not np.array_equal(np.sort(wrong, axis=1), good[:3])
+    # lower-index vs reference tie order differ on the hand-traced case
+    case = _known()[1]["queries"][2]
+    assert case["orig"] != case["orig_lower"]
