@@ -35,7 +35,7 @@ def _parse():
     ap.add_argument("--workload", default="plain_gpu", choices=["plain_gpu"])
     ap.add_argument("--levels", default="auto", help="tree depth: 'auto', 'default' or an int")
     ap.add_argument("--tie", default="reference", choices=["reference", "lower_index"])
-    ap.add_argument("--presort", default="auto", choices=["auto", "on", "off"])
+    ap.add_argument("--presort", default="auto", choices=["auto", "morton", "leaf", "off"])
     ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="queries timed on the CPU (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     return ap.parse_args()

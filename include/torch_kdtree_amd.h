@@ -45,8 +45,11 @@ enum { TK_F32 = 0, TK_F64 = 1 };
  * TK_TIE_LOWER_INDEX: the k smallest (distance, original index) pairs. */
 enum { TK_TIE_REFERENCE = 0, TK_TIE_LOWER_INDEX = 1 };
 
-/* Query presort (groups queries by the leaf they fall into before the traversal). */
-enum { TK_PRESORT_AUTO = -1, TK_PRESORT_OFF = 0, TK_PRESORT_ON = 1 };
+/* Query presort before the traversal, so that the lanes of a wavefront get nearby queries:
+ * TK_PRESORT_MORTON buckets queries by the Morton code of their position in the queries'
+ * bounding box, TK_PRESORT_LEAF by the tree leaf they fall into; AUTO = MORTON from 4096
+ * queries up.  The presort changes the order of work only, never the results. */
+enum { TK_PRESORT_AUTO = -1, TK_PRESORT_OFF = 0, TK_PRESORT_MORTON = 1, TK_PRESORT_LEAF = 2 };
 
 /* Thread-local message of the last failure on this thread. */
 const char* tk_last_error(void);

@@ -68,15 +68,21 @@ def test_exact_ties_on_integer_grid(cuda, policy):
             assert_same(oidx, bi.astype(np.int64), "original indices")
 
 
-def test_presort_does_not_change_results(cuda):
-    rng = np.random.default_rng(8)
-    ref, qry = _data(rng, 50000, 20000, 3, np.float32)
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [1, 2, 3, 8])
+def test_presort_does_not_change_results(cuda, dtype, d):
+    rng = np.random.default_rng(8 + d)
+    ref, qry = _data(rng, 50000, 20000, d, dtype)
     out = {}
-    for mode in ("on", "off"):
+    for mode in ("morton", "leaf", "off"):
         tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels=13, presort=mode)
         out[mode] = gpu_query(tree, qry, 16, cuda)
-    for a, b in zip(out["on"], out["off"]):
-        assert_same(a, b, "presort on vs off")
+    for mode in ("morton", "leaf"):
+        for a, b in zip(out[mode], out["off"]):
+            assert_same(a, b, "presort %s vs off" % mode)
+    od, oi = oracle_for(tree).search(qry, 16)
+    assert_same(out["off"][0], od, "distances")
+    assert_same(out["off"][1], oi, "structured indices")
 
 
 def test_small_and_degenerate_shapes(cuda):

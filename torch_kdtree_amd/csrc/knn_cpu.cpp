@@ -13,6 +13,14 @@
 //     inserted at the upper-bound position (src/nndistance.hpp:41-85).
 // The reference keeps a per-node tag array; here the state is the node index and the
 // child we came back from, which is all the tag array encodes on the active path.
+//
+// Extra pruning that keeps the reference's output: besides the reference's bound, a far side
+// is tested against the distance from q to that side's cell (the box cut out by every split
+// on the path, kept as per-axis offsets `bo`).  When that box distance is >= worst, every
+// point below has d >= worst and the reference would reject all of them; its visit would then
+// change nothing but pp, where it leaves pp[a] = q[a] for each axis a of the levels it walks
+// (it always walks the near path down to the leaf-parent level).  Skipping the side and
+// applying exactly those resets keeps every later decision identical to the reference's.
 #include <cstring>
 #include <limits>
 #include <vector>
@@ -45,6 +53,26 @@ inline T proj_bound(const T* q, const T* pp, int a, T m) {
     return s;
 }
 
+// Box bound of the far side of a split (a, m): offsets bo[i] to the cell on the other axes,
+// m - q[a] on axis a, each rounded once, squared and summed in axis order like a distance.
+template <typename T, int D>
+inline T box_bound(const T* q, const T* bo, int a, T m) {
+    T s = T(0);
+    for (int i = 0; i < D; ++i) {
+        const T v = (i == a) ? T(m - q[i]) : bo[i];
+        s = s + v * v;
+    }
+    return s;
+}
+
+// Bits l < 32 with l mod D == a.
+template <int D>
+inline uint32_t axis_levels(int a) {
+    uint32_t m = 0;
+    for (int l = a; l < 32; l += D) m |= 1u << l;
+    return m;
+}
+
 template <typename T, int D>
 struct Searcher {
     const Tree& t;
@@ -75,10 +103,14 @@ struct Searcher {
     }
 
     void one(const T* q, T* bd, int32_t* bi) const {
-        T pp[D];
-        for (int i = 0; i < D; ++i) pp[i] = q[i];
+        T pp[D], bo[D];
+        for (int i = 0; i < D; ++i) {
+            pp[i] = q[i];
+            bo[i] = T(0);
+        }
         const int L = t.levels;
         const int32_t nn = t.n_nodes;
+        uint32_t far_mask = 0;  // bit l: the path goes through the far child of its level-l node
         int32_t node = 0;
         int level = 0;
         auto descend = [&]() {
@@ -90,14 +122,15 @@ struct Searcher {
         };
         descend();
         for (;;) {
-            // at a leaf-parent: near leaf, then the far leaf if the bound allows it
+            // at a leaf-parent: near leaf, then the far leaf if both bounds allow it
             {
                 const int a = level % D;
                 const T m = split[node];
                 const bool lower = q[a] < m;
                 const int32_t left_leaf = 2 * node + 1 - nn;
                 scan(lower ? left_leaf : left_leaf + 1, q, bd, bi);
-                if (proj_bound<T, D>(q, pp, a, m) < bd[k - 1]) scan(lower ? left_leaf + 1 : left_leaf, q, bd, bi);
+                if (proj_bound<T, D>(q, pp, a, m) < bd[k - 1] && box_bound<T, D>(q, bo, a, m) < bd[k - 1])
+                    scan(lower ? left_leaf + 1 : left_leaf, q, bd, bi);
                 if (level == 0) return;
                 pp[a] = q[a];
             }
@@ -114,10 +147,27 @@ struct Searcher {
                 if (child == near_child) {
                     pp[a] = m;
                     if (proj_bound<T, D>(q, pp, a, m) < bd[k - 1]) {
-                        node = 2 * node + (lower ? 2 : 1);
-                        ++level;
-                        entered = true;
-                        break;
+                        if (box_bound<T, D>(q, bo, a, m) < bd[k - 1]) {
+                            far_mask |= 1u << level;
+                            bo[a] = m - q[a];
+                            node = 2 * node + (lower ? 2 : 1);
+                            ++level;
+                            entered = true;
+                            break;
+                        }
+                        // the reference enters and rejects every point; replay its pp resets
+                        for (int l = level + 1; l < L && l <= level + D; ++l) pp[l % D] = q[l % D];
+                    }
+                } else {
+                    // back from the far child: the box offset on this axis reverts to the
+                    // deepest far split of the same axis above this level
+                    far_mask &= ~(1u << level);
+                    const uint32_t above = far_mask & axis_levels<D>(a) & ((1u << level) - 1u);
+                    if (above) {
+                        const int lp = 31 - __builtin_clz(above);
+                        bo[a] = split[((node + 1) >> (level - lp)) - 1] - q[a];
+                    } else {
+                        bo[a] = T(0);
                     }
                 }
                 pp[a] = q[a];

@@ -266,22 +266,24 @@ constexpr int kMaxBucketBits = 22;
 
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
-bool presort_enabled(const Tree& t, int64_t nq, const QueryOpts& o) {
-    if (o.presort == kPresortOff) return false;
-    if (o.presort == kPresortOn) return nq > 0;
-    return nq >= kPresortMinQueries && t.levels > 1;
+int presort_mode(const Tree& t, int64_t nq, const QueryOpts& o) {
+    if (nq <= 0) return kPresortOff;
+    if (o.presort == kPresortAuto) return (nq >= kPresortMinQueries && t.levels > 1) ? kPresortMorton : kPresortOff;
+    if (o.presort != kPresortOff && o.presort != kPresortMorton && o.presort != kPresortLeaf)
+        throw std::runtime_error("unknown presort mode");
+    return o.presort;
 }
 
 int bucket_shift(const Tree& t) { return std::max(0, t.levels - kMaxBucketBits); }
 
 struct PresortLayout {
-    size_t bucket, rank, order, counts, offsets, sums, total;
+    size_t bucket, rank, order, counts, offsets, sums, bbox, total;
     int32_t nb, nsums;
 };
 
-PresortLayout presort_layout(const Tree& t, int64_t nq) {
+PresortLayout presort_layout(const Tree& t, int64_t nq, int mode) {
     PresortLayout l{};
-    l.nb = t.n_leaves >> bucket_shift(t);
+    l.nb = mode == kPresortMorton ? morton_bucket_count(t.dims) : (t.n_leaves >> bucket_shift(t));
     l.nsums = (l.nb + kScanChunk - 1) / kScanChunk;
     size_t off = 0;
     l.bucket = off; off += align256(sizeof(int32_t) * (size_t)nq);
@@ -290,17 +292,25 @@ PresortLayout presort_layout(const Tree& t, int64_t nq) {
     l.counts = off; off += align256(sizeof(int32_t) * (size_t)l.nb);
     l.offsets = off; off += align256(sizeof(int32_t) * (size_t)l.nb);
     l.sums = off; off += align256(sizeof(int32_t) * (size_t)l.nsums);
+    l.bbox = off; off += align256(2 * 8 * (size_t)t.dims);
     l.total = off;
     return l;
 }
 
 void check_launch() { TK_HIP_CHECK(hipGetLastError()); }
 
+__global__ void fill_u64_kernel(unsigned long long* __restrict__ p, int n, unsigned long long lo_v,
+                                unsigned long long hi_v, int half) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i < half ? lo_v : hi_v;
+}
+
 }  // namespace
 
 size_t query_gpu_workspace(const Tree& t, int64_t nq, const QueryOpts& o) {
-    if (!presort_enabled(t, nq, o)) return 0;
-    return presort_layout(t, nq).total;
+    const int mode = presort_mode(t, nq, o);
+    if (mode == kPresortOff) return 0;
+    return presort_layout(t, nq, mode).total;
 }
 
 void query_gpu(const Tree& t, const void* queries, int64_t nq, int k, void* dist, int32_t* idx, int64_t* orig,
@@ -317,9 +327,10 @@ void query_gpu(const Tree& t, const void* queries, int64_t nq, int k, void* dist
     DeviceGuard g(t.dev->device);
     const int32_t* order = nullptr;
     std::unique_ptr<StreamScratch> own;
-    if (presort_enabled(t, nq, o)) {
+    const int mode = presort_mode(t, nq, o);
+    if (mode != kPresortOff) {
         ProfScope ps("presort", stream);
-        const PresortLayout l = presort_layout(t, nq);
+        const PresortLayout l = presort_layout(t, nq, mode);
         unsigned char* ws = static_cast<unsigned char*>(workspace);
         if (!ws) {
             own.reset(new StreamScratch(l.total, stream));
@@ -333,14 +344,32 @@ void query_gpu(const Tree& t, const void* queries, int64_t nq, int k, void* dist
         int32_t* counts = reinterpret_cast<int32_t*>(ws + l.counts);
         int32_t* offsets = reinterpret_cast<int32_t*>(ws + l.offsets);
         int32_t* sums = reinterpret_cast<int32_t*>(ws + l.sums);
+        void* bbox = ws + l.bbox;
         hipLaunchKernelGGL(fill_i32_kernel, dim3((l.nb + 255) / 256), dim3(256), 0, stream, counts, (int64_t)l.nb, 0);
         check_launch();
-        if (t.dtype == kF32)
+        if (mode == kPresortMorton) {
+            // bbox = [min codes (all ones) | max codes (zero)] in the width of T
+            if (t.dtype == kF32) {
+                hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, stream, static_cast<int32_t*>(bbox),
+                                   (int64_t)t.dims, (int32_t)-1);
+                hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, stream, static_cast<int32_t*>(bbox) + t.dims,
+                                   (int64_t)t.dims, 0);
+                check_launch();
+                launch_morton_bucket_f32(t, static_cast<const float*>(queries), nq, bbox, bucket, rank, counts, stream);
+            } else {
+                hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, stream,
+                                   static_cast<unsigned long long*>(bbox), 2 * t.dims, ~0ull, 0ull, t.dims);
+                check_launch();
+                launch_morton_bucket_f64(t, static_cast<const double*>(queries), nq, bbox, bucket, rank, counts,
+                                         stream);
+            }
+        } else if (t.dtype == kF32) {
             launch_leaf_bucket_f32(t, static_cast<const float*>(queries), nq, bucket_shift(t), bucket, rank, counts,
                                    stream);
-        else
+        } else {
             launch_leaf_bucket_f64(t, static_cast<const double*>(queries), nq, bucket_shift(t), bucket, rank, counts,
                                    stream);
+        }
         hipLaunchKernelGGL(scan_block_sums_kernel, dim3(l.nsums), dim3(kScanBlock), 0, stream, counts, l.nb, sums);
         check_launch();
         hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, stream, sums, l.nsums);

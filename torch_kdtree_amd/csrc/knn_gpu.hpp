@@ -25,7 +25,8 @@ void upload_tree(Tree& t, int device);
 void free_device_tree(Tree& t) noexcept;
 
 enum TiePolicy : int { kTieReference = 0, kTieLowerIndex = 1 };
-enum PresortMode : int { kPresortAuto = -1, kPresortOff = 0, kPresortOn = 1 };
+// Query presort: auto = Morton buckets when there are enough queries to matter.
+enum PresortMode : int { kPresortAuto = -1, kPresortOff = 0, kPresortMorton = 1, kPresortLeaf = 2 };
 
 // Largest k served on the GPU (the reference's limit, src/kdtree_g.cu:283,421-422).
 constexpr int kMaxGpuK = 128;
@@ -52,6 +53,11 @@ void launch_leaf_bucket_f32(const Tree& t, const float* q, int64_t nq, int shift
                             int32_t* counts, hipStream_t s);
 void launch_leaf_bucket_f64(const Tree& t, const double* q, int64_t nq, int shift, int32_t* bucket, int32_t* rank,
                             int32_t* counts, hipStream_t s);
+void launch_morton_bucket_f32(const Tree& t, const float* q, int64_t nq, void* bbox, int32_t* bucket, int32_t* rank,
+                              int32_t* counts, hipStream_t s);
+void launch_morton_bucket_f64(const Tree& t, const double* q, int64_t nq, void* bbox, int32_t* bucket,
+                              int32_t* rank, int32_t* counts, hipStream_t s);
+int morton_bucket_count(int dims);
 
 // Brute force (bruteforce.hip).
 void bruteforce_gpu(const void* ref, int32_t n, const void* queries, int64_t nq, int dims, int dtype, int k,

@@ -9,9 +9,12 @@
 // the pp updates and the candidate insertion repeat the reference's arithmetic, so with the
 // reference tie policy every lane produces what the reference CPU search produces.
 //
-// Queries arrive in an order given by `order` (the leaf-bucket presort below), so the lanes
-// of a wavefront descend into the same leaves and their record loads coalesce.
+// Queries arrive in an order given by `order` (a presort below: Morton buckets of the query
+// positions by default, or buckets by near leaf), so the lanes of a wavefront walk similar paths
+// and their record loads coalesce.
 #pragma once
+
+#include <type_traits>
 
 #include "gpu_common.hpp"
 
@@ -89,73 +92,160 @@ __device__ __forceinline__ void scan_leaf(const TreeView<T>& tv, int32_t leaf, c
     }
 }
 
-template <typename T, int D, bool kLower>
-__device__ __forceinline__ bool far_side_needed(const T (&q)[D], const T (&pp)[D], int a, T m, T worst) {
-    if constexpr (kLower)
-        return clean_bound<T, D>(q, pp, a, m) <= worst;
-    else
-        return proj_bound<T, D>(q, pp, a, m) < worst;
+// Box bound of the far side of split (a, m): bo[i] is the rounded offset from q to the current
+// cell on axis i (0 when q is inside it); on axis a the far side starts at m.  Each term is one
+// rounded difference, squared and summed in axis order like a distance, so by the monotonicity
+// of rounding it never exceeds the computed distance of a point on that side.
+template <typename T, int D>
+__device__ __forceinline__ T box_bound(const T (&q)[D], const T (&bo)[D], int a, T m) {
+    T s = T(0);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        const T v = (i == a) ? (m - q[i]) : bo[i];
+        s = s + v * v;
+    }
+    return s;
 }
 
+// Bits l < 32 with l mod D == a.
+template <int D>
+__device__ __forceinline__ uint32_t axis_levels(int a) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int l = 0; l < 32; ++l)
+        if (l % D == a) m |= 1u << l;
+    return m;
+}
+
+// Far side test.  Reference policy: the reference's own test (proj_bound < worst) decides
+// whether the reference would look at the far side; the box bound then tells whether that look
+// could change anything: when box >= worst every point there has d >= worst and would be
+// rejected.  Lower-index policy: only the box bound, non-strict, so that a point at exactly the
+// worst distance with a lower index is still found.
+// Returns 0: skip (the reference skips too), 1: enter, 2: skip a side the reference enters.
+template <typename T, int D, bool kLower>
+__device__ __forceinline__ int far_side(const T (&q)[D], const T (&pp)[D], const T (&bo)[D], int a, T m, T worst) {
+    if constexpr (kLower) {
+        return box_bound<T, D>(q, bo, a, m) <= worst ? 1 : 0;
+    } else {
+        if (!(proj_bound<T, D>(q, pp, a, m) < worst)) return 0;
+        return box_bound<T, D>(q, bo, a, m) < worst ? 1 : 2;
+    }
+}
+
+template <typename T, int D>
+__device__ __forceinline__ T ax_get(const T (&v)[D], int a) {
+    T r = v[0];
+#pragma unroll
+    for (int i = 1; i < D; ++i) r = (a == i) ? v[i] : r;
+    return r;
+}
+
+template <typename T, int D>
+__device__ __forceinline__ void ax_set(T (&v)[D], int a, T x) {
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+        if (a == i) v[i] = x;
+}
+
+template <int D>
+__device__ __forceinline__ int ax_next(int a) { return a + 1 == D ? 0 : a + 1; }
+template <int D>
+__device__ __forceinline__ int ax_prev(int a) { return a == 0 ? D - 1 : a - 1; }
+
 // The search of one query; `kb` must be initialised.
+//
+// Besides pp (the reference's projected point, kept exactly as the reference keeps it) each lane
+// carries bo, the offsets from q to the cell of the current node, and far_mask, the levels at
+// which the current path took a far child.  When the reference would enter a far side whose box
+// bound is >= worst, the lane skips it and applies the only lasting effect the reference's visit
+// would have had: pp[a] = q[a] for the axes of the levels below (the reference always walks the
+// near path down to the leaf-parent level and resets each axis it leaves).
 template <typename T, int D, bool kLower, typename List>
 __device__ __forceinline__ void knn_search_one(const TreeView<T>& tv, const T (&q)[D], List& kb) {
-    T pp[D];
+    T pp[D], bo[D];
 #pragma unroll
-    for (int i = 0; i < D; ++i) pp[i] = q[i];
+    for (int i = 0; i < D; ++i) {
+        pp[i] = q[i];
+        bo[i] = T(0);
+    }
+    uint32_t far_mask = 0;
     const int L = tv.levels;
     const int32_t nn = tv.n_nodes;
     int32_t node = 0;
     int level = 0;
+    int a = 0;  // split axis of `node`: level mod D
     while (level < L - 1) {
-        const int a = level % D;
-        node = 2 * node + (q[a] < tv.split[node] ? 1 : 2);
+        node = 2 * node + (ax_get(q, a) < tv.split[node] ? 1 : 2);
         ++level;
+        a = ax_next<D>(a);
     }
     // phase 0: the near leaf of leaf-parent `node` is next; phase 1: the far leaf is to be
     // tested; phase 2: both leaves are done, climb.
     int phase = 0;
     for (;;) {
         int32_t leaf;
-        const int a = level % D;
         const T m = tv.split[node];
-        const bool lower = q[a] < m;
+        const T qa = ax_get(q, a);
+        const bool lower = qa < m;
         if (phase == 0) {
             leaf = 2 * node + (lower ? 1 : 2) - nn;
             phase = 1;
-        } else if (phase == 1 && far_side_needed<T, D, kLower>(q, pp, a, m, kb.worst())) {
+        } else if (phase == 1 && far_side<T, D, kLower>(q, pp, bo, a, m, kb.worst()) == 1) {
             leaf = 2 * node + (lower ? 2 : 1) - nn;
             phase = 2;
         } else {
             if (level == 0) break;
-            pp[a] = q[a];
+            ax_set(pp, a, qa);
             bool entered = false;
             while (level > 0) {
                 const int32_t child = node;
                 node = (node - 1) >> 1;
                 --level;
-                const int a2 = level % D;
+                a = ax_prev<D>(a);
                 const T m2 = tv.split[node];
-                const bool lower2 = q[a2] < m2;
+                const T q2 = ax_get(q, a);
+                const bool lower2 = q2 < m2;
                 if (child == 2 * node + (lower2 ? 1 : 2)) {
-                    pp[a2] = m2;
-                    if (far_side_needed<T, D, kLower>(q, pp, a2, m2, kb.worst())) {
+                    ax_set(pp, a, m2);
+                    const int f = far_side<T, D, kLower>(q, pp, bo, a, m2, kb.worst());
+                    if (f == 1) {
+                        far_mask |= 1u << level;
+                        ax_set(bo, a, T(m2 - q2));
                         node = 2 * node + (lower2 ? 2 : 1);
                         ++level;
+                        a = ax_next<D>(a);
                         entered = true;
                         break;
                     }
+                    if (f == 2) {
+                        // axis i is reset iff it belongs to one of the next min(D, L-1-level) levels
+                        const int nlev = min(D, L - 1 - level);
+#pragma unroll
+                        for (int i = 0; i < D; ++i) {
+                            const int off = (i - a - 1 + 2 * D) % D;
+                            if (off < nlev) pp[i] = q[i];
+                        }
+                    }
+                } else {
+                    far_mask &= ~(1u << level);
+                    const uint32_t above = far_mask & axis_levels<D>(a) & ((1u << level) - 1u);
+                    T off = T(0);
+                    if (above) {
+                        const int lp = 31 - __builtin_clz(above);
+                        off = tv.split[((node + 1) >> (level - lp)) - 1] - q2;
+                    }
+                    ax_set(bo, a, off);
                 }
-                pp[a2] = q[a2];
+                ax_set(pp, a, q2);
             }
             if (!entered) break;
             while (level < L - 1) {
-                const int a3 = level % D;
-                node = 2 * node + (q[a3] < tv.split[node] ? 1 : 2);
+                node = 2 * node + (ax_get(q, a) < tv.split[node] ? 1 : 2);
                 ++level;
+                a = ax_next<D>(a);
             }
-            const int a4 = level % D;
-            leaf = 2 * node + (q[a4] < tv.split[node] ? 1 : 2) - nn;
+            leaf = 2 * node + (ax_get(q, a) < tv.split[node] ? 1 : 2) - nn;
             phase = 1;
         }
         scan_leaf<T, D>(tv, leaf, q, kb);
@@ -257,13 +347,107 @@ __global__ __launch_bounds__(kPresortBlock) void leaf_bucket_kernel(TreeView<T> 
     T q[D];
     load_query<T, D>(queries, i, q);
     int32_t node = 0;
+    int a = 0;
     for (int level = 0; level < tv.levels; ++level) {
-        const int a = level % D;
-        node = 2 * node + (q[a] < tv.split[node] ? 1 : 2);
+        node = 2 * node + (ax_get(q, a) < tv.split[node] ? 1 : 2);
+        a = ax_next<D>(a);
     }
     const int32_t b = (node - tv.n_nodes) >> shift;
     bucket[i] = b;
     rank[i] = atomicAdd(&counts[b], 1);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Morton presort: queries are bucketed by the Morton code of their cell in a 2^B-per-axis grid
+// over the queries' own bounding box (B*D <= 21), so queries outside the tree's cloud are
+// grouped by where they really are (two queries that fall into the same boundary leaf can need
+// very different amounts of work), and a wavefront's lanes end up with similar work.
+// ---------------------------------------------------------------------------------------------
+template <int D>
+struct MortonBits {
+    static constexpr int kBits = D == 1 ? 20 : D == 2 ? 10 : D == 3 ? 7 : 2;
+    static constexpr int kBuckets = 1 << (kBits * D);
+};
+
+// Order-preserving map of floating-point values onto unsigned integers (for atomicMin/Max).
+__device__ __forceinline__ uint32_t ordered_bits(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float from_ordered(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+__device__ __forceinline__ unsigned long long ordered_bits(double f) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(f);
+    return (u >> 63) ? ~u : (u | (1ull << 63));
+}
+__device__ __forceinline__ double from_ordered(unsigned long long u) {
+    return __longlong_as_double((long long)((u >> 63) ? (u & ~(1ull << 63)) : ~u));
+}
+
+template <typename T>
+using OrderedT = typename std::conditional<sizeof(T) == 4, uint32_t, unsigned long long>::type;
+
+constexpr int kBoxBlock = 256;
+
+// box[0..D) = min, box[D..2D) = max, as ordered integers; initialised to (all ones, zero).
+template <typename T, int D>
+__global__ __launch_bounds__(kBoxBlock) void bbox_kernel(const T* __restrict__ queries, int64_t nq,
+                                                         OrderedT<T>* __restrict__ box) {
+    T lo[D], hi[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        lo[a] = pos_inf<T>();
+        hi[a] = -pos_inf<T>();
+    }
+    for (int64_t i = (int64_t)blockIdx.x * kBoxBlock + threadIdx.x; i < nq; i += (int64_t)gridDim.x * kBoxBlock) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            const T v = queries[i * D + a];
+            lo[a] = fmin(lo[a], v);
+            hi[a] = fmax(hi[a], v);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            lo[a] = fmin(lo[a], __shfl_xor(lo[a], off, 64));
+            hi[a] = fmax(hi[a], __shfl_xor(hi[a], off, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            atomicMin(&box[a], ordered_bits(lo[a]));
+            atomicMax(&box[D + a], ordered_bits(hi[a]));
+        }
+    }
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(kPresortBlock) void morton_bucket_kernel(const T* __restrict__ queries, int64_t nq,
+                                                                       const OrderedT<T>* __restrict__ box,
+                                                                       int32_t* __restrict__ bucket,
+                                                                       int32_t* __restrict__ rank,
+                                                                       int32_t* __restrict__ counts) {
+    const int64_t i = (int64_t)blockIdx.x * kPresortBlock + threadIdx.x;
+    if (i >= nq) return;
+    constexpr int B = MortonBits<D>::kBits;
+    constexpr float kCells = (float)(1 << B);
+    uint32_t code = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        const float lo = (float)from_ordered(box[a]);
+        const float hi = (float)from_ordered(box[D + a]);
+        const float scale = hi > lo ? kCells / (hi - lo) : 0.0f;
+        const float x = ((float)queries[i * D + a] - lo) * scale;
+        const uint32_t g = (uint32_t)fminf(fmaxf(x, 0.0f), kCells - 1.0f);
+#pragma unroll
+        for (int b = 0; b < B; ++b) code |= ((g >> b) & 1u) << (b * D + a);
+    }
+    bucket[i] = (int32_t)code;
+    rank[i] = atomicAdd(&counts[code], 1);
 }
 
 __global__ void bucket_scatter_kernel(int64_t nq, const int32_t* __restrict__ bucket, const int32_t* __restrict__ rank,

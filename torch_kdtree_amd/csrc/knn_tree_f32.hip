@@ -1,4 +1,6 @@
 // float32 instantiations of the traversal and presort kernels.
+#include <algorithm>
+
 #include "knn_tree_launch.hpp"
 
 namespace tk {
@@ -12,5 +14,12 @@ void launch_leaf_bucket_f32(const Tree& t, const float* q, int64_t nq, int shift
                             int32_t* counts, hipStream_t s) {
     launch_leaf_bucket_t<float>(t, q, nq, shift, bucket, rank, counts, s);
 }
+
+void launch_morton_bucket_f32(const Tree& t, const float* q, int64_t nq, void* bbox, int32_t* bucket, int32_t* rank,
+                              int32_t* counts, hipStream_t s) {
+    launch_morton_bucket_t<float>(t, q, nq, bbox, bucket, rank, counts, s);
+}
+
+int morton_bucket_count(int dims) { return morton_buckets(dims); }
 
 }  // namespace tk

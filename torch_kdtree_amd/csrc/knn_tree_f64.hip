@@ -1,4 +1,6 @@
 // float64 instantiations of the traversal and presort kernels.
+#include <algorithm>
+
 #include "knn_tree_launch.hpp"
 
 namespace tk {
@@ -11,6 +13,11 @@ void launch_tree_search_f64(const Tree& t, const double* q, int64_t nq, const in
 void launch_leaf_bucket_f64(const Tree& t, const double* q, int64_t nq, int shift, int32_t* bucket, int32_t* rank,
                             int32_t* counts, hipStream_t s) {
     launch_leaf_bucket_t<double>(t, q, nq, shift, bucket, rank, counts, s);
+}
+
+void launch_morton_bucket_f64(const Tree& t, const double* q, int64_t nq, void* bbox, int32_t* bucket,
+                              int32_t* rank, int32_t* counts, hipStream_t s) {
+    launch_morton_bucket_t<double>(t, q, nq, bbox, bucket, rank, counts, s);
 }
 
 }  // namespace tk

@@ -66,6 +66,40 @@ void launch_tree_search_t(const Tree& t, const T* q, int64_t nq, const int32_t* 
 #undef TK_DIMS
 }
 
+// bbox: 2*dims ordered integers of sizeof(T) bytes each, initialised by the caller.
+template <typename T>
+void launch_morton_bucket_t(const Tree& t, const T* q, int64_t nq, void* bbox, int32_t* bucket, int32_t* rank,
+                            int32_t* counts, hipStream_t s) {
+    auto* box = static_cast<OrderedT<T>*>(bbox);
+    const unsigned box_blocks = (unsigned)std::min<int64_t>(1024, (nq + kBoxBlock - 1) / kBoxBlock);
+    const dim3 grid((unsigned)((nq + kPresortBlock - 1) / kPresortBlock));
+#define TK_DIMS(DD)                                                                                              \
+    case DD:                                                                                                     \
+        hipLaunchKernelGGL((bbox_kernel<T, DD>), dim3(box_blocks), dim3(kBoxBlock), 0, s, q, nq, box);          \
+        hipLaunchKernelGGL((morton_bucket_kernel<T, DD>), grid, dim3(kPresortBlock), 0, s, q, nq, box, bucket,   \
+                           rank, counts);                                                                        \
+        break;
+    switch (t.dims) {
+        TK_DIMS(1)
+        TK_DIMS(2)
+        TK_DIMS(3)
+        TK_DIMS(8)
+        default: throw std::runtime_error("unsupported dims");
+    }
+#undef TK_DIMS
+    TK_HIP_CHECK(hipGetLastError());
+}
+
+inline int morton_buckets(int dims) {
+    switch (dims) {
+        case 1: return MortonBits<1>::kBuckets;
+        case 2: return MortonBits<2>::kBuckets;
+        case 3: return MortonBits<3>::kBuckets;
+        case 8: return MortonBits<8>::kBuckets;
+    }
+    throw std::runtime_error("unsupported dims");
+}
+
 template <typename T>
 void launch_leaf_bucket_t(const Tree& t, const T* q, int64_t nq, int shift, int32_t* bucket, int32_t* rank,
                           int32_t* counts, hipStream_t s) {

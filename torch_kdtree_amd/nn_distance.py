@@ -34,7 +34,7 @@ if not gpu_available:
 
 _transl_torch_device = {"cpu": "CPU", "cuda": "GPU"}
 _TIE_POLICIES = {"reference": 0, "lower_index": 1}
-_PRESORT = {"auto": -1, "off": 0, "on": 1, None: -1, False: 0, True: 1}
+_PRESORT = {"auto": -1, "off": 0, "on": 1, "morton": 1, "leaf": 2, None: -1, False: 0, True: 1}
 MAX_LEVELS = 24
 
 
@@ -126,7 +126,7 @@ class TorchKDTree:
             raise ValueError("tie_break must be one of %s" % sorted(_TIE_POLICIES))
         self.tie_break = tie_break
         if presort not in _PRESORT:
-            raise ValueError("presort must be 'auto', 'on' or 'off'")
+            raise ValueError("presort must be 'auto', 'on' ('morton'), 'leaf' or 'off'")
         self.presort = presort
 
         if self.use_gpu:
@@ -234,8 +234,9 @@ def build_kd_tree(points_ref: Union[torch.Tensor, np.ndarray], device: torch.dev
     tie_break : str
         "reference" (default): exact distance ties in the reference CPU search's order.
         "lower_index": ties ordered by the lower original index.
-    presort : "auto", "on" or "off"
-        Whether GPU queries are grouped by leaf before the traversal.
+    presort : "auto", "on"/"morton", "leaf" or "off"
+        How GPU queries are ordered before the traversal (Morton buckets of their positions, or
+        buckets by tree leaf); the order changes speed only, never results.
 
     Returns
     -------
