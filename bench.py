@@ -38,6 +38,9 @@ def _parse():
     ap.add_argument("--presort", default="auto", choices=["auto", "morton", "leaf", "off"])
     ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="queries timed on the CPU (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="before the timed steps, run one device-to-device copy of this many MiB (a kernel of known "
+                         "HBM traffic, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -55,7 +58,11 @@ def main():
     import torch_kdtree_amd as tk
     from torch_kdtree_amd import torch_knn
 
+    def log(msg):
+        print("[bench] " + msg, file=sys.stderr, flush=True)
+
     dev = torch.device("cuda", 0)
+    log("generating plain_gpu workload")
     w = workloads.plain_gpu()
     ref, qry, k = w["ref"], w["queries"], w["k"]
     n, d = ref.shape
@@ -67,6 +74,7 @@ def main():
     tree = tk.build_kd_tree(torch.from_numpy(ref), device=dev, levels=levels, tie_break=args.tie,
                             presort=args.presort)
     build_s = time.perf_counter() - t0
+    log("tree built: levels=%d in %.2f s" % (tree.levels, build_s))
 
     q = torch.from_numpy(qry).to(dev)
     res_d = torch.empty((nq, k), dtype=torch.float32, device=dev)
@@ -74,6 +82,15 @@ def main():
     for _ in range(args.warmup):
         tree.query(q, nr_nns_searches=k, result_dists=res_d, result_idx=res_i)
     torch.cuda.synchronize()
+    if args.calibrate_copy_mib > 0:
+        src = torch.empty(args.calibrate_copy_mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("calibration copy of %d MiB done" % args.calibrate_copy_mib)
+    log("timing %d steps" % args.steps)
 
     torch_knn.profile_reset()
     torch_knn.profile_enable(True)
@@ -103,6 +120,7 @@ def main():
 
     cpu = None
     if args.cpu_sample > 0:
+        log("timing the CPU baseline")
         import oracle
 
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

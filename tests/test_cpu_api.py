@@ -72,6 +72,20 @@ def test_cpu_query_matches_oracle(dtype, d):
             assert_same(inds.numpy(), t.kdtree.get_shuffled_inds()[oi].astype(np.int64), "original indices")
 
 
+@pytest.mark.parametrize("levels", [13, 15, 17])
+def test_cpu_query_plain_gpu_distribution(levels):
+    """Workload-2 distribution (uniform refs, queries partly outside the cube, clustered half):
+    long climbs and many skipped far sides."""
+    import workloads
+    w = workloads.plain_gpu(n=400_000, q=200_000)
+    t = tk.build_kd_tree(torch.from_numpy(w["ref"]), device="cpu", levels=levels)
+    res_i = torch.empty((200_000, 16), dtype=torch.int32)
+    dist, _ = t.query(torch.from_numpy(w["queries"]), nr_nns_searches=16, result_idx=res_i)
+    od, oi = oracle.tree_from_product(t).search(w["queries"], 16)
+    assert_same(dist.numpy(), od, "distances")
+    assert_same(res_i.numpy(), oi, "structured indices")
+
+
 def test_cpu_query_ties_match_oracle():
     rng = np.random.default_rng(3)
     ref = rng.integers(0, 6, (20000, 3)).astype(np.float32)

@@ -66,8 +66,13 @@ __device__ __forceinline__ Rec<T, D> load_rec(const unsigned char* __restrict__ 
 // each subtraction, product and sum (src/kdtree.hpp:546-614).  All kernels are compiled with
 // -ffp-contract=off, so a*b+c below is a rounded product followed by a rounded sum.
 // ---------------------------------------------------------------------------------------------
+// Per-lane point/offset vectors live in clang vector types: element access with a run-time
+// axis then lowers to register selects instead of forcing the values into memory.
 template <typename T, int D>
-__device__ __forceinline__ T sq_dist(const T (&q)[D], const T (&r)[D]) {
+using Pt = T __attribute__((ext_vector_type(D)));
+
+template <typename T, int D, typename A, typename B>
+__device__ __forceinline__ T sq_dist(const A& q, const B& r) {
     T s = T(0);
 #pragma unroll
     for (int i = 0; i < D; ++i) {
@@ -78,8 +83,8 @@ __device__ __forceinline__ T sq_dist(const T (&q)[D], const T (&r)[D]) {
 }
 
 // Reference pruning bound, src/kdtree.hpp:606-614: v = pp - q, v[a] += m - pp[a], sum of squares.
-template <typename T, int D>
-__device__ __forceinline__ T proj_bound(const T (&q)[D], const T (&pp)[D], int a, T m) {
+template <typename T, int D, typename V>
+__device__ __forceinline__ T proj_bound(const V& q, const V& pp, int a, T m) {
     T s = T(0);
 #pragma unroll
     for (int i = 0; i < D; ++i) {
@@ -93,8 +98,8 @@ __device__ __forceinline__ T proj_bound(const T (&q)[D], const T (&pp)[D], int a
 // Bound used by the lower-index tie policy: v[a] = m - q[a] rounded once, so that by the
 // monotonicity of rounding it never exceeds the computed distance of any point beyond the
 // split (the reference's two-rounding form above can exceed it by an ulp).
-template <typename T, int D>
-__device__ __forceinline__ T clean_bound(const T (&q)[D], const T (&pp)[D], int a, T m) {
+template <typename T, int D, typename V>
+__device__ __forceinline__ T clean_bound(const V& q, const V& pp, int a, T m) {
     T s = T(0);
 #pragma unroll
     for (int i = 0; i < D; ++i) {
@@ -152,19 +157,32 @@ struct KBest {
             return d[j] > dist;
     }
 
+    // Sorted insert of an accepted candidate (dist < d[K-1], or equal with a lower index under
+    // the lower-index policy).  With p the insert position, slot j becomes d[j-1] (j > p),
+    // dist (j == p) or d[j] (j < p); for a sorted list that value is the median of
+    // (d[j-1], d[j], dist) whatever p is, so distances need one med3 per slot and only the
+    // index arrays need the position masks.
     __device__ __forceinline__ void insert(T dist, int32_t sidx, int32_t orig) {
+        bool here_next = after(K - 1, dist, orig);
 #pragma unroll
         for (int j = K - 1; j > 0; --j) {
             const bool shift = after(j - 1, dist, orig);
-            const bool here = after(j, dist, orig);
-            d[j] = shift ? d[j - 1] : (here ? dist : d[j]);
+            const bool here = here_next;
+            here_next = shift;
+            d[j] = med3(d[j - 1], d[j], dist);
             s[j] = shift ? s[j - 1] : (here ? sidx : s[j]);
             if constexpr (kLower) o[j] = shift ? o[j - 1] : (here ? orig : o[j]);
         }
-        const bool here0 = after(0, dist, orig);
-        d[0] = here0 ? dist : d[0];
-        s[0] = here0 ? sidx : s[0];
-        if constexpr (kLower) o[0] = here0 ? orig : o[0];
+        d[0] = here_next ? dist : d[0];
+        s[0] = here_next ? sidx : s[0];
+        if constexpr (kLower) o[0] = here_next ? orig : o[0];
+    }
+
+    static __device__ __forceinline__ T med3(T a, T b, T c) {
+        if constexpr (sizeof(T) == 4)
+            return __builtin_amdgcn_fmed3f(a, b, c);
+        else
+            return fmax(fmin(a, b), fmin(fmax(a, b), c));
     }
 };
 

@@ -134,45 +134,50 @@ struct Searcher {
                 if (level == 0) return;
                 pp[a] = q[a];
             }
-            // climb until a far child is worth entering
+            // Climb.  Levels whose far child is on the path (far_mask) are left without a test:
+            // their axis is reset in pp and the box offset of that axis reverts to the deepest
+            // far split of the same axis still on the path.  Levels entered through their near
+            // child get the far-side test.  Bit scans jump from one such level to the next.
             bool entered = false;
-            while (!entered) {
-                const int32_t child = node;
-                node = (node - 1) >> 1;
-                --level;
-                const int a = level % D;
-                const T m = split[node];
-                const bool lower = q[a] < m;
-                const int32_t near_child = 2 * node + (lower ? 1 : 2);
-                if (child == near_child) {
-                    pp[a] = m;
-                    if (proj_bound<T, D>(q, pp, a, m) < bd[k - 1]) {
-                        if (box_bound<T, D>(q, bo, a, m) < bd[k - 1]) {
-                            far_mask |= 1u << level;
-                            bo[a] = m - q[a];
-                            node = 2 * node + (lower ? 2 : 1);
-                            ++level;
-                            entered = true;
-                            break;
-                        }
-                        // the reference enters and rejects every point; replay its pp resets
-                        for (int l = level + 1; l < L && l <= level + D; ++l) pp[l % D] = q[l % D];
-                    }
-                } else {
-                    // back from the far child: the box offset on this axis reverts to the
-                    // deepest far split of the same axis above this level
-                    far_mask &= ~(1u << level);
-                    const uint32_t above = far_mask & axis_levels<D>(a) & ((1u << level) - 1u);
-                    if (above) {
-                        const int lp = 31 - __builtin_clz(above);
-                        bo[a] = split[((node + 1) >> (level - lp)) - 1] - q[a];
-                    } else {
-                        bo[a] = T(0);
+            int l = level;
+            for (;;) {
+                const uint32_t below = (1u << l) - 1u;
+                const uint32_t near_lv = below & ~far_mask;
+                const int j = near_lv ? 31 - __builtin_clz(near_lv) : -1;
+                const uint32_t popped = far_mask & below & ~(j >= 0 ? ((2u << j) - 1u) : 0u);
+                if (popped) {
+                    far_mask &= ~popped;
+                    for (int ax = 0; ax < D; ++ax) {
+                        const uint32_t lv = axis_levels<D>(ax);
+                        if (!(popped & lv)) continue;
+                        pp[ax] = q[ax];
+                        const uint32_t rem = far_mask & lv;
+                        bo[ax] = rem ? T(split[((node + 1) >> (level - (31 - __builtin_clz(rem)))) - 1] - q[ax]) : T(0);
                     }
                 }
+                if (j < 0) return;
+                const int32_t nj = ((node + 1) >> (level - j)) - 1;
+                const int a = j % D;
+                const T m = split[nj];
+                const bool lower = q[a] < m;
+                pp[a] = m;
+                if (proj_bound<T, D>(q, pp, a, m) < bd[k - 1]) {
+                    if (box_bound<T, D>(q, bo, a, m) < bd[k - 1]) {
+                        far_mask |= 1u << j;
+                        bo[a] = m - q[a];
+                        node = 2 * nj + (lower ? 2 : 1);
+                        level = j + 1;
+                        entered = true;
+                        break;
+                    }
+                    // the reference enters and rejects every point; replay its pp resets
+                    for (int lv = j + 1; lv < L && lv <= j + D; ++lv) pp[lv % D] = q[lv % D];
+                }
                 pp[a] = q[a];
-                if (level == 0) return;
+                if (j == 0) return;
+                l = j;
             }
+            if (!entered) return;
             descend();
         }
     }

@@ -30,6 +30,10 @@ struct TreeView {
     int32_t n_nodes;
 };
 
+// Threads per block of the traversal kernel; per-lane LDS staging slots are strided by it.
+constexpr int kTreeBlock = 256;
+constexpr int kTreeBlockStride = kTreeBlock;
+
 // k-best list held in LDS for k beyond the register buckets: slot j of lane t lives at
 // base[j * blockDim + t] (consecutive lanes hit consecutive banks).
 template <typename T, bool kLower>
@@ -82,7 +86,7 @@ struct KBestLds {
 };
 
 template <typename T, int D, typename List>
-__device__ __forceinline__ void scan_leaf(const TreeView<T>& tv, int32_t leaf, const T (&q)[D], List& kb) {
+__device__ __forceinline__ void scan_leaf(const TreeView<T>& tv, int32_t leaf, const Pt<T, D>& q, List& kb) {
     const int32_t beg = tv.leaf_off[leaf];
     const int32_t end = tv.leaf_off[leaf + 1];
     for (int32_t s = beg; s < end; ++s) {
@@ -97,7 +101,7 @@ __device__ __forceinline__ void scan_leaf(const TreeView<T>& tv, int32_t leaf, c
 // rounded difference, squared and summed in axis order like a distance, so by the monotonicity
 // of rounding it never exceeds the computed distance of a point on that side.
 template <typename T, int D>
-__device__ __forceinline__ T box_bound(const T (&q)[D], const T (&bo)[D], int a, T m) {
+__device__ __forceinline__ T box_bound(const Pt<T, D>& q, const Pt<T, D>& bo, int a, T m) {
     T s = T(0);
 #pragma unroll
     for (int i = 0; i < D; ++i) {
@@ -124,7 +128,8 @@ __device__ __forceinline__ uint32_t axis_levels(int a) {
 // worst distance with a lower index is still found.
 // Returns 0: skip (the reference skips too), 1: enter, 2: skip a side the reference enters.
 template <typename T, int D, bool kLower>
-__device__ __forceinline__ int far_side(const T (&q)[D], const T (&pp)[D], const T (&bo)[D], int a, T m, T worst) {
+__device__ __forceinline__ int far_side(const Pt<T, D>& q, const Pt<T, D>& pp, const Pt<T, D>& bo, int a, T m,
+                                       T worst) {
     if constexpr (kLower) {
         return box_bound<T, D>(q, bo, a, m) <= worst ? 1 : 0;
     } else {
@@ -134,24 +139,66 @@ __device__ __forceinline__ int far_side(const T (&q)[D], const T (&pp)[D], const
 }
 
 template <typename T, int D>
-__device__ __forceinline__ T ax_get(const T (&v)[D], int a) {
-    T r = v[0];
-#pragma unroll
-    for (int i = 1; i < D; ++i) r = (a == i) ? v[i] : r;
-    return r;
+__device__ __forceinline__ T ax_get(const Pt<T, D>& v, int a) {
+    if constexpr (D == 1) return v[0];
+    else return v[a];
 }
 
 template <typename T, int D>
-__device__ __forceinline__ void ax_set(T (&v)[D], int a, T x) {
-#pragma unroll
-    for (int i = 0; i < D; ++i)
-        if (a == i) v[i] = x;
+__device__ __forceinline__ void ax_set(Pt<T, D>& v, int a, T x) {
+    if constexpr (D == 1) v[0] = x;
+    else v[a] = x;
 }
 
 template <int D>
 __device__ __forceinline__ int ax_next(int a) { return a + 1 == D ? 0 : a + 1; }
 template <int D>
 __device__ __forceinline__ int ax_prev(int a) { return a == 0 ? D - 1 : a - 1; }
+
+// Leaf scan in two passes, for register lists.  Pass 1 computes the distances of up to CH
+// points of the leaf (all lanes busy), tests them against the worst distance at the start of
+// the chunk and stages the accepted ones in LDS; pass 2 pops each lane's accepted points in
+// scan order and inserts those still accepted by the current list.  A wavefront then runs the
+// insertion network once per accepted point of its busiest lane instead of once per point any
+// lane accepts.  The outcome is that of a one-pass scan: the list after pass 2 holds exactly
+// the points a one-pass scan would have inserted, because a point rejected by the stale worst
+// is rejected by every later (smaller or equal) worst as well.
+template <typename T, int D, int K, bool kLower, int CH>
+__device__ __forceinline__ void scan_leaf_staged(const TreeView<T>& tv, int32_t leaf, const Pt<T, D>& q,
+                                                 KBest<T, K, kLower>& kb, T* __restrict__ st_d,
+                                                 int32_t* __restrict__ st_o) {
+    const int32_t beg = tv.leaf_off[leaf];
+    const int32_t end = tv.leaf_off[leaf + 1];
+    for (int32_t c0 = beg; c0 < end; c0 += CH) {
+        const int n = min(CH, end - c0);
+        const T w = kb.d[K - 1];
+        const int32_t wo = kLower ? kb.o[K - 1] : 0;
+        uint32_t mask = 0;
+#pragma unroll 8
+        for (int j = 0; j < CH; ++j) {
+            if (j < n) {
+                const Rec<T, D> r = load_rec<T, D>(tv.recs, c0 + j);
+                const T dd = sq_dist<T, D>(q, r.c);
+                const bool acc = kLower ? (dd < w || (dd == w && r.o < wo)) : (dd < w);
+                if (acc) {
+                    mask |= 1u << j;
+                    st_d[j * kTreeBlockStride] = dd;
+                    if constexpr (kLower) st_o[j * kTreeBlockStride] = r.o;
+                }
+            }
+        }
+        while (mask) {
+            const int j = __builtin_ctz(mask);
+            mask &= mask - 1u;
+            const T dd = st_d[j * kTreeBlockStride];
+            const int32_t o = kLower ? st_o[j * kTreeBlockStride] : 0;
+            if (kb.accepts(dd, o)) kb.insert(dd, c0 + j, o);
+        }
+    }
+}
+
+template <int D>
+__device__ __forceinline__ int axis_of_level(int l) { return l % D; }
 
 // The search of one query; `kb` must be initialised.
 //
@@ -161,14 +208,117 @@ __device__ __forceinline__ int ax_prev(int a) { return a == 0 ? D - 1 : a - 1; }
 // bound is >= worst, the lane skips it and applies the only lasting effect the reference's visit
 // would have had: pp[a] = q[a] for the axes of the levels below (the reference always walks the
 // near path down to the leaf-parent level and resets each axis it leaves).
-template <typename T, int D, bool kLower, typename List>
-__device__ __forceinline__ void knn_search_one(const TreeView<T>& tv, const T (&q)[D], List& kb) {
-    T pp[D], bo[D];
+//
+// Climbing: coming back up, a level whose far child is on the path (bit set in far_mask) is
+// simply left (its axis is reset in pp and the box offset of that axis reverts to the deepest
+// far split of the same axis still on the path); only levels entered through their near child
+// need a far-side test.  So the climb jumps from one such level to the next with bit scans and
+// handles the far levels in between in one step.
+template <typename T, int D, int K, bool kLower, int CH>
+__device__ __forceinline__ void knn_search_staged(const TreeView<T>& tv, const Pt<T, D>& q, KBest<T, K, kLower>& kb,
+                                                  T* __restrict__ st_d, int32_t* __restrict__ st_o) {
+    Pt<T, D> pp = q;
+    Pt<T, D> bo;
 #pragma unroll
-    for (int i = 0; i < D; ++i) {
-        pp[i] = q[i];
-        bo[i] = T(0);
+    for (int i = 0; i < D; ++i) bo[i] = T(0);
+    uint32_t far_mask = 0;
+    const int L = tv.levels;
+    const int32_t nn = tv.n_nodes;
+    const int a_lp = axis_of_level<D>(L - 1);  // split axis of every leaf-parent
+    int32_t node = 0;
+    int level = 0;
+    {
+        int a = 0;
+        while (level < L - 1) {
+            node = 2 * node + (ax_get(q, a) < tv.split[node] ? 1 : 2);
+            ++level;
+            a = ax_next<D>(a);
+        }
     }
+    for (;;) {
+        // leaf pair of leaf-parent `node`
+        {
+            const T m = tv.split[node];
+            const T qa = ax_get(q, a_lp);
+            const bool lower = qa < m;
+            const int32_t left_leaf = 2 * node + 1 - nn;
+            scan_leaf_staged<T, D, K, kLower, CH>(tv, lower ? left_leaf : left_leaf + 1, q, kb, st_d, st_o);
+            if (far_side<T, D, kLower>(q, pp, bo, a_lp, m, kb.worst()) == 1)
+                scan_leaf_staged<T, D, K, kLower, CH>(tv, lower ? left_leaf + 1 : left_leaf, q, kb, st_d, st_o);
+            if (level == 0) return;
+            ax_set(pp, a_lp, qa);
+        }
+        // climb from the leaf-parent
+        bool entered = false;
+        int l = level;
+        for (;;) {
+            const uint32_t below = (1u << l) - 1u;
+            const uint32_t near_lv = below & ~far_mask;
+            const int j = near_lv ? 31 - __builtin_clz(near_lv) : -1;
+            const uint32_t popped = far_mask & below & ~(j >= 0 ? ((2u << j) - 1u) : 0u);
+            if (popped) {
+                far_mask &= ~popped;
+#pragma unroll
+                for (int ax = 0; ax < D; ++ax) {
+                    const uint32_t lv = axis_levels<D>(ax);
+                    if (popped & lv) {
+                        pp[ax] = q[ax];
+                        const uint32_t rem = far_mask & lv;
+                        T off = T(0);
+                        if (rem) {
+                            const int lp = 31 - __builtin_clz(rem);
+                            off = tv.split[((node + 1) >> (level - lp)) - 1] - q[ax];
+                        }
+                        bo[ax] = off;
+                    }
+                }
+            }
+            if (j < 0) break;
+            const int32_t nj = ((node + 1) >> (level - j)) - 1;
+            const int aj = axis_of_level<D>(j);
+            const T mj = tv.split[nj];
+            const T qj = ax_get(q, aj);
+            ax_set(pp, aj, mj);
+            const int f = far_side<T, D, kLower>(q, pp, bo, aj, mj, kb.worst());
+            if (f == 1) {
+                far_mask |= 1u << j;
+                ax_set(bo, aj, T(mj - qj));
+                node = 2 * nj + (qj < mj ? 2 : 1);
+                level = j + 1;
+                entered = true;
+                break;
+            }
+            if (f == 2) {
+                // the reference's pass over this side resets pp on the axes of the next
+                // min(D, L-1-j) levels
+                const int nlev = min(D, L - 1 - j);
+#pragma unroll
+                for (int i = 0; i < D; ++i) {
+                    const int off = (i - aj - 1 + 2 * D) % D;
+                    if (off < nlev) pp[i] = q[i];
+                }
+            }
+            ax_set(pp, aj, qj);
+            if (j == 0) break;
+            l = j;
+        }
+        if (!entered) return;
+        int a = axis_of_level<D>(level);
+        while (level < L - 1) {
+            node = 2 * node + (ax_get(q, a) < tv.split[node] ? 1 : 2);
+            ++level;
+            a = ax_next<D>(a);
+        }
+    }
+}
+
+// Reference-order search with a generic list (used with the LDS list for large k).
+template <typename T, int D, bool kLower, typename List>
+__device__ __forceinline__ void knn_search_one(const TreeView<T>& tv, const Pt<T, D>& q, List& kb) {
+    Pt<T, D> pp = q;
+    Pt<T, D> bo;
+#pragma unroll
+    for (int i = 0; i < D; ++i) bo[i] = T(0);
     uint32_t far_mask = 0;
     const int L = tv.levels;
     const int32_t nn = tv.n_nodes;
@@ -219,7 +369,6 @@ __device__ __forceinline__ void knn_search_one(const TreeView<T>& tv, const T (&
                         break;
                     }
                     if (f == 2) {
-                        // axis i is reset iff it belongs to one of the next min(D, L-1-level) levels
                         const int nlev = min(D, L - 1 - level);
 #pragma unroll
                         for (int i = 0; i < D; ++i) {
@@ -253,26 +402,33 @@ __device__ __forceinline__ void knn_search_one(const TreeView<T>& tv, const T (&
 }
 
 template <typename T, int D>
-__device__ __forceinline__ void load_query(const T* __restrict__ queries, int64_t qi, T (&q)[D]) {
+__device__ __forceinline__ void load_query(const T* __restrict__ queries, int64_t qi, Pt<T, D>& q) {
 #pragma unroll
     for (int i = 0; i < D; ++i) q[i] = queries[qi * D + i];
 }
 
-constexpr int kTreeBlock = 256;
+template <typename T, int K, bool kLower>
+struct StageBytes {
+    static constexpr int kChunk = sizeof(T) == 4 ? 32 : 16;
+    static constexpr int kBytes = kChunk * kTreeBlock * ((int)sizeof(T) + (kLower ? 4 : 0));
+};
 
 template <typename T, int D, int K, bool kLower>
 __global__ __launch_bounds__(kTreeBlock) void knn_tree_kernel(TreeView<T> tv, const T* __restrict__ queries,
                                                                int64_t nq, const int32_t* __restrict__ order, int k,
                                                                T* __restrict__ out_d, int32_t* __restrict__ out_i,
                                                                int64_t* __restrict__ out_o) {
+    constexpr int CH = StageBytes<T, K, kLower>::kChunk;
+    __shared__ T stage_d[CH * kTreeBlock];
+    __shared__ int32_t stage_o[kLower ? CH * kTreeBlock : 1];
     const int64_t t = (int64_t)blockIdx.x * kTreeBlock + threadIdx.x;
     if (t >= nq) return;
     const int64_t qi = order ? (int64_t)order[t] : t;
-    T q[D];
+    Pt<T, D> q;
     load_query<T, D>(queries, qi, q);
     KBest<T, K, kLower> kb;
     kb.init(k);
-    knn_search_one<T, D, kLower>(tv, q, kb);
+    knn_search_staged<T, D, K, kLower, CH>(tv, q, kb, stage_d + threadIdx.x, stage_o + (kLower ? threadIdx.x : 0));
     const int pad = K - k;
     T* od = out_d + qi * k;
     int32_t* oi = out_i + qi * k;
@@ -314,7 +470,7 @@ __global__ __launch_bounds__(kLdsBlock) void knn_tree_lds_kernel(TreeView<T> tv,
     int32_t* so = ss + (size_t)k * kLdsBlock;
     KBestLds<T, kLower> kb{sd + threadIdx.x, ss + threadIdx.x, so + threadIdx.x, k, kLdsBlock, T(0), 0};
     kb.init();
-    T q[D];
+    Pt<T, D> q;
     load_query<T, D>(queries, qi, q);
     knn_search_one<T, D, kLower>(tv, q, kb);
     for (int j = 0; j < k; ++j) {
@@ -344,7 +500,7 @@ __global__ __launch_bounds__(kPresortBlock) void leaf_bucket_kernel(TreeView<T> 
                                                                      int32_t* __restrict__ counts) {
     const int64_t i = (int64_t)blockIdx.x * kPresortBlock + threadIdx.x;
     if (i >= nq) return;
-    T q[D];
+    Pt<T, D> q;
     load_query<T, D>(queries, i, q);
     int32_t node = 0;
     int a = 0;
