@@ -381,13 +381,15 @@ void query_gpu(const Tree& t, const void* queries, int64_t nq, int k, void* dist
         check_launch();
         order = ord;
     }
+    int per_wave = o.queries_per_wave > 0 ? o.queries_per_wave : kDefaultQueriesPerWave;
+    if (per_wave % 64 != 0 || per_wave > (1 << 20)) throw std::runtime_error("queries_per_wave must be a multiple of 64");
     ProfScope pt("traverse", stream);
     if (t.dtype == kF32)
         launch_tree_search_f32(t, static_cast<const float*>(queries), nq, order, k, static_cast<float*>(dist), idx, orig,
-                               o.tie, stream);
+                               o.tie, per_wave, stream);
     else
         launch_tree_search_f64(t, static_cast<const double*>(queries), nq, order, k, static_cast<double*>(dist), idx,
-                               orig, o.tie, stream);
+                               orig, o.tie, per_wave, stream);
 }
 
 }  // namespace tk

@@ -31,7 +31,16 @@ struct TreeView {
 };
 
 // Threads per block of the traversal kernel; per-lane LDS staging slots are strided by it.
-constexpr int kTreeBlock = 256;
+#ifndef TK_TREE_BLOCK
+#define TK_TREE_BLOCK 128
+#endif
+#ifndef TK_STAGE_CHUNK_F32
+#define TK_STAGE_CHUNK_F32 16
+#endif
+#ifndef TK_SCAN_UNROLL
+#define TK_SCAN_UNROLL 8
+#endif
+constexpr int kTreeBlock = TK_TREE_BLOCK;
 constexpr int kTreeBlockStride = kTreeBlock;
 
 // k-best list held in LDS for k beyond the register buckets: slot j of lane t lives at
@@ -174,7 +183,7 @@ __device__ __forceinline__ void scan_leaf_staged(const TreeView<T>& tv, int32_t 
         const T w = kb.d[K - 1];
         const int32_t wo = kLower ? kb.o[K - 1] : 0;
         uint32_t mask = 0;
-#pragma unroll 8
+#pragma unroll TK_SCAN_UNROLL
         for (int j = 0; j < CH; ++j) {
             if (j < n) {
                 const Rec<T, D> r = load_rec<T, D>(tv.recs, c0 + j);
@@ -214,42 +223,55 @@ __device__ __forceinline__ int axis_of_level(int l) { return l % D; }
 // far split of the same axis still on the path); only levels entered through their near child
 // need a far-side test.  So the climb jumps from one such level to the next with bit scans and
 // handles the far levels in between in one step.
-template <typename T, int D, int K, bool kLower, int CH>
-__device__ __forceinline__ void knn_search_staged(const TreeView<T>& tv, const Pt<T, D>& q, KBest<T, K, kLower>& kb,
-                                                  T* __restrict__ st_d, int32_t* __restrict__ st_o) {
-    Pt<T, D> pp = q;
+template <typename T, int D, int K, bool kLower>
+struct LaneSearch {
+    Pt<T, D> q;
+    Pt<T, D> pp;
     Pt<T, D> bo;
-#pragma unroll
-    for (int i = 0; i < D; ++i) bo[i] = T(0);
-    uint32_t far_mask = 0;
-    const int L = tv.levels;
-    const int32_t nn = tv.n_nodes;
-    const int a_lp = axis_of_level<D>(L - 1);  // split axis of every leaf-parent
-    int32_t node = 0;
-    int level = 0;
-    {
-        int a = 0;
-        while (level < L - 1) {
+    KBest<T, K, kLower> kb;
+    uint32_t far_mask;
+    int32_t node;
+    int level;
+
+    // Walks the near path from `node` down to the leaf-parent level.
+    __device__ __forceinline__ void descend(const TreeView<T>& tv) {
+        int a = axis_of_level<D>(level);
+        while (level < tv.levels - 1) {
             node = 2 * node + (ax_get(q, a) < tv.split[node] ? 1 : 2);
             ++level;
             a = ax_next<D>(a);
         }
     }
-    for (;;) {
-        // leaf pair of leaf-parent `node`
+
+    // New search for the query already in q.
+    __device__ __forceinline__ void start(const TreeView<T>& tv, int k) {
+        pp = q;
+#pragma unroll
+        for (int i = 0; i < D; ++i) bo[i] = T(0);
+        kb.init(k);
+        far_mask = 0;
+        node = 0;
+        level = 0;
+        descend(tv);
+    }
+
+    // Scans the leaf pair of the current leaf-parent, then climbs to the next far side to enter
+    // and walks down to its leaf-parent.  Returns true when the search is complete.
+    template <int CH>
+    __device__ __forceinline__ bool step(const TreeView<T>& tv, T* __restrict__ st_d, int32_t* __restrict__ st_o) {
+        const int L = tv.levels;
+        const int a_lp = axis_of_level<D>(L - 1);  // split axis of every leaf-parent
         {
             const T m = tv.split[node];
             const T qa = ax_get(q, a_lp);
             const bool lower = qa < m;
-            const int32_t left_leaf = 2 * node + 1 - nn;
+            const int32_t left_leaf = 2 * node + 1 - tv.n_nodes;
             scan_leaf_staged<T, D, K, kLower, CH>(tv, lower ? left_leaf : left_leaf + 1, q, kb, st_d, st_o);
             if (far_side<T, D, kLower>(q, pp, bo, a_lp, m, kb.worst()) == 1)
                 scan_leaf_staged<T, D, K, kLower, CH>(tv, lower ? left_leaf + 1 : left_leaf, q, kb, st_d, st_o);
-            if (level == 0) return;
+            if (level == 0) return true;
             ax_set(pp, a_lp, qa);
         }
-        // climb from the leaf-parent
-        bool entered = false;
         int l = level;
         for (;;) {
             const uint32_t below = (1u << l) - 1u;
@@ -273,7 +295,7 @@ __device__ __forceinline__ void knn_search_staged(const TreeView<T>& tv, const P
                     }
                 }
             }
-            if (j < 0) break;
+            if (j < 0) return true;
             const int32_t nj = ((node + 1) >> (level - j)) - 1;
             const int aj = axis_of_level<D>(j);
             const T mj = tv.split[nj];
@@ -285,8 +307,8 @@ __device__ __forceinline__ void knn_search_staged(const TreeView<T>& tv, const P
                 ax_set(bo, aj, T(mj - qj));
                 node = 2 * nj + (qj < mj ? 2 : 1);
                 level = j + 1;
-                entered = true;
-                break;
+                descend(tv);
+                return false;
             }
             if (f == 2) {
                 // the reference's pass over this side resets pp on the axes of the next
@@ -299,18 +321,11 @@ __device__ __forceinline__ void knn_search_staged(const TreeView<T>& tv, const P
                 }
             }
             ax_set(pp, aj, qj);
-            if (j == 0) break;
+            if (j == 0) return true;
             l = j;
         }
-        if (!entered) return;
-        int a = axis_of_level<D>(level);
-        while (level < L - 1) {
-            node = 2 * node + (ax_get(q, a) < tv.split[node] ? 1 : 2);
-            ++level;
-            a = ax_next<D>(a);
-        }
     }
-}
+};
 
 // Reference-order search with a generic list (used with the LDS list for large k).
 template <typename T, int D, bool kLower, typename List>
@@ -409,47 +424,82 @@ __device__ __forceinline__ void load_query(const T* __restrict__ queries, int64_
 
 template <typename T, int K, bool kLower>
 struct StageBytes {
-    static constexpr int kChunk = sizeof(T) == 4 ? 32 : 16;
+    static constexpr int kChunk = sizeof(T) == 4 ? TK_STAGE_CHUNK_F32 : 16;
     static constexpr int kBytes = kChunk * kTreeBlock * ((int)sizeof(T) + (kLower ? 4 : 0));
 };
 
+// One lane per query, refilled: wavefront w owns the presorted queries
+// [w * per_wave, (w + 1) * per_wave); a lane whose search completes writes its results and takes
+// the next query of that run (ballot + prefix count), so the lanes stay busy until the run is
+// used up and neighbouring lanes keep working on nearby queries.
 template <typename T, int D, int K, bool kLower>
 __global__ __launch_bounds__(kTreeBlock) void knn_tree_kernel(TreeView<T> tv, const T* __restrict__ queries,
                                                                int64_t nq, const int32_t* __restrict__ order, int k,
                                                                T* __restrict__ out_d, int32_t* __restrict__ out_i,
-                                                               int64_t* __restrict__ out_o) {
+                                                               int64_t* __restrict__ out_o, int per_wave) {
     constexpr int CH = StageBytes<T, K, kLower>::kChunk;
     __shared__ T stage_d[CH * kTreeBlock];
     __shared__ int32_t stage_o[kLower ? CH * kTreeBlock : 1];
-    const int64_t t = (int64_t)blockIdx.x * kTreeBlock + threadIdx.x;
-    if (t >= nq) return;
-    const int64_t qi = order ? (int64_t)order[t] : t;
-    Pt<T, D> q;
-    load_query<T, D>(queries, qi, q);
-    KBest<T, K, kLower> kb;
-    kb.init(k);
-    knn_search_staged<T, D, K, kLower, CH>(tv, q, kb, stage_d + threadIdx.x, stage_o + (kLower ? threadIdx.x : 0));
-    const int pad = K - k;
-    T* od = out_d + qi * k;
-    int32_t* oi = out_i + qi * k;
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-        if (j >= pad) {
-            od[j - pad] = kb.d[j];
-            oi[j - pad] = kb.s[j];
-        }
+    T* st_d = stage_d + threadIdx.x;
+    int32_t* st_o = stage_o + (kLower ? threadIdx.x : 0);
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * kTreeBlock + threadIdx.x) >> 6;
+    const int64_t base = wave * per_wave;
+    if (base >= nq) return;  // whole wavefront
+    const int64_t lim = min(nq, base + (int64_t)per_wave);
+    int64_t next = base + 64;
+    int64_t t = base + lane;
+    bool live = t < lim;
+    int64_t qi = 0;
+    LaneSearch<T, D, K, kLower> ls;
+    if (live) {
+        qi = order ? (int64_t)order[t] : t;
+        load_query<T, D>(queries, qi, ls.q);
+        ls.start(tv, k);
     }
-    if (out_o) {
-        int64_t* oo = out_o + qi * k;
+    const int pad = K - k;
+    for (;;) {
+        bool done = false;
+        if (live) done = ls.template step<CH>(tv, st_d, st_o);
+        if (done) {
+            T* od = out_d + qi * k;
+            int32_t* oi = out_i + qi * k;
 #pragma unroll
-        for (int j = 0; j < K; ++j) {
-            if (j >= pad) {
-                if constexpr (kLower)
-                    oo[j - pad] = kb.o[j];
-                else
-                    oo[j - pad] = tv.shuffled[kb.s[j]];
+            for (int j = 0; j < K; ++j) {
+                if (j >= pad) {
+                    od[j - pad] = ls.kb.d[j];
+                    oi[j - pad] = ls.kb.s[j];
+                }
+            }
+            if (out_o) {
+                int64_t* oo = out_o + qi * k;
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    if (j >= pad) {
+                        if constexpr (kLower)
+                            oo[j - pad] = ls.kb.o[j];
+                        else
+                            oo[j - pad] = tv.shuffled[ls.kb.s[j]];
+                    }
+                }
             }
         }
+        const unsigned long long dm = __ballot(done);
+        if (dm) {
+            const int64_t mine = next + __popcll(dm & ((1ull << lane) - 1ull));
+            next += __popcll(dm);
+            if (done) {
+                if (mine < lim) {
+                    t = mine;
+                    qi = order ? (int64_t)order[t] : t;
+                    load_query<T, D>(queries, qi, ls.q);
+                    ls.start(tv, k);
+                } else {
+                    live = false;
+                }
+            }
+        }
+        if (__ballot(live) == 0ull) break;
     }
 }
 
