@@ -78,11 +78,6 @@ int tk_tree_upload(tk_tree* tree, int device);
  * (src/kdtree_g.cu:140-157).  Frees host and device copies; NULL is ignored. */
 void tk_tree_free(tk_tree* tree);
 
-/* GPU tuning of a tree's queries (no counterpart in the reference): how many presorted queries
- * each wavefront of the traversal kernel works through, refilling lanes as their searches end.
- * 0 = library default (256); otherwise a multiple of 64.  Never changes results. */
-int tk_tree_set_queries_per_wave(tk_tree* tree, int queries_per_wave);
-
 /* Tree facts: out[0]=dims, out[1]=dtype, out[2]=levels, out[3]=n_points, out[4]=device
  * (-1 when not uploaded). */
 int tk_tree_info(const tk_tree* tree, int32_t out[5]);

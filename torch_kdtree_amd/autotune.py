@@ -19,7 +19,7 @@ import torch
 _CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 # Target points per leaf when no measurement is cached.
-DEFAULT_LEAF = {1: 8, 2: 8, 3: 8, 8: 32}
+DEFAULT_LEAF = {1: 16, 2: 16, 3: 32, 8: 32}
 
 
 def _levels_for_leaf(n: int, leaf: float) -> int:

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(kBfBlock) void bf_kernel(const T* __restrict__ ref,
 #pragma unroll
                 for (int i = 0; i < D; ++i) r[i] = tile[j * D + i];
                 const T dd = sq_dist<T, D>(q, r);
-                if (dd < kb.worst()) kb.insert(dd, t0 + j, 0);
+                if (dd < kb.worst()) kb.insert(dd, t0 + j);
             }
         }
     }
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(kBfBlock) void bf_kernel(const T* __restrict__ ref,
     for (int j = 0; j < K; ++j) {
         if (j >= pad) {
             out_d[row + j - pad] = kb.d[j];
-            out_i[row + j - pad] = kb.s[j];
+            out_i[row + j - pad] = kb.x[j];
         }
     }
 }
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(256) void bf_merge_kernel(const T* __restrict__ par
         for (int j = 0; j < k; ++j) {
             const T dd = part_d[row + j];
             if (!(dd < kb.worst())) break;  // the chunk list is sorted: nothing further enters
-            kb.insert(dd, part_i[row + j], 0);
+            kb.insert(dd, part_i[row + j]);
         }
     }
     const int pad = K - k;
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void bf_merge_kernel(const T* __restrict__ par
     for (int j = 0; j < K; ++j) {
         if (j >= pad) {
             out_d[qi * k + j - pad] = kb.d[j];
-            out_i[qi * k + j - pad] = kb.s[j];
+            out_i[qi * k + j - pad] = kb.x[j];
         }
     }
 }

@@ -86,15 +86,6 @@ void tk_tree_free(tk_tree* tree) {
     delete tree;
 }
 
-int tk_tree_set_queries_per_wave(tk_tree* tree, int queries_per_wave) {
-    return guarded([&] {
-        if (!tree || !tree->t) throw std::runtime_error("null tree handle");
-        if (queries_per_wave < 0 || queries_per_wave % 64 != 0 || queries_per_wave > (1 << 20))
-            throw std::runtime_error("queries_per_wave must be 0 (default) or a multiple of 64 up to 2^20");
-        tree->t->queries_per_wave = queries_per_wave;
-    });
-}
-
 int tk_tree_info(const tk_tree* tree, int32_t out[5]) {
     return guarded([&] {
         const tk::Tree& t = tree_of(tree);
@@ -146,7 +137,6 @@ int tk_knn_query_gpu(const tk_tree* tree, const void* queries, int64_t n_queries
         tk::QueryOpts o;
         o.tie = tie;
         o.presort = presort;
-        o.queries_per_wave = t.queries_per_wave;
         tk::query_gpu(t, queries, n_queries, k, dist, idx, orig_idx, o, static_cast<hipStream_t>(stream), workspace,
                       workspace_bytes);
     });

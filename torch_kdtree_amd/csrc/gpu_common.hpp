@@ -121,61 +121,57 @@ __device__ __forceinline__ double pos_inf<double>() { return __builtin_huge_val(
 // slots hold -inf and never move, so the live list is slots K-k .. K-1 and the worst entry is
 // always slot K-1.  Fully unrolled, branch-free compare-and-shift.
 //
-// kLower == false (reference policy, src/nndistance.hpp:41-85): insert iff d < worst, at the
-//   upper-bound position (after every equal distance already present).
-// kLower == true (lower-index policy): the list is ordered by (d, original index).
+// Each entry is (distance, key).
+// kLower == false (reference policy, src/nndistance.hpp:41-85): key = structured index; insert
+//   iff d < worst, at the upper-bound position (after every equal distance already present).
+// kLower == true (lower-index policy): key = original index; the list is ordered by (d, key).
 // ---------------------------------------------------------------------------------------------
 template <typename T, int K, bool kLower>
 struct KBest {
     T d[K];
-    int32_t s[K];                 // structured index
-    int32_t o[kLower ? K : 1];    // original index (lower-index policy only)
+    int32_t x[K];
 
     __device__ __forceinline__ void init(int k) {
         const int pad = K - k;
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             d[j] = j < pad ? -pos_inf<T>() : pos_inf<T>();
-            s[j] = 0;
-            if constexpr (kLower) o[j] = j < pad ? -1 : 0x7fffffff;
+            x[j] = kLower ? (j < pad ? -1 : 0x7fffffff) : 0;
         }
     }
 
     __device__ __forceinline__ T worst() const { return d[K - 1]; }
 
-    __device__ __forceinline__ bool accepts(T dist, int32_t orig) const {
+    __device__ __forceinline__ bool accepts(T dist, int32_t key) const {
         if constexpr (kLower)
-            return dist < d[K - 1] || (dist == d[K - 1] && orig < o[K - 1]);
+            return dist < d[K - 1] || (dist == d[K - 1] && key < x[K - 1]);
         else
             return dist < d[K - 1];
     }
 
-    __device__ __forceinline__ bool after(int j, T dist, int32_t orig) const {
+    __device__ __forceinline__ bool after(int j, T dist, int32_t key) const {
         if constexpr (kLower)
-            return d[j] > dist || (d[j] == dist && o[j] > orig);
+            return d[j] > dist || (d[j] == dist && x[j] > key);
         else
             return d[j] > dist;
     }
 
-    // Sorted insert of an accepted candidate (dist < d[K-1], or equal with a lower index under
-    // the lower-index policy).  With p the insert position, slot j becomes d[j-1] (j > p),
-    // dist (j == p) or d[j] (j < p); for a sorted list that value is the median of
-    // (d[j-1], d[j], dist) whatever p is, so distances need one med3 per slot and only the
-    // index arrays need the position masks.
-    __device__ __forceinline__ void insert(T dist, int32_t sidx, int32_t orig) {
-        bool here_next = after(K - 1, dist, orig);
+    // Sorted insert of an accepted candidate.  With p the insert position, slot j becomes d[j-1]
+    // (j > p), dist (j == p) or d[j] (j < p); for a sorted list that value is the median of
+    // (d[j-1], d[j], dist) whatever p is, so distances need one med3 per slot and only the keys
+    // need the position masks.
+    __device__ __forceinline__ void insert(T dist, int32_t key) {
+        bool here_next = after(K - 1, dist, key);
 #pragma unroll
         for (int j = K - 1; j > 0; --j) {
-            const bool shift = after(j - 1, dist, orig);
+            const bool shift = after(j - 1, dist, key);
             const bool here = here_next;
             here_next = shift;
             d[j] = med3(d[j - 1], d[j], dist);
-            s[j] = shift ? s[j - 1] : (here ? sidx : s[j]);
-            if constexpr (kLower) o[j] = shift ? o[j - 1] : (here ? orig : o[j]);
+            x[j] = shift ? x[j - 1] : (here ? key : x[j]);
         }
         d[0] = here_next ? dist : d[0];
-        s[0] = here_next ? sidx : s[0];
-        if constexpr (kLower) o[0] = here_next ? orig : o[0];
+        x[0] = here_next ? key : x[0];
     }
 
     static __device__ __forceinline__ T med3(T a, T b, T c) {

@@ -116,6 +116,7 @@ void free_device_tree(Tree& t) noexcept {
     (void)hipFree(d->recs);
     (void)hipFree(d->leaf_off);
     (void)hipFree(d->shuffled);
+    (void)hipFree(d->inv_shuffled);
     if (prev >= 0) (void)hipSetDevice(prev);
     delete d;
     t.dev = nullptr;
@@ -132,6 +133,8 @@ void upload_tree(Tree& t, int device) {
     const size_t coord_bytes = ts * (size_t)t.dims;
     const size_t n = (size_t)t.n_points;
     std::vector<unsigned char> recs(n * (size_t)d->rec_bytes, 0);
+    std::vector<int32_t> inv(n);
+    for (size_t s = 0; s < n; ++s) inv[(size_t)t.shuffled[s]] = (int32_t)s;
     for (size_t s = 0; s < n; ++s) {
         unsigned char* r = &recs[s * (size_t)d->rec_bytes];
         std::memcpy(r, &t.points[s * coord_bytes], coord_bytes);
@@ -142,16 +145,19 @@ void upload_tree(Tree& t, int device) {
         TK_HIP_CHECK(hipMalloc((void**)&d->recs, recs.size()));
         TK_HIP_CHECK(hipMalloc((void**)&d->leaf_off, sizeof(int32_t) * t.leaf_off.size()));
         TK_HIP_CHECK(hipMalloc((void**)&d->shuffled, sizeof(int32_t) * n));
+        TK_HIP_CHECK(hipMalloc((void**)&d->inv_shuffled, sizeof(int32_t) * n));
         TK_HIP_CHECK(hipMemcpy(d->split, t.split.data(), (size_t)t.n_nodes * ts, hipMemcpyHostToDevice));
         TK_HIP_CHECK(hipMemcpy(d->recs, recs.data(), recs.size(), hipMemcpyHostToDevice));
         TK_HIP_CHECK(hipMemcpy(d->leaf_off, t.leaf_off.data(), sizeof(int32_t) * t.leaf_off.size(),
                                hipMemcpyHostToDevice));
         TK_HIP_CHECK(hipMemcpy(d->shuffled, t.shuffled.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+        TK_HIP_CHECK(hipMemcpy(d->inv_shuffled, inv.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
     } catch (...) {
         (void)hipFree(d->split);
         (void)hipFree(d->recs);
         (void)hipFree(d->leaf_off);
         (void)hipFree(d->shuffled);
+        (void)hipFree(d->inv_shuffled);
         throw;
     }
     t.dev = d.release();
@@ -381,15 +387,13 @@ void query_gpu(const Tree& t, const void* queries, int64_t nq, int k, void* dist
         check_launch();
         order = ord;
     }
-    int per_wave = o.queries_per_wave > 0 ? o.queries_per_wave : kDefaultQueriesPerWave;
-    if (per_wave % 64 != 0 || per_wave > (1 << 20)) throw std::runtime_error("queries_per_wave must be a multiple of 64");
     ProfScope pt("traverse", stream);
     if (t.dtype == kF32)
         launch_tree_search_f32(t, static_cast<const float*>(queries), nq, order, k, static_cast<float*>(dist), idx, orig,
-                               o.tie, per_wave, stream);
+                               o.tie, stream);
     else
         launch_tree_search_f64(t, static_cast<const double*>(queries), nq, order, k, static_cast<double*>(dist), idx,
-                               orig, o.tie, per_wave, stream);
+                               orig, o.tie, stream);
 }
 
 }  // namespace tk

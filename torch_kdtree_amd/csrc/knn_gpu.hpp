@@ -16,7 +16,8 @@ struct DeviceTree {
     void* split = nullptr;            // T[n_nodes]
     unsigned char* recs = nullptr;    // records, RecLayout<T, dims>::kBytes each
     int32_t* leaf_off = nullptr;      // [n_leaves + 1]
-    int32_t* shuffled = nullptr;      // [n_points]
+    int32_t* shuffled = nullptr;      // [n_points] original index of each structured position
+    int32_t* inv_shuffled = nullptr;  // [n_points] structured position of each original index
     int rec_bytes = 0;
 };
 
@@ -31,13 +32,9 @@ enum PresortMode : int { kPresortAuto = -1, kPresortOff = 0, kPresortMorton = 1,
 // Largest k served on the GPU (the reference's limit, src/kdtree_g.cu:283,421-422).
 constexpr int kMaxGpuK = 128;
 
-// Queries a wavefront of the traversal kernel works through, refilling lanes as searches finish.
-constexpr int kDefaultQueriesPerWave = 256;
-
 struct QueryOpts {
     int tie = kTieReference;
     int presort = kPresortAuto;
-    int queries_per_wave = 0;  // 0 = kDefaultQueriesPerWave; else a multiple of 64
 };
 
 // Workspace (bytes) the query needs for the presort; 0 when no presort runs.
@@ -50,9 +47,9 @@ void query_gpu(const Tree& t, const void* queries, int64_t nq, int k, void* dist
 
 // Per-dtype kernel launchers (knn_tree_f32.hip / knn_tree_f64.hip).
 void launch_tree_search_f32(const Tree& t, const float* q, int64_t nq, const int32_t* order, int k, float* dist,
-                            int32_t* idx, int64_t* orig, int tie, int per_wave, hipStream_t s);
+                            int32_t* idx, int64_t* orig, int tie, hipStream_t s);
 void launch_tree_search_f64(const Tree& t, const double* q, int64_t nq, const int32_t* order, int k, double* dist,
-                            int32_t* idx, int64_t* orig, int tie, int per_wave, hipStream_t s);
+                            int32_t* idx, int64_t* orig, int tie, hipStream_t s);
 void launch_leaf_bucket_f32(const Tree& t, const float* q, int64_t nq, int shift, int32_t* bucket, int32_t* rank,
                             int32_t* counts, hipStream_t s);
 void launch_leaf_bucket_f64(const Tree& t, const double* q, int64_t nq, int shift, int32_t* bucket, int32_t* rank,

@@ -25,7 +25,8 @@ struct TreeView {
     const T* __restrict__ split;            // [n_nodes] heap order
     const unsigned char* __restrict__ recs; // [n_points] records, structured order
     const int32_t* __restrict__ leaf_off;   // [n_leaves + 1]
-    const int32_t* __restrict__ shuffled;   // [n_points]
+    const int32_t* __restrict__ shuffled;   // [n_points] original index of a structured position
+    const int32_t* __restrict__ inv_shuffled; // [n_points] structured position of an original index
     int levels;
     int32_t n_nodes;
 };
@@ -35,7 +36,7 @@ struct TreeView {
 #define TK_TREE_BLOCK 128
 #endif
 #ifndef TK_STAGE_CHUNK_F32
-#define TK_STAGE_CHUNK_F32 16
+#define TK_STAGE_CHUNK_F32 8
 #endif
 #ifndef TK_SCAN_UNROLL
 #define TK_SCAN_UNROLL 8
@@ -147,16 +148,28 @@ __device__ __forceinline__ int far_side(const Pt<T, D>& q, const Pt<T, D>& pp, c
     }
 }
 
+// Run-time axis access.  The element is taken from / put into a copy of the whole vector, so
+// the lane state itself is only ever read and written whole and stays in registers (an
+// element address computed from a run-time index would pin the surrounding object in memory).
 template <typename T, int D>
 __device__ __forceinline__ T ax_get(const Pt<T, D>& v, int a) {
-    if constexpr (D == 1) return v[0];
-    else return v[a];
+    if constexpr (D == 1) {
+        return v[0];
+    } else {
+        const Pt<T, D> c = v;
+        return c[a];
+    }
 }
 
 template <typename T, int D>
 __device__ __forceinline__ void ax_set(Pt<T, D>& v, int a, T x) {
-    if constexpr (D == 1) v[0] = x;
-    else v[a] = x;
+    if constexpr (D == 1) {
+        v[0] = x;
+    } else {
+        Pt<T, D> c = v;
+        c[a] = x;
+        v = c;
+    }
 }
 
 template <int D>
@@ -181,7 +194,7 @@ __device__ __forceinline__ void scan_leaf_staged(const TreeView<T>& tv, int32_t 
     for (int32_t c0 = beg; c0 < end; c0 += CH) {
         const int n = min(CH, end - c0);
         const T w = kb.d[K - 1];
-        const int32_t wo = kLower ? kb.o[K - 1] : 0;
+        const int32_t wo = kLower ? kb.x[K - 1] : 0;
         uint32_t mask = 0;
 #pragma unroll TK_SCAN_UNROLL
         for (int j = 0; j < CH; ++j) {
@@ -201,7 +214,7 @@ __device__ __forceinline__ void scan_leaf_staged(const TreeView<T>& tv, int32_t 
             mask &= mask - 1u;
             const T dd = st_d[j * kTreeBlockStride];
             const int32_t o = kLower ? st_o[j * kTreeBlockStride] : 0;
-            if (kb.accepts(dd, o)) kb.insert(dd, c0 + j, o);
+            if (kb.accepts(dd, o)) kb.insert(dd, kLower ? o : c0 + j);
         }
     }
 }
@@ -223,109 +236,107 @@ __device__ __forceinline__ int axis_of_level(int l) { return l % D; }
 // far split of the same axis still on the path); only levels entered through their near child
 // need a far-side test.  So the climb jumps from one such level to the next with bit scans and
 // handles the far levels in between in one step.
+// Lane state of one search: q, pp, bo, the k-list, far_mask, node, level.  They are kept as
+// separate variables of the kernel (not one aggregate), so that each can live in registers.
+
+// Walks the near path from `node` down to the leaf-parent level.
+template <typename T, int D>
+__device__ __forceinline__ void lane_descend(const TreeView<T>& tv, const Pt<T, D>& q, int32_t& node, int& level) {
+    int a = axis_of_level<D>(level);
+    while (level < tv.levels - 1) {
+        node = 2 * node + (ax_get(q, a) < tv.split[node] ? 1 : 2);
+        ++level;
+        a = ax_next<D>(a);
+    }
+}
+
+// Starts the search of the query in q.
 template <typename T, int D, int K, bool kLower>
-struct LaneSearch {
-    Pt<T, D> q;
-    Pt<T, D> pp;
-    Pt<T, D> bo;
-    KBest<T, K, kLower> kb;
-    uint32_t far_mask;
-    int32_t node;
-    int level;
-
-    // Walks the near path from `node` down to the leaf-parent level.
-    __device__ __forceinline__ void descend(const TreeView<T>& tv) {
-        int a = axis_of_level<D>(level);
-        while (level < tv.levels - 1) {
-            node = 2 * node + (ax_get(q, a) < tv.split[node] ? 1 : 2);
-            ++level;
-            a = ax_next<D>(a);
-        }
-    }
-
-    // New search for the query already in q.
-    __device__ __forceinline__ void start(const TreeView<T>& tv, int k) {
-        pp = q;
+__device__ __forceinline__ void lane_start(const TreeView<T>& tv, int k, const Pt<T, D>& q, Pt<T, D>& pp,
+                                           Pt<T, D>& bo, KBest<T, K, kLower>& kb, uint32_t& far_mask,
+                                           int32_t& node, int& level) {
+    pp = q;
 #pragma unroll
-        for (int i = 0; i < D; ++i) bo[i] = T(0);
-        kb.init(k);
-        far_mask = 0;
-        node = 0;
-        level = 0;
-        descend(tv);
-    }
+    for (int i = 0; i < D; ++i) bo[i] = T(0);
+    kb.init(k);
+    far_mask = 0;
+    node = 0;
+    level = 0;
+    lane_descend<T, D>(tv, q, node, level);
+}
 
-    // Scans the leaf pair of the current leaf-parent, then climbs to the next far side to enter
-    // and walks down to its leaf-parent.  Returns true when the search is complete.
-    template <int CH>
-    __device__ __forceinline__ bool step(const TreeView<T>& tv, T* __restrict__ st_d, int32_t* __restrict__ st_o) {
-        const int L = tv.levels;
-        const int a_lp = axis_of_level<D>(L - 1);  // split axis of every leaf-parent
-        {
-            const T m = tv.split[node];
-            const T qa = ax_get(q, a_lp);
-            const bool lower = qa < m;
-            const int32_t left_leaf = 2 * node + 1 - tv.n_nodes;
-            scan_leaf_staged<T, D, K, kLower, CH>(tv, lower ? left_leaf : left_leaf + 1, q, kb, st_d, st_o);
-            if (far_side<T, D, kLower>(q, pp, bo, a_lp, m, kb.worst()) == 1)
-                scan_leaf_staged<T, D, K, kLower, CH>(tv, lower ? left_leaf + 1 : left_leaf, q, kb, st_d, st_o);
-            if (level == 0) return true;
-            ax_set(pp, a_lp, qa);
-        }
-        int l = level;
-        for (;;) {
-            const uint32_t below = (1u << l) - 1u;
-            const uint32_t near_lv = below & ~far_mask;
-            const int j = near_lv ? 31 - __builtin_clz(near_lv) : -1;
-            const uint32_t popped = far_mask & below & ~(j >= 0 ? ((2u << j) - 1u) : 0u);
-            if (popped) {
-                far_mask &= ~popped;
+// Scans the leaf pair of the current leaf-parent, then climbs to the next far side to enter and
+// walks down to its leaf-parent.  Returns true when the search is complete.
+template <typename T, int D, int K, bool kLower, int CH>
+__device__ __forceinline__ bool lane_step(const TreeView<T>& tv, const Pt<T, D>& q, Pt<T, D>& pp, Pt<T, D>& bo,
+                                          KBest<T, K, kLower>& kb, uint32_t& far_mask, int32_t& node, int& level,
+                                          T* __restrict__ st_d, int32_t* __restrict__ st_o) {
+    const int L = tv.levels;
+    const int a_lp = axis_of_level<D>(L - 1);  // split axis of every leaf-parent
+    {
+        const T m = tv.split[node];
+        const T qa = ax_get(q, a_lp);
+        const bool lower = qa < m;
+        const int32_t left_leaf = 2 * node + 1 - tv.n_nodes;
+        scan_leaf_staged<T, D, K, kLower, CH>(tv, lower ? left_leaf : left_leaf + 1, q, kb, st_d, st_o);
+        if (far_side<T, D, kLower>(q, pp, bo, a_lp, m, kb.worst()) == 1)
+            scan_leaf_staged<T, D, K, kLower, CH>(tv, lower ? left_leaf + 1 : left_leaf, q, kb, st_d, st_o);
+        if (level == 0) return true;
+        ax_set(pp, a_lp, qa);
+    }
+    int l = level;
+    for (;;) {
+        const uint32_t below = (1u << l) - 1u;
+        const uint32_t near_lv = below & ~far_mask;
+        const int j = near_lv ? 31 - __builtin_clz(near_lv) : -1;
+        const uint32_t popped = far_mask & below & ~(j >= 0 ? ((2u << j) - 1u) : 0u);
+        if (popped) {
+            far_mask &= ~popped;
 #pragma unroll
-                for (int ax = 0; ax < D; ++ax) {
-                    const uint32_t lv = axis_levels<D>(ax);
-                    if (popped & lv) {
-                        pp[ax] = q[ax];
-                        const uint32_t rem = far_mask & lv;
-                        T off = T(0);
-                        if (rem) {
-                            const int lp = 31 - __builtin_clz(rem);
-                            off = tv.split[((node + 1) >> (level - lp)) - 1] - q[ax];
-                        }
-                        bo[ax] = off;
+            for (int ax = 0; ax < D; ++ax) {
+                const uint32_t lv = axis_levels<D>(ax);
+                if (popped & lv) {
+                    pp[ax] = q[ax];
+                    const uint32_t rem = far_mask & lv;
+                    T off = T(0);
+                    if (rem) {
+                        const int lp = 31 - __builtin_clz(rem);
+                        off = tv.split[((node + 1) >> (level - lp)) - 1] - q[ax];
                     }
+                    bo[ax] = off;
                 }
             }
-            if (j < 0) return true;
-            const int32_t nj = ((node + 1) >> (level - j)) - 1;
-            const int aj = axis_of_level<D>(j);
-            const T mj = tv.split[nj];
-            const T qj = ax_get(q, aj);
-            ax_set(pp, aj, mj);
-            const int f = far_side<T, D, kLower>(q, pp, bo, aj, mj, kb.worst());
-            if (f == 1) {
-                far_mask |= 1u << j;
-                ax_set(bo, aj, T(mj - qj));
-                node = 2 * nj + (qj < mj ? 2 : 1);
-                level = j + 1;
-                descend(tv);
-                return false;
-            }
-            if (f == 2) {
-                // the reference's pass over this side resets pp on the axes of the next
-                // min(D, L-1-j) levels
-                const int nlev = min(D, L - 1 - j);
-#pragma unroll
-                for (int i = 0; i < D; ++i) {
-                    const int off = (i - aj - 1 + 2 * D) % D;
-                    if (off < nlev) pp[i] = q[i];
-                }
-            }
-            ax_set(pp, aj, qj);
-            if (j == 0) return true;
-            l = j;
         }
+        if (j < 0) return true;
+        const int32_t nj = ((node + 1) >> (level - j)) - 1;
+        const int aj = axis_of_level<D>(j);
+        const T mj = tv.split[nj];
+        const T qj = ax_get(q, aj);
+        ax_set(pp, aj, mj);
+        const int f = far_side<T, D, kLower>(q, pp, bo, aj, mj, kb.worst());
+        if (f == 1) {
+            far_mask |= 1u << j;
+            ax_set(bo, aj, T(mj - qj));
+            node = 2 * nj + (qj < mj ? 2 : 1);
+            level = j + 1;
+            lane_descend<T, D>(tv, q, node, level);
+            return false;
+        }
+        if (f == 2) {
+            // the reference's pass over this side resets pp on the axes of the next
+            // min(D, L-1-j) levels
+            const int nlev = min(D, L - 1 - j);
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                const int off = (i - aj - 1 + 2 * D) % D;
+                if (off < nlev) pp[i] = q[i];
+            }
+        }
+        ax_set(pp, aj, qj);
+        if (j == 0) return true;
+        l = j;
     }
-};
+}
 
 // Reference-order search with a generic list (used with the LDS list for large k).
 template <typename T, int D, bool kLower, typename List>
@@ -428,78 +439,47 @@ struct StageBytes {
     static constexpr int kBytes = kChunk * kTreeBlock * ((int)sizeof(T) + (kLower ? 4 : 0));
 };
 
-// One lane per query, refilled: wavefront w owns the presorted queries
-// [w * per_wave, (w + 1) * per_wave); a lane whose search completes writes its results and takes
-// the next query of that run (ballot + prefix count), so the lanes stay busy until the run is
-// used up and neighbouring lanes keep working on nearby queries.
+// One lane per query.  The lanes of a wavefront take 64 consecutive presorted queries and run
+// their searches to completion together: neighbouring queries walk the tree nearly in step, which
+// keeps the wavefront's control flow and loads coherent.  (Handing a finished lane the next query
+// was measured to be slower: it breaks that lockstep.)
 template <typename T, int D, int K, bool kLower>
 __global__ __launch_bounds__(kTreeBlock) void knn_tree_kernel(TreeView<T> tv, const T* __restrict__ queries,
                                                                int64_t nq, const int32_t* __restrict__ order, int k,
                                                                T* __restrict__ out_d, int32_t* __restrict__ out_i,
-                                                               int64_t* __restrict__ out_o, int per_wave) {
+                                                               int64_t* __restrict__ out_o) {
     constexpr int CH = StageBytes<T, K, kLower>::kChunk;
     __shared__ T stage_d[CH * kTreeBlock];
     __shared__ int32_t stage_o[kLower ? CH * kTreeBlock : 1];
-    T* st_d = stage_d + threadIdx.x;
-    int32_t* st_o = stage_o + (kLower ? threadIdx.x : 0);
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = ((int64_t)blockIdx.x * kTreeBlock + threadIdx.x) >> 6;
-    const int64_t base = wave * per_wave;
-    if (base >= nq) return;  // whole wavefront
-    const int64_t lim = min(nq, base + (int64_t)per_wave);
-    int64_t next = base + 64;
-    int64_t t = base + lane;
-    bool live = t < lim;
-    int64_t qi = 0;
-    LaneSearch<T, D, K, kLower> ls;
-    if (live) {
-        qi = order ? (int64_t)order[t] : t;
-        load_query<T, D>(queries, qi, ls.q);
-        ls.start(tv, k);
+    const int64_t t = (int64_t)blockIdx.x * kTreeBlock + threadIdx.x;
+    if (t >= nq) return;
+    const int64_t qi = order ? (int64_t)order[t] : t;
+    Pt<T, D> q, pp, bo;
+    KBest<T, K, kLower> kb;
+    uint32_t far_mask;
+    int32_t node;
+    int level;
+    load_query<T, D>(queries, qi, q);
+    lane_start<T, D, K, kLower>(tv, k, q, pp, bo, kb, far_mask, node, level);
+    while (!lane_step<T, D, K, kLower, CH>(tv, q, pp, bo, kb, far_mask, node, level, stage_d + threadIdx.x,
+                                            stage_o + (kLower ? threadIdx.x : 0))) {
     }
     const int pad = K - k;
-    for (;;) {
-        bool done = false;
-        if (live) done = ls.template step<CH>(tv, st_d, st_o);
-        if (done) {
-            T* od = out_d + qi * k;
-            int32_t* oi = out_i + qi * k;
+    T* od = out_d + qi * k;
+    int32_t* oi = out_i + qi * k;
 #pragma unroll
-            for (int j = 0; j < K; ++j) {
-                if (j >= pad) {
-                    od[j - pad] = ls.kb.d[j];
-                    oi[j - pad] = ls.kb.s[j];
-                }
-            }
-            if (out_o) {
-                int64_t* oo = out_o + qi * k;
+    for (int j = 0; j < K; ++j) {
+        if (j >= pad) {
+            od[j - pad] = kb.d[j];
+            oi[j - pad] = kLower ? tv.inv_shuffled[kb.x[j]] : kb.x[j];
+        }
+    }
+    if (out_o) {
+        int64_t* oo = out_o + qi * k;
 #pragma unroll
-                for (int j = 0; j < K; ++j) {
-                    if (j >= pad) {
-                        if constexpr (kLower)
-                            oo[j - pad] = ls.kb.o[j];
-                        else
-                            oo[j - pad] = tv.shuffled[ls.kb.s[j]];
-                    }
-                }
-            }
+        for (int j = 0; j < K; ++j) {
+            if (j >= pad) oo[j - pad] = kLower ? kb.x[j] : tv.shuffled[kb.x[j]];
         }
-        const unsigned long long dm = __ballot(done);
-        if (dm) {
-            const int64_t mine = next + __popcll(dm & ((1ull << lane) - 1ull));
-            next += __popcll(dm);
-            if (done) {
-                if (mine < lim) {
-                    t = mine;
-                    qi = order ? (int64_t)order[t] : t;
-                    load_query<T, D>(queries, qi, ls.q);
-                    ls.start(tv, k);
-                } else {
-                    live = false;
-                }
-            }
-        }
-        if (__ballot(live) == 0ull) break;
     }
 }
 

@@ -6,8 +6,8 @@
 namespace tk {
 
 void launch_tree_search_f32(const Tree& t, const float* q, int64_t nq, const int32_t* order, int k, float* dist,
-                            int32_t* idx, int64_t* orig, int tie, int per_wave, hipStream_t s) {
-    launch_tree_search_t<float>(t, q, nq, order, k, dist, idx, orig, tie, per_wave, s);
+                            int32_t* idx, int64_t* orig, int tie, hipStream_t s) {
+    launch_tree_search_t<float>(t, q, nq, order, k, dist, idx, orig, tie, s);
 }
 
 void launch_leaf_bucket_f32(const Tree& t, const float* q, int64_t nq, int shift, int32_t* bucket, int32_t* rank,

@@ -6,8 +6,8 @@
 namespace tk {
 
 void launch_tree_search_f64(const Tree& t, const double* q, int64_t nq, const int32_t* order, int k, double* dist,
-                            int32_t* idx, int64_t* orig, int tie, int per_wave, hipStream_t s) {
-    launch_tree_search_t<double>(t, q, nq, order, k, dist, idx, orig, tie, per_wave, s);
+                            int32_t* idx, int64_t* orig, int tie, hipStream_t s) {
+    launch_tree_search_t<double>(t, q, nq, order, k, dist, idx, orig, tie, s);
 }
 
 void launch_leaf_bucket_f64(const Tree& t, const double* q, int64_t nq, int shift, int32_t* bucket, int32_t* rank,

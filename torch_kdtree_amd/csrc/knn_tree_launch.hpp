@@ -9,23 +9,21 @@ namespace tk {
 
 template <typename T>
 TreeView<T> tree_view(const Tree& t) {
-    return TreeView<T>{static_cast<const T*>(t.dev->split), t.dev->recs, t.dev->leaf_off, t.dev->shuffled, t.levels,
-                       t.n_nodes};
+    return TreeView<T>{static_cast<const T*>(t.dev->split), t.dev->recs, t.dev->leaf_off, t.dev->shuffled,
+                       t.dev->inv_shuffled, t.levels, t.n_nodes};
 }
 
 template <typename T, int D, bool kLower>
 void launch_tree_td(const TreeView<T>& tv, const T* q, int64_t nq, const int32_t* order, int k, T* dist, int32_t* idx,
-                    int64_t* orig, int per_wave, hipStream_t s) {
-    const int64_t waves = (nq + per_wave - 1) / per_wave;
-    constexpr int kWavesPerBlock = kTreeBlock / 64;
-    const dim3 grid((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock));
+                    int64_t* orig, hipStream_t s) {
+    const dim3 grid((unsigned)((nq + kTreeBlock - 1) / kTreeBlock));
     const dim3 block(kTreeBlock);
     int kb = 1;
     while (kb < k) kb <<= 1;
 #define TK_LAUNCH_K(KK)                                                                                        \
     case KK:                                                                                                   \
         hipLaunchKernelGGL((knn_tree_kernel<T, D, KK, kLower>), grid, block, 0, s, tv, q, nq, order, k, dist, \
-                           idx, orig, per_wave);                                                               \
+                           idx, orig);                                                                         \
         break;
     switch (kb) {
         TK_LAUNCH_K(1)
@@ -49,14 +47,14 @@ void launch_tree_td(const TreeView<T>& tv, const T* q, int64_t nq, const int32_t
 
 template <typename T>
 void launch_tree_search_t(const Tree& t, const T* q, int64_t nq, const int32_t* order, int k, T* dist, int32_t* idx,
-                          int64_t* orig, int tie, int per_wave, hipStream_t s) {
+                          int64_t* orig, int tie, hipStream_t s) {
     const TreeView<T> tv = tree_view<T>(t);
 #define TK_DIMS(DD)                                                            \
     case DD:                                                                   \
         if (tie == kTieLowerIndex)                                             \
-            launch_tree_td<T, DD, true>(tv, q, nq, order, k, dist, idx, orig, per_wave, s); \
+            launch_tree_td<T, DD, true>(tv, q, nq, order, k, dist, idx, orig, s); \
         else                                                                   \
-            launch_tree_td<T, DD, false>(tv, q, nq, order, k, dist, idx, orig, per_wave, s); \
+            launch_tree_td<T, DD, false>(tv, q, nq, order, k, dist, idx, orig, s); \
         break;
     switch (t.dims) {
         TK_DIMS(1)

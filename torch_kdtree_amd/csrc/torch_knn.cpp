@@ -88,8 +88,6 @@ struct KDTree {
         return py::make_tuple(axis, split, leaf_off);
     }
 
-    void set_queries_per_wave(int qpw) { check(tk_tree_set_queries_per_wave(tree, qpw)); }
-
     void query(size_t points_query_ptr, size_t nr_query_points, int32_t nr_nns_searches, size_t dist_arr_ptr,
                size_t knn_idx_ptr, size_t orig_idx_ptr, int tie, int presort, size_t stream, int n_threads) {
         int rc;
@@ -126,8 +124,6 @@ void bind_tree(py::module_& m, const char* name) {
              py::arg("tie") = 0, py::arg("presort") = -1, py::arg("stream") = 0, py::arg("n_threads") = 0,
              "Queries the KNN from the KD-Tree and puts the results in the array pointed to by dist_arr_ptr and "
              "knn_idx_ptr")
-        .def("set_queries_per_wave", &C::set_queries_per_wave, py::arg("queries_per_wave"),
-             "GPU tuning: queries per wavefront of the traversal kernel (0 = default, else a multiple of 64)")
         .def_readonly("levels", &C::levels);
 }
 

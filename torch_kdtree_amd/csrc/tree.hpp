@@ -38,7 +38,6 @@ struct Tree {
     std::vector<unsigned char> points;  // [n_points * dims] values of type T, structured order
     std::vector<int32_t> shuffled;      // [n_points] original index of each structured position
     DeviceTree* dev = nullptr;          // GPU copy, owned
-    int queries_per_wave = 0;           // GPU tuning: queries each wavefront works through (0 = default)
 
     template <typename T> T* split_ptr() { return reinterpret_cast<T*>(split.data()); }
     template <typename T> const T* split_ptr() const { return reinterpret_cast<const T*>(split.data()); }
