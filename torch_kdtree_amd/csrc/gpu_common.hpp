@@ -95,15 +95,61 @@ __device__ __forceinline__ T proj_bound(const V& q, const V& pp, int a, T m) {
     return s;
 }
 
-// Bound used by the lower-index tie policy: v[a] = m - q[a] rounded once, so that by the
+// Cell bound used by the lower-index tie policy: v[a] = m - q[a] rounded once, so that by the
 // monotonicity of rounding it never exceeds the computed distance of any point beyond the
-// split (the reference's two-rounding form above can exceed it by an ulp).
+// split (the reference's two-rounding form above can exceed it by an ulp).  pp[i] is q[i] or a
+// split the path crossed on axis i, so |p[i] - q[i]| >= |pp[i] - q[i]| for every point p there.
 template <typename T, int D, typename V>
 __device__ __forceinline__ T clean_bound(const V& q, const V& pp, int a, T m) {
     T s = T(0);
 #pragma unroll
     for (int i = 0; i < D; ++i) {
         const T v = (i == a) ? (m - q[i]) : (pp[i] - q[i]);
+        s = s + v * v;
+    }
+    return s;
+}
+
+// Point bounding box of a tree node, lo[D] then hi[D], each padded to kPad elements so that a
+// box is a whole number of aligned 16-byte words (8 bytes for a 1-d f32 box).
+template <typename T, int D>
+struct BoxLayout {
+    static constexpr int kPad = D == 3 ? 4 : D;
+    static constexpr int kBytes = 2 * kPad * (int)sizeof(T);
+    static constexpr int kWords = kBytes / 4;
+};
+
+// Squared distance from q to the box of heap node `node`: per axis the gap to the box, one
+// rounded difference (max(lo - q, q - hi, 0), the two candidates cannot both be positive),
+// squared and summed in axis order like a distance.  By the monotonicity of rounding it never
+// exceeds the computed distance of a point in the box.  An empty box (lo = +inf, hi = -inf)
+// gives +inf.
+template <typename T, int D>
+__device__ __forceinline__ T aabb_bound(const unsigned char* __restrict__ boxes, int32_t node, const Pt<T, D>& q) {
+    using L = BoxLayout<T, D>;
+    uint32_t w[L::kWords];
+    if constexpr (L::kBytes % 16 == 0) {
+        const uint4* p = reinterpret_cast<const uint4*>(boxes + (size_t)node * L::kBytes);
+#pragma unroll
+        for (int i = 0; i < L::kBytes / 16; ++i) {
+            const uint4 v = p[i];
+            w[4 * i + 0] = v.x;
+            w[4 * i + 1] = v.y;
+            w[4 * i + 2] = v.z;
+            w[4 * i + 3] = v.w;
+        }
+    } else {
+        const uint2 v = reinterpret_cast<const uint2*>(boxes + (size_t)node * L::kBytes)[0];
+        w[0] = v.x;
+        w[1] = v.y;
+    }
+    T lo[D], hi[D];
+    __builtin_memcpy(lo, w, sizeof(T) * D);
+    __builtin_memcpy(hi, reinterpret_cast<const unsigned char*>(w) + sizeof(T) * L::kPad, sizeof(T) * D);
+    T s = T(0);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        const T v = fmax(fmax(lo[i] - q[i], q[i] - hi[i]), T(0));
         s = s + v * v;
     }
     return s;

@@ -14,13 +14,13 @@
 // The reference keeps a per-node tag array; here the state is the node index and the
 // child we came back from, which is all the tag array encodes on the active path.
 //
-// Extra pruning that keeps the reference's output: besides the reference's bound, a far side
-// is tested against the distance from q to that side's cell (the box cut out by every split
-// on the path, kept as per-axis offsets `bo`).  When that box distance is >= worst, every
-// point below has d >= worst and the reference would reject all of them; its visit would then
-// change nothing but pp, where it leaves pp[a] = q[a] for each axis a of the levels it walks
-// (it always walks the near path down to the leaf-parent level).  Skipping the side and
-// applying exactly those resets keeps every later decision identical to the reference's.
+// Extra pruning that keeps the reference's output: besides the reference's bound, a leaf or a
+// far side is tested against the distance from q to the bounding box of its points.  When
+// that box distance is >= worst, every point there has d >= worst and the reference would
+// reject all of them; its visit would then change nothing but pp, where it leaves pp[a] = q[a]
+// for each axis a of the levels it walks (it always walks the near path down to the
+// leaf-parent level).  Skipping and applying exactly those resets keeps every later decision
+// identical to the reference's.
 #include <cstring>
 #include <limits>
 #include <vector>
@@ -53,13 +53,18 @@ inline T proj_bound(const T* q, const T* pp, int a, T m) {
     return s;
 }
 
-// Box bound of the far side of a split (a, m): offsets bo[i] to the cell on the other axes,
-// m - q[a] on axis a, each rounded once, squared and summed in axis order like a distance.
+// Squared distance from q to a point bounding box (lo[D], hi[D]): per axis the gap to the box,
+// one rounded difference, squared and summed in axis order like a distance, so by the
+// monotonicity of rounding it never exceeds the computed distance of a point in the box.
 template <typename T, int D>
-inline T box_bound(const T* q, const T* bo, int a, T m) {
+inline T aabb_bound(const T* q, const T* box) {
     T s = T(0);
     for (int i = 0; i < D; ++i) {
-        const T v = (i == a) ? T(m - q[i]) : bo[i];
+        T v = T(0);
+        if (q[i] < box[i])
+            v = box[i] - q[i];
+        else if (q[i] > box[D + i])
+            v = q[i] - box[D + i];
         s = s + v * v;
     }
     return s;
@@ -78,6 +83,7 @@ struct Searcher {
     const Tree& t;
     const T* split;
     const T* pts;
+    const T* aabb;
     int k;
 
     void scan(int32_t leaf, const T* q, T* bd, int32_t* bi) const {
@@ -102,12 +108,11 @@ struct Searcher {
         }
     }
 
+    const T* box(int32_t heap_node) const { return aabb + (size_t)heap_node * 2 * D; }
+
     void one(const T* q, T* bd, int32_t* bi) const {
-        T pp[D], bo[D];
-        for (int i = 0; i < D; ++i) {
-            pp[i] = q[i];
-            bo[i] = T(0);
-        }
+        T pp[D];
+        for (int i = 0; i < D; ++i) pp[i] = q[i];
         const int L = t.levels;
         const int32_t nn = t.n_nodes;
         uint32_t far_mask = 0;  // bit l: the path goes through the far child of its level-l node
@@ -120,24 +125,25 @@ struct Searcher {
                 ++level;
             }
         };
+        auto scan_if_useful = [&](int32_t leaf) {
+            if (aabb_bound<T, D>(q, box(nn + leaf)) < bd[k - 1]) scan(leaf, q, bd, bi);
+        };
         descend();
         for (;;) {
-            // at a leaf-parent: near leaf, then the far leaf if both bounds allow it
+            // at a leaf-parent: near leaf, then the far leaf if the reference looks at it
             {
                 const int a = level % D;
                 const T m = split[node];
                 const bool lower = q[a] < m;
                 const int32_t left_leaf = 2 * node + 1 - nn;
-                scan(lower ? left_leaf : left_leaf + 1, q, bd, bi);
-                if (proj_bound<T, D>(q, pp, a, m) < bd[k - 1] && box_bound<T, D>(q, bo, a, m) < bd[k - 1])
-                    scan(lower ? left_leaf + 1 : left_leaf, q, bd, bi);
+                scan_if_useful(lower ? left_leaf : left_leaf + 1);
+                if (proj_bound<T, D>(q, pp, a, m) < bd[k - 1]) scan_if_useful(lower ? left_leaf + 1 : left_leaf);
                 if (level == 0) return;
                 pp[a] = q[a];
             }
-            // Climb.  Levels whose far child is on the path (far_mask) are left without a test:
-            // their axis is reset in pp and the box offset of that axis reverts to the deepest
-            // far split of the same axis still on the path.  Levels entered through their near
-            // child get the far-side test.  Bit scans jump from one such level to the next.
+            // Climb.  Levels whose far child is on the path (far_mask) are left without a test
+            // (their axis is reset in pp); levels entered through their near child get the
+            // far-side test.  Bit scans jump from one such level to the next.
             bool entered = false;
             int l = level;
             for (;;) {
@@ -147,13 +153,8 @@ struct Searcher {
                 const uint32_t popped = far_mask & below & ~(j >= 0 ? ((2u << j) - 1u) : 0u);
                 if (popped) {
                     far_mask &= ~popped;
-                    for (int ax = 0; ax < D; ++ax) {
-                        const uint32_t lv = axis_levels<D>(ax);
-                        if (!(popped & lv)) continue;
-                        pp[ax] = q[ax];
-                        const uint32_t rem = far_mask & lv;
-                        bo[ax] = rem ? T(split[((node + 1) >> (level - (31 - __builtin_clz(rem)))) - 1] - q[ax]) : T(0);
-                    }
+                    for (int ax = 0; ax < D; ++ax)
+                        if (popped & axis_levels<D>(ax)) pp[ax] = q[ax];
                 }
                 if (j < 0) return;
                 const int32_t nj = ((node + 1) >> (level - j)) - 1;
@@ -162,10 +163,10 @@ struct Searcher {
                 const bool lower = q[a] < m;
                 pp[a] = m;
                 if (proj_bound<T, D>(q, pp, a, m) < bd[k - 1]) {
-                    if (box_bound<T, D>(q, bo, a, m) < bd[k - 1]) {
+                    const int32_t far_child = 2 * nj + (lower ? 2 : 1);
+                    if (aabb_bound<T, D>(q, box(far_child)) < bd[k - 1]) {
                         far_mask |= 1u << j;
-                        bo[a] = m - q[a];
-                        node = 2 * nj + (lower ? 2 : 1);
+                        node = far_child;
                         level = j + 1;
                         entered = true;
                         break;
@@ -186,7 +187,7 @@ struct Searcher {
 template <typename T, int D>
 void query_typed(const Tree& t, const T* q, int64_t nq, int k, T* dist, int32_t* idx, int64_t* orig,
                  int n_threads) {
-    Searcher<T, D> s{t, t.split_ptr<T>(), t.points_ptr<T>(), k};
+    Searcher<T, D> s{t, t.split_ptr<T>(), t.points_ptr<T>(), t.aabb_ptr<T>(), k};
     if (n_threads <= 0) n_threads = omp_get_max_threads();
 #pragma omp parallel num_threads(n_threads)
     {

@@ -106,6 +106,23 @@ int rec_bytes_for(int dtype, int dims) {
     throw std::runtime_error("unsupported dims");
 }
 
+int box_bytes_for(int dtype, int dims) {
+#define TK_BB(TT)                                  \
+    switch (dims) {                                \
+        case 1: return BoxLayout<TT, 1>::kBytes;   \
+        case 2: return BoxLayout<TT, 2>::kBytes;   \
+        case 3: return BoxLayout<TT, 3>::kBytes;   \
+        case 8: return BoxLayout<TT, 8>::kBytes;   \
+    }
+    if (dtype == kF32) {
+        TK_BB(float)
+    } else {
+        TK_BB(double)
+    }
+#undef TK_BB
+    throw std::runtime_error("unsupported dims");
+}
+
 void free_device_tree(Tree& t) noexcept {
     DeviceTree* d = t.dev;
     if (!d) return;
@@ -117,6 +134,7 @@ void free_device_tree(Tree& t) noexcept {
     (void)hipFree(d->leaf_off);
     (void)hipFree(d->shuffled);
     (void)hipFree(d->inv_shuffled);
+    (void)hipFree(d->boxes);
     if (prev >= 0) (void)hipSetDevice(prev);
     delete d;
     t.dev = nullptr;
@@ -140,24 +158,38 @@ void upload_tree(Tree& t, int device) {
         std::memcpy(r, &t.points[s * coord_bytes], coord_bytes);
         std::memcpy(r + coord_bytes, &t.shuffled[s], 4);
     }
+    // boxes: lo then hi, each padded from dims to box_bytes / (2 * ts) elements
+    d->box_bytes = box_bytes_for(t.dtype, t.dims);
+    const size_t n_boxes = (size_t)t.n_nodes + (size_t)t.n_leaves;
+    const size_t half = (size_t)d->box_bytes / 2;
+    std::vector<unsigned char> boxes(n_boxes * (size_t)d->box_bytes, 0);
+    for (size_t b = 0; b < n_boxes; ++b) {
+        const unsigned char* src = &t.aabb[b * 2 * coord_bytes];
+        unsigned char* dst = &boxes[b * (size_t)d->box_bytes];
+        std::memcpy(dst, src, coord_bytes);
+        std::memcpy(dst + half, src + coord_bytes, coord_bytes);
+    }
     try {
         TK_HIP_CHECK(hipMalloc(&d->split, (size_t)t.n_nodes * ts));
         TK_HIP_CHECK(hipMalloc((void**)&d->recs, recs.size()));
         TK_HIP_CHECK(hipMalloc((void**)&d->leaf_off, sizeof(int32_t) * t.leaf_off.size()));
         TK_HIP_CHECK(hipMalloc((void**)&d->shuffled, sizeof(int32_t) * n));
         TK_HIP_CHECK(hipMalloc((void**)&d->inv_shuffled, sizeof(int32_t) * n));
+        TK_HIP_CHECK(hipMalloc((void**)&d->boxes, boxes.size()));
         TK_HIP_CHECK(hipMemcpy(d->split, t.split.data(), (size_t)t.n_nodes * ts, hipMemcpyHostToDevice));
         TK_HIP_CHECK(hipMemcpy(d->recs, recs.data(), recs.size(), hipMemcpyHostToDevice));
         TK_HIP_CHECK(hipMemcpy(d->leaf_off, t.leaf_off.data(), sizeof(int32_t) * t.leaf_off.size(),
                                hipMemcpyHostToDevice));
         TK_HIP_CHECK(hipMemcpy(d->shuffled, t.shuffled.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
         TK_HIP_CHECK(hipMemcpy(d->inv_shuffled, inv.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+        TK_HIP_CHECK(hipMemcpy(d->boxes, boxes.data(), boxes.size(), hipMemcpyHostToDevice));
     } catch (...) {
         (void)hipFree(d->split);
         (void)hipFree(d->recs);
         (void)hipFree(d->leaf_off);
         (void)hipFree(d->shuffled);
         (void)hipFree(d->inv_shuffled);
+        (void)hipFree(d->boxes);
         throw;
     }
     t.dev = d.release();

@@ -18,10 +18,13 @@ struct DeviceTree {
     int32_t* leaf_off = nullptr;      // [n_leaves + 1]
     int32_t* shuffled = nullptr;      // [n_points] original index of each structured position
     int32_t* inv_shuffled = nullptr;  // [n_points] structured position of each original index
+    unsigned char* boxes = nullptr;   // [n_nodes + n_leaves] point boxes, BoxLayout<T, dims>::kBytes each
     int rec_bytes = 0;
+    int box_bytes = 0;
 };
 
 int rec_bytes_for(int dtype, int dims);
+int box_bytes_for(int dtype, int dims);
 void upload_tree(Tree& t, int device);
 void free_device_tree(Tree& t) noexcept;
 

@@ -27,6 +27,7 @@ struct TreeView {
     const int32_t* __restrict__ leaf_off;   // [n_leaves + 1]
     const int32_t* __restrict__ shuffled;   // [n_points] original index of a structured position
     const int32_t* __restrict__ inv_shuffled; // [n_points] structured position of an original index
+    const unsigned char* __restrict__ boxes;  // [n_nodes + n_leaves] point boxes, heap order
     int levels;
     int32_t n_nodes;
 };
@@ -106,21 +107,6 @@ __device__ __forceinline__ void scan_leaf(const TreeView<T>& tv, int32_t leaf, c
     }
 }
 
-// Box bound of the far side of split (a, m): bo[i] is the rounded offset from q to the current
-// cell on axis i (0 when q is inside it); on axis a the far side starts at m.  Each term is one
-// rounded difference, squared and summed in axis order like a distance, so by the monotonicity
-// of rounding it never exceeds the computed distance of a point on that side.
-template <typename T, int D>
-__device__ __forceinline__ T box_bound(const Pt<T, D>& q, const Pt<T, D>& bo, int a, T m) {
-    T s = T(0);
-#pragma unroll
-    for (int i = 0; i < D; ++i) {
-        const T v = (i == a) ? (m - q[i]) : bo[i];
-        s = s + v * v;
-    }
-    return s;
-}
-
 // Bits l < 32 with l mod D == a.
 template <int D>
 __device__ __forceinline__ uint32_t axis_levels(int a) {
@@ -131,21 +117,31 @@ __device__ __forceinline__ uint32_t axis_levels(int a) {
     return m;
 }
 
-// Far side test.  Reference policy: the reference's own test (proj_bound < worst) decides
-// whether the reference would look at the far side; the box bound then tells whether that look
-// could change anything: when box >= worst every point there has d >= worst and would be
-// rejected.  Lower-index policy: only the box bound, non-strict, so that a point at exactly the
-// worst distance with a lower index is still found.
+// Far side test for the child `far_child` of split (a, m).  Reference policy: the reference's own
+// test (proj_bound < worst) decides whether the reference would look at the far side; the
+// distance to the box of the points there then tells whether that look could change anything:
+// when it is >= worst every point there has d >= worst and would be rejected.  Lower-index
+// policy: the cell bound with one rounding per term and the box distance, both non-strict, so
+// that a point at exactly the worst distance with a lower index is still found.
 // Returns 0: skip (the reference skips too), 1: enter, 2: skip a side the reference enters.
 template <typename T, int D, bool kLower>
-__device__ __forceinline__ int far_side(const Pt<T, D>& q, const Pt<T, D>& pp, const Pt<T, D>& bo, int a, T m,
-                                       T worst) {
+__device__ __forceinline__ int far_side(const TreeView<T>& tv, const Pt<T, D>& q, const Pt<T, D>& pp, int a, T m,
+                                       int32_t far_child, T worst) {
     if constexpr (kLower) {
-        return box_bound<T, D>(q, bo, a, m) <= worst ? 1 : 0;
+        if (!(clean_bound<T, D>(q, pp, a, m) <= worst)) return 0;
+        return aabb_bound<T, D>(tv.boxes, far_child, q) <= worst ? 1 : 0;
     } else {
         if (!(proj_bound<T, D>(q, pp, a, m) < worst)) return 0;
-        return box_bound<T, D>(q, bo, a, m) < worst ? 1 : 2;
+        return aabb_bound<T, D>(tv.boxes, far_child, q) < worst ? 1 : 2;
     }
+}
+
+// Whether a leaf can hold a point the list accepts (the reference scans it anyway; when the
+// distance to its box is >= worst it would reject every point).
+template <typename T, int D, bool kLower>
+__device__ __forceinline__ bool leaf_useful(const TreeView<T>& tv, const Pt<T, D>& q, int32_t leaf, T worst) {
+    const T b = aabb_bound<T, D>(tv.boxes, tv.n_nodes + leaf, q);
+    return kLower ? b <= worst : b < worst;
 }
 
 // Run-time axis access.  The element is taken from / put into a copy of the whole vector, so
@@ -225,18 +221,18 @@ __device__ __forceinline__ int axis_of_level(int l) { return l % D; }
 // The search of one query; `kb` must be initialised.
 //
 // Besides pp (the reference's projected point, kept exactly as the reference keeps it) each lane
-// carries bo, the offsets from q to the cell of the current node, and far_mask, the levels at
-// which the current path took a far child.  When the reference would enter a far side whose box
-// bound is >= worst, the lane skips it and applies the only lasting effect the reference's visit
-// would have had: pp[a] = q[a] for the axes of the levels below (the reference always walks the
-// near path down to the leaf-parent level and resets each axis it leaves).
+// carries far_mask, the levels at which the current path took a far child.  Leaves and far sides
+// are also tested against the distance from q to the bounding box of their points.  When the
+// reference would enter a far side whose box distance is >= worst, the lane skips it and applies
+// the only lasting effect the reference's visit would have had: pp[a] = q[a] for the axes of the
+// levels below (the reference always walks the near path down to the leaf-parent level and
+// resets each axis it leaves).
 //
 // Climbing: coming back up, a level whose far child is on the path (bit set in far_mask) is
-// simply left (its axis is reset in pp and the box offset of that axis reverts to the deepest
-// far split of the same axis still on the path); only levels entered through their near child
-// need a far-side test.  So the climb jumps from one such level to the next with bit scans and
+// simply left (its axis is reset in pp); only levels entered through their near child need a
+// far-side test.  So the climb jumps from one such level to the next with bit scans and
 // handles the far levels in between in one step.
-// Lane state of one search: q, pp, bo, the k-list, far_mask, node, level.  They are kept as
+// Lane state of one search: q, pp, the k-list, far_mask, node, level.  They are kept as
 // separate variables of the kernel (not one aggregate), so that each can live in registers.
 
 // Walks the near path from `node` down to the leaf-parent level.
@@ -253,11 +249,9 @@ __device__ __forceinline__ void lane_descend(const TreeView<T>& tv, const Pt<T, 
 // Starts the search of the query in q.
 template <typename T, int D, int K, bool kLower>
 __device__ __forceinline__ void lane_start(const TreeView<T>& tv, int k, const Pt<T, D>& q, Pt<T, D>& pp,
-                                           Pt<T, D>& bo, KBest<T, K, kLower>& kb, uint32_t& far_mask,
-                                           int32_t& node, int& level) {
+                                           KBest<T, K, kLower>& kb, uint32_t& far_mask, int32_t& node,
+                                           int& level) {
     pp = q;
-#pragma unroll
-    for (int i = 0; i < D; ++i) bo[i] = T(0);
     kb.init(k);
     far_mask = 0;
     node = 0;
@@ -268,7 +262,7 @@ __device__ __forceinline__ void lane_start(const TreeView<T>& tv, int k, const P
 // Scans the leaf pair of the current leaf-parent, then climbs to the next far side to enter and
 // walks down to its leaf-parent.  Returns true when the search is complete.
 template <typename T, int D, int K, bool kLower, int CH>
-__device__ __forceinline__ bool lane_step(const TreeView<T>& tv, const Pt<T, D>& q, Pt<T, D>& pp, Pt<T, D>& bo,
+__device__ __forceinline__ bool lane_step(const TreeView<T>& tv, const Pt<T, D>& q, Pt<T, D>& pp,
                                           KBest<T, K, kLower>& kb, uint32_t& far_mask, int32_t& node, int& level,
                                           T* __restrict__ st_d, int32_t* __restrict__ st_o) {
     const int L = tv.levels;
@@ -278,9 +272,14 @@ __device__ __forceinline__ bool lane_step(const TreeView<T>& tv, const Pt<T, D>&
         const T qa = ax_get(q, a_lp);
         const bool lower = qa < m;
         const int32_t left_leaf = 2 * node + 1 - tv.n_nodes;
-        scan_leaf_staged<T, D, K, kLower, CH>(tv, lower ? left_leaf : left_leaf + 1, q, kb, st_d, st_o);
-        if (far_side<T, D, kLower>(q, pp, bo, a_lp, m, kb.worst()) == 1)
-            scan_leaf_staged<T, D, K, kLower, CH>(tv, lower ? left_leaf + 1 : left_leaf, q, kb, st_d, st_o);
+        const int32_t near_leaf = lower ? left_leaf : left_leaf + 1;
+        const int32_t far_leaf = lower ? left_leaf + 1 : left_leaf;
+        if (leaf_useful<T, D, kLower>(tv, q, near_leaf, kb.worst()))
+            scan_leaf_staged<T, D, K, kLower, CH>(tv, near_leaf, q, kb, st_d, st_o);
+        if ((kLower ? clean_bound<T, D>(q, pp, a_lp, m) <= kb.worst()
+                    : proj_bound<T, D>(q, pp, a_lp, m) < kb.worst()) &&
+            leaf_useful<T, D, kLower>(tv, q, far_leaf, kb.worst()))
+            scan_leaf_staged<T, D, K, kLower, CH>(tv, far_leaf, q, kb, st_d, st_o);
         if (level == 0) return true;
         ax_set(pp, a_lp, qa);
     }
@@ -293,31 +292,20 @@ __device__ __forceinline__ bool lane_step(const TreeView<T>& tv, const Pt<T, D>&
         if (popped) {
             far_mask &= ~popped;
 #pragma unroll
-            for (int ax = 0; ax < D; ++ax) {
-                const uint32_t lv = axis_levels<D>(ax);
-                if (popped & lv) {
-                    pp[ax] = q[ax];
-                    const uint32_t rem = far_mask & lv;
-                    T off = T(0);
-                    if (rem) {
-                        const int lp = 31 - __builtin_clz(rem);
-                        off = tv.split[((node + 1) >> (level - lp)) - 1] - q[ax];
-                    }
-                    bo[ax] = off;
-                }
-            }
+            for (int ax = 0; ax < D; ++ax)
+                if (popped & axis_levels<D>(ax)) pp[ax] = q[ax];
         }
         if (j < 0) return true;
         const int32_t nj = ((node + 1) >> (level - j)) - 1;
         const int aj = axis_of_level<D>(j);
         const T mj = tv.split[nj];
         const T qj = ax_get(q, aj);
+        const int32_t far_child = 2 * nj + (qj < mj ? 2 : 1);
         ax_set(pp, aj, mj);
-        const int f = far_side<T, D, kLower>(q, pp, bo, aj, mj, kb.worst());
+        const int f = far_side<T, D, kLower>(tv, q, pp, aj, mj, far_child, kb.worst());
         if (f == 1) {
             far_mask |= 1u << j;
-            ax_set(bo, aj, T(mj - qj));
-            node = 2 * nj + (qj < mj ? 2 : 1);
+            node = far_child;
             level = j + 1;
             lane_descend<T, D>(tv, q, node, level);
             return false;
@@ -338,92 +326,68 @@ __device__ __forceinline__ bool lane_step(const TreeView<T>& tv, const Pt<T, D>&
     }
 }
 
-// Reference-order search with a generic list (used with the LDS list for large k).
+// Reference-order search with a generic list (used with the LDS list for large k): the same walk
+// as lane_start/lane_step, one node at a time.
 template <typename T, int D, bool kLower, typename List>
 __device__ __forceinline__ void knn_search_one(const TreeView<T>& tv, const Pt<T, D>& q, List& kb) {
     Pt<T, D> pp = q;
-    Pt<T, D> bo;
-#pragma unroll
-    for (int i = 0; i < D; ++i) bo[i] = T(0);
-    uint32_t far_mask = 0;
     const int L = tv.levels;
     const int32_t nn = tv.n_nodes;
     int32_t node = 0;
     int level = 0;
     int a = 0;  // split axis of `node`: level mod D
-    while (level < L - 1) {
-        node = 2 * node + (ax_get(q, a) < tv.split[node] ? 1 : 2);
-        ++level;
-        a = ax_next<D>(a);
-    }
-    // phase 0: the near leaf of leaf-parent `node` is next; phase 1: the far leaf is to be
-    // tested; phase 2: both leaves are done, climb.
-    int phase = 0;
     for (;;) {
-        int32_t leaf;
-        const T m = tv.split[node];
-        const T qa = ax_get(q, a);
-        const bool lower = qa < m;
-        if (phase == 0) {
-            leaf = 2 * node + (lower ? 1 : 2) - nn;
-            phase = 1;
-        } else if (phase == 1 && far_side<T, D, kLower>(q, pp, bo, a, m, kb.worst()) == 1) {
-            leaf = 2 * node + (lower ? 2 : 1) - nn;
-            phase = 2;
-        } else {
-            if (level == 0) break;
-            ax_set(pp, a, qa);
-            bool entered = false;
-            while (level > 0) {
-                const int32_t child = node;
-                node = (node - 1) >> 1;
-                --level;
-                a = ax_prev<D>(a);
-                const T m2 = tv.split[node];
-                const T q2 = ax_get(q, a);
-                const bool lower2 = q2 < m2;
-                if (child == 2 * node + (lower2 ? 1 : 2)) {
-                    ax_set(pp, a, m2);
-                    const int f = far_side<T, D, kLower>(q, pp, bo, a, m2, kb.worst());
-                    if (f == 1) {
-                        far_mask |= 1u << level;
-                        ax_set(bo, a, T(m2 - q2));
-                        node = 2 * node + (lower2 ? 2 : 1);
-                        ++level;
-                        a = ax_next<D>(a);
-                        entered = true;
-                        break;
-                    }
-                    if (f == 2) {
-                        const int nlev = min(D, L - 1 - level);
-#pragma unroll
-                        for (int i = 0; i < D; ++i) {
-                            const int off = (i - a - 1 + 2 * D) % D;
-                            if (off < nlev) pp[i] = q[i];
-                        }
-                    }
-                } else {
-                    far_mask &= ~(1u << level);
-                    const uint32_t above = far_mask & axis_levels<D>(a) & ((1u << level) - 1u);
-                    T off = T(0);
-                    if (above) {
-                        const int lp = 31 - __builtin_clz(above);
-                        off = tv.split[((node + 1) >> (level - lp)) - 1] - q2;
-                    }
-                    ax_set(bo, a, off);
-                }
-                ax_set(pp, a, q2);
-            }
-            if (!entered) break;
-            while (level < L - 1) {
-                node = 2 * node + (ax_get(q, a) < tv.split[node] ? 1 : 2);
-                ++level;
-                a = ax_next<D>(a);
-            }
-            leaf = 2 * node + (ax_get(q, a) < tv.split[node] ? 1 : 2) - nn;
-            phase = 1;
+        while (level < L - 1) {
+            node = 2 * node + (ax_get(q, a) < tv.split[node] ? 1 : 2);
+            ++level;
+            a = ax_next<D>(a);
         }
-        scan_leaf<T, D>(tv, leaf, q, kb);
+        {
+            const T m = tv.split[node];
+            const T qa = ax_get(q, a);
+            const bool lower = qa < m;
+            const int32_t near_leaf = 2 * node + (lower ? 1 : 2) - nn;
+            const int32_t far_leaf = 2 * node + (lower ? 2 : 1) - nn;
+            if (leaf_useful<T, D, kLower>(tv, q, near_leaf, kb.worst())) scan_leaf<T, D>(tv, near_leaf, q, kb);
+            if ((kLower ? clean_bound<T, D>(q, pp, a, m) <= kb.worst()
+                        : proj_bound<T, D>(q, pp, a, m) < kb.worst()) &&
+                leaf_useful<T, D, kLower>(tv, q, far_leaf, kb.worst()))
+                scan_leaf<T, D>(tv, far_leaf, q, kb);
+            if (level == 0) return;
+            ax_set(pp, a, qa);
+        }
+        bool entered = false;
+        while (level > 0) {
+            const int32_t child = node;
+            node = (node - 1) >> 1;
+            --level;
+            a = ax_prev<D>(a);
+            const T m2 = tv.split[node];
+            const T q2 = ax_get(q, a);
+            const bool lower2 = q2 < m2;
+            if (child == 2 * node + (lower2 ? 1 : 2)) {
+                const int32_t far_child = 2 * node + (lower2 ? 2 : 1);
+                ax_set(pp, a, m2);
+                const int f = far_side<T, D, kLower>(tv, q, pp, a, m2, far_child, kb.worst());
+                if (f == 1) {
+                    node = far_child;
+                    ++level;
+                    a = ax_next<D>(a);
+                    entered = true;
+                    break;
+                }
+                if (f == 2) {
+                    const int nlev = min(D, L - 1 - level);
+#pragma unroll
+                    for (int i = 0; i < D; ++i) {
+                        const int off = (i - a - 1 + 2 * D) % D;
+                        if (off < nlev) pp[i] = q[i];
+                    }
+                }
+            }
+            ax_set(pp, a, q2);
+        }
+        if (!entered) return;
     }
 }
 
@@ -454,14 +418,14 @@ __global__ __launch_bounds__(kTreeBlock) void knn_tree_kernel(TreeView<T> tv, co
     const int64_t t = (int64_t)blockIdx.x * kTreeBlock + threadIdx.x;
     if (t >= nq) return;
     const int64_t qi = order ? (int64_t)order[t] : t;
-    Pt<T, D> q, pp, bo;
+    Pt<T, D> q, pp;
     KBest<T, K, kLower> kb;
     uint32_t far_mask;
     int32_t node;
     int level;
     load_query<T, D>(queries, qi, q);
-    lane_start<T, D, K, kLower>(tv, k, q, pp, bo, kb, far_mask, node, level);
-    while (!lane_step<T, D, K, kLower, CH>(tv, q, pp, bo, kb, far_mask, node, level, stage_d + threadIdx.x,
+    lane_start<T, D, K, kLower>(tv, k, q, pp, kb, far_mask, node, level);
+    while (!lane_step<T, D, K, kLower, CH>(tv, q, pp, kb, far_mask, node, level, stage_d + threadIdx.x,
                                             stage_o + (kLower ? threadIdx.x : 0))) {
     }
     const int pad = K - k;

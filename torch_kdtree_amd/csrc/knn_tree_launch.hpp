@@ -10,7 +10,7 @@ namespace tk {
 template <typename T>
 TreeView<T> tree_view(const Tree& t) {
     return TreeView<T>{static_cast<const T*>(t.dev->split), t.dev->recs, t.dev->leaf_off, t.dev->shuffled,
-                       t.dev->inv_shuffled, t.levels, t.n_nodes};
+                       t.dev->inv_shuffled, t.dev->boxes, t.levels, t.n_nodes};
 }
 
 template <typename T, int D, bool kLower>

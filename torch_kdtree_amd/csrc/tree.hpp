@@ -37,12 +37,17 @@ struct Tree {
     std::vector<int32_t> leaf_off;      // [n_leaves + 1]
     std::vector<unsigned char> points;  // [n_points * dims] values of type T, structured order
     std::vector<int32_t> shuffled;      // [n_points] original index of each structured position
+    // Bounding box of the points below each node, heap order over internal nodes and leaves
+    // (leaf j is heap node n_nodes + j): [(n_nodes + n_leaves) * 2 * dims] values of type T,
+    // per node lo[dims] then hi[dims]; an empty leaf has lo = +inf, hi = -inf.
+    std::vector<unsigned char> aabb;
     DeviceTree* dev = nullptr;          // GPU copy, owned
 
     template <typename T> T* split_ptr() { return reinterpret_cast<T*>(split.data()); }
     template <typename T> const T* split_ptr() const { return reinterpret_cast<const T*>(split.data()); }
     template <typename T> T* points_ptr() { return reinterpret_cast<T*>(points.data()); }
     template <typename T> const T* points_ptr() const { return reinterpret_cast<const T*>(points.data()); }
+    template <typename T> const T* aabb_ptr() const { return reinterpret_cast<const T*>(aabb.data()); }
 };
 
 // Largest tree depth accepted by the build (the reference caps `levels` at 13 in Python,

@@ -15,6 +15,7 @@
 // depend on the thread count.
 #include <algorithm>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <utility>
 
@@ -79,6 +80,38 @@ struct Builder {
     }
 };
 
+// Point bounding boxes: leaves from their points, internal nodes as the union of their children.
+template <typename T, int D>
+void compute_aabb(Tree& t) {
+    const int32_t nn = t.n_nodes, nl = t.n_leaves;
+    t.aabb.assign((size_t)(nn + nl) * 2 * D * sizeof(T), 0);
+    T* box = reinterpret_cast<T*>(t.aabb.data());
+    const T* pts = t.points_ptr<T>();
+    const T inf = std::numeric_limits<T>::infinity();
+#pragma omp parallel for schedule(static)
+    for (int32_t j = 0; j < nl; ++j) {
+        T* b = box + (size_t)(nn + j) * 2 * D;
+        for (int a = 0; a < D; ++a) {
+            b[a] = inf;
+            b[D + a] = -inf;
+        }
+        for (int32_t s = t.leaf_off[j]; s < t.leaf_off[j + 1]; ++s)
+            for (int a = 0; a < D; ++a) {
+                b[a] = std::min(b[a], pts[(size_t)s * D + a]);
+                b[D + a] = std::max(b[D + a], pts[(size_t)s * D + a]);
+            }
+    }
+    for (int32_t i = nn - 1; i >= 0; --i) {
+        T* b = box + (size_t)i * 2 * D;
+        const T* l = box + (size_t)(2 * i + 1) * 2 * D;
+        const T* r = box + (size_t)(2 * i + 2) * 2 * D;
+        for (int a = 0; a < D; ++a) {
+            b[a] = std::min(l[a], r[a]);
+            b[D + a] = std::max(l[D + a], r[D + a]);
+        }
+    }
+}
+
 template <typename T, int D>
 void build_typed(Tree& t, const T* src, int n_threads) {
     std::memcpy(t.points.data(), src, sizeof(T) * D * (size_t)t.n_points);
@@ -88,6 +121,7 @@ void build_typed(Tree& t, const T* src, int n_threads) {
 #pragma omp single
     b.node(0, 0, t.n_points, t.levels - 1, 0);
     t.leaf_off[t.n_leaves] = t.n_points;
+    compute_aabb<T, D>(t);
 }
 
 }  // namespace
