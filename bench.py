@@ -3,14 +3,16 @@
 
 Default workload (``plain_gpu``): d=3 fp32, 4,000,000 reference points U[0,1]^3, 16,000,000
 queries (half U[-0.1,1.1]^3, half from a 32-component Gaussian mixture, sigma 0.02), k=16.
-A step is one ``TorchKDTree.query`` over all 16M queries (leaf presort + traversal + outputs:
+A step is one ``TorchKDTree.query`` over all 16M queries (Morton presort + traversal + outputs:
 squared distances, structured int32 indices and original int64 indices), with the tree and the
 queries already resident in HBM.  The tree build and upload are outside the timed region, as
 in the reference's benchmark (README.md:75).
 
 Prints one JSON line.  ``roofline`` refers to the traversal kernel: its average duration is
 measured with HIP events on the launch stream during the timed steps; ``achieved`` divides
-the algorithmic bytes of one launch by it.  ``cpu_baseline`` times the oracle (the C++
+the algorithmic bytes of one launch by it; ``traffic`` is the HBM traffic of one launch from the
+rocprofv3 FETCH_SIZE / WRITE_SIZE summary under profiles/, reported when that summary was taken
+on the same configuration (tree depth, tie policy, presort).  ``cpu_baseline`` times the oracle (the C++
 restatement of the reference CPU search, OpenMP over queries) on a bounded sample of the same
 queries and the same tree on this host.
 """
@@ -25,6 +27,21 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "knn_tree_plain_gpu_traffic.txt")
+
+
+def _measured_traffic(levels, tie, presort):
+    """HBM bytes per traversal launch from the committed counter summary (key: value lines)."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            kv = dict(line.split(":", 1) for line in f if ":" in line and not line.startswith("#"))
+    except OSError:
+        return None
+    kv = {k.strip(): v.strip() for k, v in kv.items()}
+    if (kv.get("levels") != str(levels) or kv.get("tie_break") != tie or kv.get("presort") != presort
+            or "traffic_bytes_per_launch" not in kv):
+        return None
+    return int(kv["traffic_bytes_per_launch"])
 
 
 def _parse():
@@ -36,7 +53,7 @@ def _parse():
     ap.add_argument("--levels", default="auto", help="tree depth: 'auto', 'default' or an int")
     ap.add_argument("--tie", default="reference", choices=["reference", "lower_index"])
     ap.add_argument("--presort", default="auto", choices=["auto", "morton", "leaf", "off"])
-    ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="queries timed on the CPU (0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 22, help="queries timed on the CPU (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before the timed steps, run one device-to-device copy of this many MiB (a kernel of known "
@@ -47,8 +64,8 @@ def _parse():
 def main():
     args = _parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (queries shard with no exchange; "
-              "see the sharding note in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (each GPU would hold the whole tree and a "
+              "slice of the queries, with no exchange between GPUs). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -155,7 +172,8 @@ def main():
                    "tree_build_s": round(build_s, 3),
                    "presort_ms_per_step": pre_ms / max(pre_n, 1), "traverse_ms_per_step": trav_avg_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": _measured_traffic(tree.levels, args.tie, args.presort),
                      "kernel": "knn_tree_kernel<float,3,16,false>",
                      "algorithmic_bytes_per_launch": launch_bytes},
         "cpu_baseline": cpu,

@@ -150,7 +150,8 @@ void upload_tree(Tree& t, int device) {
     const size_t ts = dtype_size(t.dtype);
     const size_t coord_bytes = ts * (size_t)t.dims;
     const size_t n = (size_t)t.n_points;
-    std::vector<unsigned char> recs(n * (size_t)d->rec_bytes, 0);
+    // kRecPad zero records after the last point: chunked leaf scans read them and ignore them
+    std::vector<unsigned char> recs((n + kRecPad) * (size_t)d->rec_bytes, 0);
     std::vector<int32_t> inv(n);
     for (size_t s = 0; s < n; ++s) inv[(size_t)t.shuffled[s]] = (int32_t)s;
     for (size_t s = 0; s < n; ++s) {

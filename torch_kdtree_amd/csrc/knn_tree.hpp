@@ -37,10 +37,20 @@ struct TreeView {
 #define TK_TREE_BLOCK 128
 #endif
 #ifndef TK_STAGE_CHUNK_F32
-#define TK_STAGE_CHUNK_F32 8
+#define TK_STAGE_CHUNK_F32 4
 #endif
-#ifndef TK_SCAN_UNROLL
-#define TK_SCAN_UNROLL 8
+#ifndef TK_SCAN_LOADS_F32
+#define TK_SCAN_LOADS_F32 4
+#endif
+// Occupancy target of the register-list traversal kernel (waves per SIMD); 0 leaves it to the
+// compiler.
+#ifndef TK_WAVES_PER_EU
+#define TK_WAVES_PER_EU 0
+#endif
+#if TK_WAVES_PER_EU > 0
+#define TK_TREE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(TK_WAVES_PER_EU)))
+#else
+#define TK_TREE_OCCUPANCY
 #endif
 constexpr int kTreeBlock = TK_TREE_BLOCK;
 constexpr int kTreeBlockStride = kTreeBlock;
@@ -173,18 +183,25 @@ __device__ __forceinline__ int ax_next(int a) { return a + 1 == D ? 0 : a + 1; }
 template <int D>
 __device__ __forceinline__ int ax_prev(int a) { return a == 0 ? D - 1 : a - 1; }
 
-// Leaf scan in two passes, for register lists.  Pass 1 computes the distances of up to CH
-// points of the leaf (all lanes busy), tests them against the worst distance at the start of
-// the chunk and stages the accepted ones in LDS; pass 2 pops each lane's accepted points in
-// scan order and inserts those still accepted by the current list.  A wavefront then runs the
-// insertion network once per accepted point of its busiest lane instead of once per point any
-// lane accepts.  The outcome is that of a one-pass scan: the list after pass 2 holds exactly
-// the points a one-pass scan would have inserted, because a point rejected by the stale worst
-// is rejected by every later (smaller or equal) worst as well.
-template <typename T, int D, int K, bool kLower, int CH>
+// Records past the last point that a chunked leaf scan may read (and ignore): the record buffer is
+// allocated with this many extra records, so the CH loads of a chunk need no per-point guard.
+constexpr int kRecPad = 16;
+
+// Leaf scan in two passes, for register lists.  Pass 1 computes the distances of CH consecutive
+// records (loads issued back to back; those past the leaf are ignored), tests them against the
+// worst distance at the start of the chunk and stages them in LDS; pass 2 pops each lane's
+// accepted points in scan order and inserts those still accepted by the current list.  A
+// wavefront then runs the insertion network once per accepted point of its busiest lane instead
+// of once per point any lane accepts.  The outcome is that of a one-pass scan: the list after
+// pass 2 holds exactly the points a one-pass scan would have inserted, because a point rejected
+// by the stale worst is rejected by every later (smaller or equal) worst as well.
+// In pass 2 a candidate that the current list rejects is inserted as +inf, which leaves the list
+// unchanged: the insertion is then unconditional and updates the list in place.
+template <typename T, int D, int K, bool kLower, int CH, int LD>
 __device__ __forceinline__ void scan_leaf_staged(const TreeView<T>& tv, int32_t leaf, const Pt<T, D>& q,
                                                  KBest<T, K, kLower>& kb, T* __restrict__ st_d,
                                                  int32_t* __restrict__ st_o) {
+    static_assert(CH % LD == 0 && CH <= 32, "a chunk is a whole number of load groups, at most 32 points");
     const int32_t beg = tv.leaf_off[leaf];
     const int32_t end = tv.leaf_off[leaf + 1];
     for (int32_t c0 = beg; c0 < end; c0 += CH) {
@@ -192,25 +209,28 @@ __device__ __forceinline__ void scan_leaf_staged(const TreeView<T>& tv, int32_t 
         const T w = kb.d[K - 1];
         const int32_t wo = kLower ? kb.x[K - 1] : 0;
         uint32_t mask = 0;
-#pragma unroll TK_SCAN_UNROLL
-        for (int j = 0; j < CH; ++j) {
-            if (j < n) {
+        // pass 1, LD records in flight at a time
+        for (int g = 0; g < CH; g += LD) {
+            if (g >= n) break;
+#pragma unroll
+            for (int jj = 0; jj < LD; ++jj) {
+                const int j = g + jj;
                 const Rec<T, D> r = load_rec<T, D>(tv.recs, c0 + j);
                 const T dd = sq_dist<T, D>(q, r.c);
                 const bool acc = kLower ? (dd < w || (dd == w && r.o < wo)) : (dd < w);
-                if (acc) {
-                    mask |= 1u << j;
-                    st_d[j * kTreeBlockStride] = dd;
-                    if constexpr (kLower) st_o[j * kTreeBlockStride] = r.o;
-                }
+                mask |= (acc && j < n) ? (1u << j) : 0u;
+                st_d[j * kTreeBlockStride] = dd;
+                if constexpr (kLower) st_o[j * kTreeBlockStride] = r.o;
             }
         }
+        // pass 2
         while (mask) {
             const int j = __builtin_ctz(mask);
             mask &= mask - 1u;
             const T dd = st_d[j * kTreeBlockStride];
             const int32_t o = kLower ? st_o[j * kTreeBlockStride] : 0;
-            if (kb.accepts(dd, o)) kb.insert(dd, kLower ? o : c0 + j);
+            const bool ok = kb.accepts(dd, o);
+            kb.insert(ok ? dd : pos_inf<T>(), ok ? (kLower ? o : c0 + j) : 0x7fffffff);
         }
     }
 }
@@ -234,6 +254,14 @@ __device__ __forceinline__ int axis_of_level(int l) { return l % D; }
 // handles the far levels in between in one step.
 // Lane state of one search: q, pp, the k-list, far_mask, node, level.  They are kept as
 // separate variables of the kernel (not one aggregate), so that each can live in registers.
+
+template <typename T, int D>
+__device__ __forceinline__ bool pp_is_q(const Pt<T, D>& pp, const Pt<T, D>& q) {
+    bool e = true;
+#pragma unroll
+    for (int i = 0; i < D; ++i) e = e && (pp[i] == q[i]);
+    return e;
+}
 
 // Walks the near path from `node` down to the leaf-parent level.
 template <typename T, int D>
@@ -261,7 +289,7 @@ __device__ __forceinline__ void lane_start(const TreeView<T>& tv, int k, const P
 
 // Scans the leaf pair of the current leaf-parent, then climbs to the next far side to enter and
 // walks down to its leaf-parent.  Returns true when the search is complete.
-template <typename T, int D, int K, bool kLower, int CH>
+template <typename T, int D, int K, bool kLower, int CH, int LD>
 __device__ __forceinline__ bool lane_step(const TreeView<T>& tv, const Pt<T, D>& q, Pt<T, D>& pp,
                                           KBest<T, K, kLower>& kb, uint32_t& far_mask, int32_t& node, int& level,
                                           T* __restrict__ st_d, int32_t* __restrict__ st_o) {
@@ -274,17 +302,44 @@ __device__ __forceinline__ bool lane_step(const TreeView<T>& tv, const Pt<T, D>&
         const int32_t left_leaf = 2 * node + 1 - tv.n_nodes;
         const int32_t near_leaf = lower ? left_leaf : left_leaf + 1;
         const int32_t far_leaf = lower ? left_leaf + 1 : left_leaf;
-        if (leaf_useful<T, D, kLower>(tv, q, near_leaf, kb.worst()))
-            scan_leaf_staged<T, D, K, kLower, CH>(tv, near_leaf, q, kb, st_d, st_o);
-        if ((kLower ? clean_bound<T, D>(q, pp, a_lp, m) <= kb.worst()
-                    : proj_bound<T, D>(q, pp, a_lp, m) < kb.worst()) &&
-            leaf_useful<T, D, kLower>(tv, q, far_leaf, kb.worst()))
-            scan_leaf_staged<T, D, K, kLower, CH>(tv, far_leaf, q, kb, st_d, st_o);
+        // near leaf, then the far leaf if the reference looks at it (one scan site for both)
+#pragma nounroll
+        for (int side = 0; side < 2; ++side) {
+            const int32_t leaf = side ? far_leaf : near_leaf;
+            const bool looks = side == 0 || (kLower ? clean_bound<T, D>(q, pp, a_lp, m) <= kb.worst()
+                                                    : proj_bound<T, D>(q, pp, a_lp, m) < kb.worst());
+            if (looks && leaf_useful<T, D, kLower>(tv, q, leaf, kb.worst()))
+                scan_leaf_staged<T, D, K, kLower, CH, LD>(tv, leaf, q, kb, st_d, st_o);
+        }
         if (level == 0) return true;
         ax_set(pp, a_lp, qa);
     }
     int l = level;
     for (;;) {
+        if (pp_is_q<T, D>(pp, q)) {
+            // With pp equal to q the reference's bound for a far side is fl(m - q[a])^2, which the
+            // distance to the far child's box never undercuts (the box lies beyond m on axis a), so
+            // the far side is entered iff that box distance passes, and a skip leaves pp as it is.
+            // The far child of a level entered through its near child is the sibling of the path's
+            // child one level down, so these tests need neither the split nor the axis.
+            uint32_t near_lv = ((1u << l) - 1u) & ~far_mask;
+            while (near_lv) {
+                const int j = 31 - __builtin_clz(near_lv);
+                const int32_t c = ((node + 1) >> (level - 1 - j)) - 1;
+                const int32_t fc = ((c - 1) ^ 1) + 1;
+                const T b = aabb_bound<T, D>(tv.boxes, fc, q);
+                if (kLower ? b <= kb.worst() : b < kb.worst()) {
+                    ax_set(pp, axis_of_level<D>(j), tv.split[(c - 1) >> 1]);
+                    far_mask = (far_mask & ((1u << j) - 1u)) | (1u << j);
+                    node = fc;
+                    level = j + 1;
+                    lane_descend<T, D>(tv, q, node, level);
+                    return false;
+                }
+                near_lv ^= 1u << j;
+            }
+            return true;
+        }
         const uint32_t below = (1u << l) - 1u;
         const uint32_t near_lv = below & ~far_mask;
         const int j = near_lv ? 31 - __builtin_clz(near_lv) : -1;
@@ -391,6 +446,12 @@ __device__ __forceinline__ void knn_search_one(const TreeView<T>& tv, const Pt<T
     }
 }
 
+// base[i] for 0 <= i < 2^31, addressed with a 32-bit byte offset (keeps i a 32-bit value; a
+// sign-extended 64-bit index would tie each k-list key to an aligned register pair).
+__device__ __forceinline__ int32_t load_i32_at(const int32_t* __restrict__ base, int32_t i) {
+    return *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(base) + ((uint32_t)i << 2));
+}
+
 template <typename T, int D>
 __device__ __forceinline__ void load_query(const T* __restrict__ queries, int64_t qi, Pt<T, D>& q) {
 #pragma unroll
@@ -400,6 +461,7 @@ __device__ __forceinline__ void load_query(const T* __restrict__ queries, int64_
 template <typename T, int K, bool kLower>
 struct StageBytes {
     static constexpr int kChunk = sizeof(T) == 4 ? TK_STAGE_CHUNK_F32 : 16;
+    static constexpr int kLoads = sizeof(T) == 4 ? TK_SCAN_LOADS_F32 : 4;
     static constexpr int kBytes = kChunk * kTreeBlock * ((int)sizeof(T) + (kLower ? 4 : 0));
 };
 
@@ -408,11 +470,13 @@ struct StageBytes {
 // keeps the wavefront's control flow and loads coherent.  (Handing a finished lane the next query
 // was measured to be slower: it breaks that lockstep.)
 template <typename T, int D, int K, bool kLower>
-__global__ __launch_bounds__(kTreeBlock) void knn_tree_kernel(TreeView<T> tv, const T* __restrict__ queries,
+__global__ __launch_bounds__(kTreeBlock) TK_TREE_OCCUPANCY void knn_tree_kernel(TreeView<T> tv, const T* __restrict__ queries,
                                                                int64_t nq, const int32_t* __restrict__ order, int k,
                                                                T* __restrict__ out_d, int32_t* __restrict__ out_i,
                                                                int64_t* __restrict__ out_o) {
     constexpr int CH = StageBytes<T, K, kLower>::kChunk;
+    constexpr int LD = StageBytes<T, K, kLower>::kLoads;
+    static_assert(LD <= kRecPad, "a load group may read at most kRecPad records past a leaf's end");
     __shared__ T stage_d[CH * kTreeBlock];
     __shared__ int32_t stage_o[kLower ? CH * kTreeBlock : 1];
     const int64_t t = (int64_t)blockIdx.x * kTreeBlock + threadIdx.x;
@@ -425,7 +489,7 @@ __global__ __launch_bounds__(kTreeBlock) void knn_tree_kernel(TreeView<T> tv, co
     int level;
     load_query<T, D>(queries, qi, q);
     lane_start<T, D, K, kLower>(tv, k, q, pp, kb, far_mask, node, level);
-    while (!lane_step<T, D, K, kLower, CH>(tv, q, pp, kb, far_mask, node, level, stage_d + threadIdx.x,
+    while (!lane_step<T, D, K, kLower, CH, LD>(tv, q, pp, kb, far_mask, node, level, stage_d + threadIdx.x,
                                             stage_o + (kLower ? threadIdx.x : 0))) {
     }
     const int pad = K - k;
@@ -435,14 +499,14 @@ __global__ __launch_bounds__(kTreeBlock) void knn_tree_kernel(TreeView<T> tv, co
     for (int j = 0; j < K; ++j) {
         if (j >= pad) {
             od[j - pad] = kb.d[j];
-            oi[j - pad] = kLower ? tv.inv_shuffled[kb.x[j]] : kb.x[j];
+            oi[j - pad] = kLower ? load_i32_at(tv.inv_shuffled, kb.x[j]) : kb.x[j];
         }
     }
     if (out_o) {
         int64_t* oo = out_o + qi * k;
 #pragma unroll
         for (int j = 0; j < K; ++j) {
-            if (j >= pad) oo[j - pad] = kLower ? kb.x[j] : tv.shuffled[kb.x[j]];
+            if (j >= pad) oo[j - pad] = kLower ? kb.x[j] : load_i32_at(tv.shuffled, kb.x[j]);
         }
     }
 }
