@@ -44,6 +44,94 @@ def _measured_traffic(levels, tie, presort):
     return int(kv["traffic_bytes_per_launch"])
 
 
+def _events(torch_knn, stage, fn, reps):
+    """Average milliseconds of one call of `fn` in the library's HIP-event bracket `stage`."""
+    torch_knn.profile_reset()
+    torch_knn.profile_enable(True)
+    for _ in range(reps):
+        fn()
+    torch_knn.profile_enable(False)
+    ms, n = torch_knn.profile_read(stage)
+    torch_knn.profile_reset()
+    return ms / max(n, 1)
+
+
+def _extra_bruteforce(np, torch, torch_knn, dev, w):
+    """Brute-force kernel on the 65,536 x 65,536 cross-check subset of plain_gpu, k=16."""
+    n, k = 65536, 16
+    half = w["halves"][0]
+    ref = torch.from_numpy(np.ascontiguousarray(w["ref"][:n])).to(dev)
+    qn = np.ascontiguousarray(np.concatenate([w["queries"][:n // 2], w["queries"][half:half + n // 2]]))
+    q = torch.from_numpy(qn).to(dev)
+    d = torch.empty((n, k), dtype=torch.float32, device=dev)
+    i = torch.empty((n, k), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def run():
+        torch_knn.knn_bruteforce(ref.data_ptr(), n, q.data_ptr(), n, 3, "float32", k, d.data_ptr(), i.data_ptr(),
+                                 stream)
+
+    run()
+    torch.cuda.synchronize()
+    ms = _events(torch_knn, "bruteforce", run, 5)
+    flops = 8.0 * n * n  # 3 sub + 3 mul + 2 add per (query, reference) pair
+    return {"workload": "plain_gpu refs[:65536] x 65,536 queries (32,768 per half), k=16",
+            "ms": ms, "tflops": flops / (ms * 1e-3) / 1e12, "peak_tflops": 157.3,
+            "frac": flops / (ms * 1e-3) / 1e12 / 157.3}
+
+
+def _extra_fwd_bwd(np, torch, torch_knn, tk, workloads, dev, steps):
+    """Differentiation workload: query + loss sum(w * dists) + gradients w.r.t. both clouds."""
+    wd = workloads.diff_sphere()
+    k = wd["k"]
+    ref = torch.from_numpy(wd["ref"]).to(dev).requires_grad_(True)
+    q = torch.from_numpy(wd["queries"]).to(dev).requires_grad_(True)
+    wts = torch.from_numpy(wd["weights"]).to(dev)
+    tree = tk.build_kd_tree(ref, device=dev, levels="auto")
+
+    def step():
+        dists, _ = tree.query(q, nr_nns_searches=k)
+        loss = (wts * dists).sum()
+        return torch.autograd.grad(loss, [q, ref])
+
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    torch_knn.profile_reset()
+    torch_knn.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    torch_knn.profile_enable(False)
+    bw_ms, bw_n = torch_knn.profile_read("backward")
+    tr_ms, tr_n = torch_knn.profile_read("traverse")
+    torch_knn.profile_reset()
+    nq, n = q.shape[0], ref.shape[0]
+    bw_avg = bw_ms / max(bw_n, 1)
+    # backward bytes: per query g (4k) + int64 idx (8k) + q (12) + dq (12); per reference point r (12) + dr (12)
+    bw_bytes = nq * (4 * k + 8 * k + 24) + n * 24
+    return {"workload": "diff_sphere: d=3 fp32, 1,000,000 refs on a perturbed sphere, 1,000,000 queries, k=8, "
+                        "loss sum(w*dists), grads w.r.t. queries and refs",
+            "levels": tree.levels, "fwd_bwd_per_s": nq * steps / el, "ms_per_step": el * 1e3 / steps,
+            "traverse_ms": tr_ms / max(tr_n, 1), "backward_ms": bw_avg,
+            "backward_gbs": bw_bytes / (bw_avg * 1e-3) / 1e9, "backward_bytes": bw_bytes}
+
+
+def _extra_default_levels(torch, tk, dev, ref, q, k, res_d, res_i, steps):
+    """The plain_gpu query on a tree of the reference's default depth (nn_distance.py:146-147)."""
+    tree = tk.build_kd_tree(torch.from_numpy(ref), device=dev, levels=None)
+    tree.query(q, nr_nns_searches=k, result_dists=res_d, result_idx=res_i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tree.query(q, nr_nns_searches=k, result_dists=res_d, result_idx=res_i)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return {"levels": tree.levels, "queries_per_s": q.shape[0] * steps / el, "ms_per_step": el * 1e3 / steps}
+
+
 def _parse():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -55,6 +143,8 @@ def _parse():
     ap.add_argument("--presort", default="auto", choices=["auto", "morton", "leaf", "off"])
     ap.add_argument("--cpu-sample", type=int, default=1 << 22, help="queries timed on the CPU (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the secondary measurements (brute force, fwd+bwd, reference default depth)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before the timed steps, run one device-to-device copy of this many MiB (a kernel of known "
                          "HBM traffic, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -154,6 +244,18 @@ def main():
                          "oracle = C++ restatement of KDTreeKNNSearch, OpenMP over queries, %d threads; %.2f s wall"
                          % (s, sample.shape[0], tree.levels, k, threads, cpu_s)}
 
+    extras = None
+    if not args.no_extras:
+        extras = {}
+        log("secondary: brute force")
+        extras["bruteforce"] = _extra_bruteforce(np, torch, torch_knn, dev, w)
+        log("secondary: plain_gpu at the reference's default depth")
+        extras["reference_default_levels"] = _extra_default_levels(torch, tk, dev, ref, q, k, res_d, res_i, 3)
+        del res_d, res_i, q
+        log("secondary: differentiation fwd+bwd")
+        extras["fwd_bwd"] = _extra_fwd_bwd(np, torch, torch_knn, tk, workloads, dev, 10)
+        extras["batched_vs_per_cloud"] = None
+
     line = {
         "metric": "KNN queries/s at d=3,k=16 fp32 (+fwd+bwd/s); batched vs per-cloud, same GPU",
         "value": value,
@@ -177,6 +279,7 @@ def main():
                      "kernel": "knn_tree_kernel<float,3,16,false>",
                      "algorithmic_bytes_per_launch": launch_bytes},
         "cpu_baseline": cpu,
+        "secondary": extras,
     }
     print(json.dumps(line), flush=True)
 
