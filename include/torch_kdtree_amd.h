@@ -125,6 +125,15 @@ int tk_knn_backward_gpu(const void* queries, const void* ref, int64_t n_queries,
                         void* workspace, size_t workspace_bytes, void* stream);
 int tk_knn_backward_gpu_workspace(int64_t n_queries, int32_t n_ref, int k, size_t* bytes);
 
+/* Stable radix sort of n (key, value) uint32 pairs on the GPU by the low key_bits bits of the
+ * keys (no counterpart in the reference: it is the sort behind the deterministic grad_ref of
+ * tk_knn_backward_gpu, exported for callers that need the same order).  Inputs and outputs
+ * are distinct device arrays; n <= 2^28.  Launches kernels only (no memset, no host wait), so
+ * it may be captured in a HIP graph. */
+int tk_sort_pairs_gpu(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
+                      int64_t n, int key_bits, void* workspace, size_t workspace_bytes, void* stream);
+int tk_sort_pairs_gpu_workspace(int64_t n, size_t* bytes);
+
 /* Kernel timing with HIP events (no counterpart in the reference, whose only timing is
  * commented out, src/kdtree_g.cu:435-456).  When enabled, every GPU entry point brackets its
  * kernels with events on its stream, per stage: "presort", "traverse", "bruteforce",

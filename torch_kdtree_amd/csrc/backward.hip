@@ -10,12 +10,10 @@
 //
 // dL/dq: one lane per query, terms added in neighbour order j = 0..k-1.
 // dL/dr: deterministic, no atomics.  The (neighbour index, flat position) pairs are sorted
-// by neighbour index with a stable LSD radix sort, so each reference point's contributions
+// by neighbour index with a stable LSD radix sort (radix_sort.hip), so each reference point's contributions
 // form a contiguous run in increasing flat position; one lane per run adds them in that
 // order.  The result is bit-identical from run to run.
 #include <memory>
-
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include "gpu_common.hpp"
 #include "knn_gpu.hpp"
@@ -99,18 +97,12 @@ int key_bits(int32_t n) {
     return b;
 }
 
-size_t sort_temp_bytes(int64_t m, int32_t n) {
-    size_t bytes = 0;
-    TK_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                           (uint32_t*)nullptr, (size_t)m, 0, key_bits(n), (hipStream_t)0));
-    return bytes;
-}
-
 struct BwLayout {
     size_t keys, vals, keys_s, vals_s, temp, temp_bytes, total;
 };
 
 BwLayout bw_layout(int64_t nq, int32_t n, int k) {
+    (void)n;
     const int64_t m = nq * k;
     BwLayout l{};
     size_t off = 0;
@@ -119,7 +111,7 @@ BwLayout bw_layout(int64_t nq, int32_t n, int k) {
     l.keys_s = off; off += a256(4 * (size_t)m);
     l.vals_s = off; off += a256(4 * (size_t)m);
     l.temp = off;
-    l.temp_bytes = sort_temp_bytes(m, n);
+    l.temp_bytes = radix_sort_workspace(m);
     off += a256(l.temp_bytes);
     l.total = off;
     return l;
@@ -145,8 +137,7 @@ void backward_typed(const T* q, const T* ref, int64_t nq, int32_t n, int k, cons
     uint32_t* vals_s = reinterpret_cast<uint32_t*>(ws + l.vals_s);
     hipLaunchKernelGGL(pairs_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, idx, m, keys, vals);
     TK_HIP_CHECK(hipGetLastError());
-    size_t tb = l.temp_bytes;
-    TK_HIP_CHECK(rocprim::radix_sort_pairs(ws + l.temp, tb, keys, keys_s, vals, vals_s, (size_t)m, 0, key_bits(n), s));
+    radix_sort_pairs(keys, vals, keys_s, vals_s, m, key_bits(n), ws + l.temp, l.temp_bytes, s);
     hipLaunchKernelGGL((grad_r_kernel<T, D>), dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, q, ref, m, k, keys_s,
                        vals_s, g, dr);
     TK_HIP_CHECK(hipGetLastError());
@@ -161,7 +152,7 @@ void knn_backward_gpu(const void* queries, const void* ref, int64_t nq, int32_t 
                       void* workspace, size_t workspace_bytes) {
     if (!dims_supported(dims)) throw std::runtime_error("unsupported dims");
     if (k < 1 || n < 1) throw std::runtime_error("invalid backward shape");
-    if (nq * (int64_t)k >= (int64_t(1) << 32)) throw std::runtime_error("backward: nr_query * k must be < 2^32");
+    if (nq * (int64_t)k > (int64_t(1) << 28)) throw std::runtime_error("backward: nr_query * k must be <= 2^28");
     const BwLayout l = bw_layout(nq, n, k);
     std::unique_ptr<StreamScratch> own;
     unsigned char* ws = static_cast<unsigned char*>(workspace);

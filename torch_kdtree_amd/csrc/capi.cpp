@@ -2,6 +2,7 @@
 // C++ exceptions and turns them into a nonzero return code plus a thread-local message.
 #include <cstring>
 #include <exception>
+#include <memory>
 #include <string>
 
 #include <hip/hip_runtime.h>
@@ -181,6 +182,25 @@ int tk_knn_backward_gpu(const void* queries, const void* ref, int64_t n_queries,
 
 int tk_knn_backward_gpu_workspace(int64_t n_queries, int32_t n_ref, int k, size_t* bytes) {
     return guarded([&] { *bytes = tk::knn_backward_workspace(n_queries, n_ref, k); });
+}
+
+int tk_sort_pairs_gpu(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
+                      int64_t n, int key_bits, void* workspace, size_t workspace_bytes, void* stream) {
+    return guarded([&] {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const size_t need = tk::radix_sort_workspace(n);
+        std::unique_ptr<tk::StreamScratch> own;
+        if (!workspace) {
+            own.reset(new tk::StreamScratch(need, s));
+            workspace = own->ptr;
+            workspace_bytes = need;
+        }
+        tk::radix_sort_pairs(keys_in, vals_in, keys_out, vals_out, n, key_bits, workspace, workspace_bytes, s);
+    });
+}
+
+int tk_sort_pairs_gpu_workspace(int64_t n, size_t* bytes) {
+    return guarded([&] { *bytes = tk::radix_sort_workspace(n); });
 }
 
 int tk_profile_enable(int on) {

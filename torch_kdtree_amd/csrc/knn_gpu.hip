@@ -298,6 +298,20 @@ __global__ void bucket_scatter_kernel(int64_t nq, const int32_t* __restrict__ bu
 // ------------------------------------------------------------------------------------------
 // Query driver
 // ------------------------------------------------------------------------------------------
+int32_t scan_sums_count(int32_t n) { return (n + kScanChunk - 1) / kScanChunk; }
+
+void exclusive_scan_i32(const int32_t* in, int32_t n, int32_t* sums, int32_t* out, hipStream_t stream) {
+    if (n > kScanChunk * kScanChunk) throw std::runtime_error("exclusive scan: too many values");
+    const int32_t nsums = scan_sums_count(n);
+    if (nsums == 0) return;
+    hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nsums), dim3(kScanBlock), 0, stream, in, n, sums);
+    TK_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, stream, sums, nsums);
+    TK_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(scan_apply_kernel, dim3(nsums), dim3(kScanBlock), 0, stream, in, n, sums, out);
+    TK_HIP_CHECK(hipGetLastError());
+}
+
 namespace {
 
 constexpr int64_t kPresortMinQueries = 4096;
@@ -409,12 +423,7 @@ void query_gpu(const Tree& t, const void* queries, int64_t nq, int k, void* dist
             launch_leaf_bucket_f64(t, static_cast<const double*>(queries), nq, bucket_shift(t), bucket, rank, counts,
                                    stream);
         }
-        hipLaunchKernelGGL(scan_block_sums_kernel, dim3(l.nsums), dim3(kScanBlock), 0, stream, counts, l.nb, sums);
-        check_launch();
-        hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, stream, sums, l.nsums);
-        check_launch();
-        hipLaunchKernelGGL(scan_apply_kernel, dim3(l.nsums), dim3(kScanBlock), 0, stream, counts, l.nb, sums, offsets);
-        check_launch();
+        exclusive_scan_i32(counts, l.nb, sums, offsets, stream);
         hipLaunchKernelGGL(bucket_scatter_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, stream, nq, bucket,
                            rank, offsets, ord);
         check_launch();

@@ -74,6 +74,19 @@ void knn_backward_gpu(const void* queries, const void* ref, int64_t nq, int32_t 
                       void* workspace, size_t workspace_bytes);
 size_t knn_backward_workspace(int64_t nq, int32_t n, int k);
 
+// Exclusive prefix sum of n int32 values (n <= 2^24) into out; sums needs scan_sums_count(n)
+// int32 of scratch.  Kernels only (no memset, no host synchronisation).
+int32_t scan_sums_count(int32_t n);
+void exclusive_scan_i32(const int32_t* in, int32_t n, int32_t* sums, int32_t* out, hipStream_t stream);
+
+// Stable LSD radix sort of (key, value) pairs of uint32 by the low key_bits bits of the keys
+// (radix_sort.hip).  Reads keys_in / vals_in, writes keys_out / vals_out (distinct from the
+// inputs); workspace of radix_sort_workspace(n) bytes.  Kernels only, so it can be captured in
+// a graph.  n <= 2^28.
+size_t radix_sort_workspace(int64_t n);
+void radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
+                      int64_t n, int key_bits, void* workspace, size_t workspace_bytes, hipStream_t stream);
+
 // RAII stream-ordered scratch allocation used when the caller passes no workspace.
 struct StreamScratch {
     void* ptr = nullptr;

@@ -1,0 +1,175 @@
+// Stable least-significant-digit radix sort of (uint32 key, uint32 value) pairs, 8 bits per pass.
+//
+// Each pass over n pairs:
+//   1. radix_hist_kernel: every block counts the digits of its tile (kRadixItems items per
+//      thread, 256 threads) in LDS and writes count[digit * n_blocks + block];
+//   2. an exclusive scan of those counts gives, for each (digit, block), the output position of
+//      the block's first pair with that digit (all smaller digits, then earlier blocks);
+//   3. radix_scatter_kernel: every block walks its tile in index order, 256 pairs at a time.
+//      Within a wavefront, lanes holding the same digit are found with one ballot per digit bit;
+//      a pair's position is its block offset, plus the pairs of its digit already placed from
+//      this tile, plus those in lower waves and lower lanes of the current 256.  The order of
+//      equal digits is therefore the input order: the sort is stable, and the result does not
+//      depend on scheduling.
+// Nothing is cleared with a memset and nothing waits on the host, so the sort can be captured in
+// a HIP graph and replayed.
+#include <algorithm>
+
+#include "gpu_common.hpp"
+#include "knn_gpu.hpp"
+
+namespace tk {
+namespace {
+
+constexpr int kRadixBlock = 256;
+constexpr int kRadixItems = 16;
+constexpr int kRadixTile = kRadixBlock * kRadixItems;
+constexpr int kRadixBins = 256;
+
+__global__ __launch_bounds__(kRadixBlock) void radix_hist_kernel(const uint32_t* __restrict__ keys, int64_t n,
+                                                                  int shift, int32_t n_blocks,
+                                                                  int32_t* __restrict__ hist) {
+    __shared__ int32_t cnt[kRadixBins];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kRadixTile;
+#pragma unroll
+    for (int s = 0; s < kRadixItems; ++s) {
+        const int64_t i = base + (int64_t)s * kRadixBlock + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & (kRadixBins - 1)], 1);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * n_blocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kRadixBlock) void radix_scatter_kernel(const uint32_t* __restrict__ keys_in,
+                                                                     const uint32_t* __restrict__ vals_in, int64_t n,
+                                                                     int shift, int32_t n_blocks,
+                                                                     const int32_t* __restrict__ offsets,
+                                                                     uint32_t* __restrict__ keys_out,
+                                                                     uint32_t* __restrict__ vals_out) {
+    constexpr int kWaves = kRadixBlock / 64;
+    __shared__ int32_t placed[kRadixBins];        // pairs of each digit placed from this tile so far
+    __shared__ int32_t wave_cnt[kWaves][kRadixBins];  // pairs of each digit per wave in the current 256
+    __shared__ int32_t block_off[kRadixBins];
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int w = t >> 6;
+    placed[t] = 0;
+    block_off[t] = offsets[(int64_t)t * n_blocks + blockIdx.x];
+#pragma unroll
+    for (int v = 0; v < kWaves; ++v) wave_cnt[v][t] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kRadixTile;
+    const uint64_t lower_lanes = (1ull << lane) - 1ull;
+    for (int s = 0; s < kRadixItems; ++s) {
+        const int64_t i = base + (int64_t)s * kRadixBlock + t;
+        const bool valid = i < n;
+        const uint32_t key = valid ? keys_in[i] : 0u;
+        const uint32_t val = valid ? vals_in[i] : 0u;
+        // invalid slots get digit 256, a group of their own
+        const int dg = valid ? (int)((key >> shift) & (kRadixBins - 1)) : kRadixBins;
+        uint64_t same = ~0ull;
+#pragma unroll
+        for (int b = 0; b < 9; ++b) {
+            const bool bit = (dg >> b) & 1;
+            const uint64_t bal = __ballot(bit);
+            same &= bit ? bal : ~bal;
+        }
+        const int rank = __popcll(same & lower_lanes);
+        const bool last_of_group = (same >> lane) == 1ull;
+        if (valid && last_of_group) wave_cnt[w][dg] = rank + 1;
+        __syncthreads();
+        if (valid) {
+            int pos = block_off[dg] + placed[dg] + rank;
+            for (int v = 0; v < w; ++v) pos += wave_cnt[v][dg];
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();
+        int add = 0;
+#pragma unroll
+        for (int v = 0; v < kWaves; ++v) {
+            add += wave_cnt[v][t];
+            wave_cnt[v][t] = 0;
+        }
+        placed[t] += add;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void copy_u32_kernel(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+                                                       int64_t n, uint32_t* __restrict__ oa, uint32_t* __restrict__ ob) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        oa[i] = a[i];
+        ob[i] = b[i];
+    }
+}
+
+inline size_t a256(size_t b) { return (b + 255) & ~size_t(255); }
+
+struct RadixLayout {
+    int32_t n_blocks, n_hist, n_sums;
+    size_t keys_tmp, vals_tmp, hist, offs, sums, total;
+};
+
+RadixLayout radix_layout(int64_t n) {
+    RadixLayout l{};
+    l.n_blocks = (int32_t)std::max<int64_t>(1, (n + kRadixTile - 1) / kRadixTile);
+    l.n_hist = l.n_blocks * kRadixBins;
+    l.n_sums = scan_sums_count(l.n_hist);
+    size_t off = 0;
+    l.keys_tmp = off; off += a256(4 * (size_t)n);
+    l.vals_tmp = off; off += a256(4 * (size_t)n);
+    l.hist = off; off += a256(4 * (size_t)l.n_hist);
+    l.offs = off; off += a256(4 * (size_t)l.n_hist);
+    l.sums = off; off += a256(4 * (size_t)l.n_sums);
+    l.total = off;
+    return l;
+}
+
+}  // namespace
+
+size_t radix_sort_workspace(int64_t n) { return radix_layout(n).total; }
+
+void radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
+                      int64_t n, int key_bits, void* workspace, size_t workspace_bytes, hipStream_t stream) {
+    if (n < 0 || n > (int64_t(1) << 28)) throw std::runtime_error("radix sort: n must be in [0, 2^28]");
+    if (key_bits < 0 || key_bits > 32) throw std::runtime_error("radix sort: key_bits must be in [0, 32]");
+    if (n == 0) return;
+    const RadixLayout l = radix_layout(n);
+    if (!workspace || workspace_bytes < l.total) throw std::runtime_error("radix sort: workspace too small");
+    unsigned char* ws = static_cast<unsigned char*>(workspace);
+    uint32_t* kt = reinterpret_cast<uint32_t*>(ws + l.keys_tmp);
+    uint32_t* vt = reinterpret_cast<uint32_t*>(ws + l.vals_tmp);
+    int32_t* hist = reinterpret_cast<int32_t*>(ws + l.hist);
+    int32_t* offs = reinterpret_cast<int32_t*>(ws + l.offs);
+    int32_t* sums = reinterpret_cast<int32_t*>(ws + l.sums);
+    const int passes = (key_bits + 7) / 8;
+    if (passes == 0) {
+        hipLaunchKernelGGL(copy_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, keys_in, vals_in,
+                           n, keys_out, vals_out);
+        TK_HIP_CHECK(hipGetLastError());
+        return;
+    }
+    // the last pass writes the output: alternate so that it does
+    const uint32_t* ks = keys_in;
+    const uint32_t* vs = vals_in;
+    for (int p = 0; p < passes; ++p) {
+        const bool to_out = ((passes - 1 - p) % 2) == 0;
+        uint32_t* kd = to_out ? keys_out : kt;
+        uint32_t* vd = to_out ? vals_out : vt;
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(l.n_blocks), dim3(kRadixBlock), 0, stream, ks, n, 8 * p, l.n_blocks,
+                           hist);
+        TK_HIP_CHECK(hipGetLastError());
+        exclusive_scan_i32(hist, l.n_hist, sums, offs, stream);
+        hipLaunchKernelGGL(radix_scatter_kernel, dim3(l.n_blocks), dim3(kRadixBlock), 0, stream, ks, vs, n, 8 * p,
+                           l.n_blocks, offs, kd, vd);
+        TK_HIP_CHECK(hipGetLastError());
+        ks = kd;
+        vs = vd;
+    }
+}
+
+}  // namespace tk

@@ -199,6 +199,22 @@ PYBIND11_MODULE(torch_knn, m) {
         py::arg("dtype"), py::arg("k"), py::arg("idx_ptr"), py::arg("grad_dist_ptr"), py::arg("grad_queries_ptr"),
         py::arg("grad_ref_ptr"), py::arg("stream") = 0);
 
+    m.def(
+        "sort_pairs",
+        [](size_t keys_in, size_t vals_in, size_t keys_out, size_t vals_out, int64_t n, int key_bits, size_t stream) {
+            int rc;
+            {
+                py::gil_scoped_release release;
+                rc = tk_sort_pairs_gpu(reinterpret_cast<const uint32_t*>(keys_in),
+                                       reinterpret_cast<const uint32_t*>(vals_in), reinterpret_cast<uint32_t*>(keys_out),
+                                       reinterpret_cast<uint32_t*>(vals_out), n, key_bits, nullptr, 0,
+                                       reinterpret_cast<void*>(stream));
+            }
+            check(rc);
+        },
+        py::arg("keys_in_ptr"), py::arg("vals_in_ptr"), py::arg("keys_out_ptr"), py::arg("vals_out_ptr"), py::arg("n"),
+        py::arg("key_bits"), py::arg("stream") = 0,
+        "Stable GPU radix sort of uint32 (key, value) pairs by the low key_bits bits of the keys");
     m.def("profile_enable", [](bool on) { check(tk_profile_enable(on ? 1 : 0)); }, py::arg("on"));
     m.def("profile_reset", [] { check(tk_profile_reset()); });
     m.def(
