@@ -70,6 +70,12 @@ int tk_device_count(int* count);
 int tk_tree_build(const void* points, int64_t n_points, int dims, int dtype, int levels, int n_threads,
                   tk_tree** out);
 
+/* Builds n_trees trees in parallel (one tree per thread, largest first), each as tk_tree_build
+ * would: out[i] from points[i][n_points[i]*dims] with levels[i].  On failure no tree is left
+ * allocated.  (No counterpart in the reference, whose trees are built one by one.) */
+int tk_tree_build_many(const void* const* points, const int64_t* n_points, int n_trees, int dims, int dtype,
+                       const int32_t* levels, int n_threads, tk_tree** out);
+
 /* Replaces copyPartitionToGPU (src/kdtree_g.cu:63-98), called from the constructor at
  * src/interface.cpp:38-41.  Copies the tree to GPU `device` (-1 = current device). */
 int tk_tree_upload(tk_tree* tree, int device);
@@ -124,6 +130,40 @@ int tk_knn_backward_gpu(const void* queries, const void* ref, int64_t n_queries,
                         int k, const int64_t* idx, const void* grad_dist, void* grad_queries, void* grad_ref,
                         void* workspace, size_t workspace_bytes, void* stream);
 int tk_knn_backward_gpu_workspace(int64_t n_queries, int32_t n_ref, int k, size_t* bytes);
+
+/* Batched many-cloud mode (no counterpart in the reference, which queries one tree per call;
+ * it replaces a loop of KDTree::query calls, src/interface.cpp:60-86, over many clouds).
+ * A plan packs n_trees trees (same dims and dtype; built, not necessarily uploaded) with their
+ * query counts n_queries[c] and k values ks[c] (1..32) into one device arena.  The trees may be
+ * freed afterwards.
+ *   Queries: one row-major [sum n_queries, dims] array; cloud c owns rows
+ *     query_off[c] .. query_off[c+1] (tk_batch_offsets).
+ *   Outputs: dist (dtype), idx (int32, structured indices of the cloud's tree) and orig_idx
+ *     (int64, original indices within the cloud's point array; may be NULL) are flat arrays of
+ *     sum(n_queries[c] * ks[c]) elements; cloud c owns out_off[c] .. out_off[c+1], row-major
+ *     [n_queries[c], ks[c]].  Results equal those of tk_knn_query_gpu on each tree.
+ *   Backward: grad_dist has the layout of dist; grad_queries the layout of the queries;
+ *     grad_ref is [sum n_points, dims], cloud c owning rows ref_off[c] .. ref_off[c+1] in its
+ *     original point order.  Either gradient may be NULL.  Deterministic, and equal to
+ *     tk_knn_backward_gpu on each cloud.
+ * All scratch memory is allocated with the plan; a plan serves one stream at a time.
+ * tk_batch_capture records forward (+ backward when grad_dist and a gradient are given) with
+ * the given buffers into a HIP graph; tk_batch_launch replays it on a stream. */
+typedef struct tk_batch tk_batch;
+int tk_batch_create(const tk_tree* const* trees, int n_trees, const int64_t* n_queries, const int32_t* ks, int tie,
+                    int device, tk_batch** out);
+void tk_batch_free(tk_batch* batch);
+/* out[0] = total queries, out[1] = total output elements, out[2] = total reference points,
+ * out[3] = number of clouds. */
+int tk_batch_sizes(const tk_batch* batch, int64_t out[4]);
+/* Each array (any may be NULL) receives n_clouds + 1 prefix offsets. */
+int tk_batch_offsets(const tk_batch* batch, int64_t* query_off, int64_t* out_off, int64_t* ref_off);
+int tk_batch_forward(tk_batch* batch, const void* queries, void* dist, int32_t* idx, int64_t* orig_idx, void* stream);
+int tk_batch_backward(tk_batch* batch, const void* queries, const int32_t* idx, const void* grad_dist,
+                      void* grad_queries, void* grad_ref, void* stream);
+int tk_batch_capture(tk_batch* batch, const void* queries, void* dist, int32_t* idx, int64_t* orig_idx,
+                     const void* grad_dist, void* grad_queries, void* grad_ref, void* stream);
+int tk_batch_launch(tk_batch* batch, void* stream);
 
 /* Stable radix sort of n (key, value) uint32 pairs on the GPU by the low key_bits bits of the
  * keys (no counterpart in the reference: it is the sort behind the deterministic grad_ref of

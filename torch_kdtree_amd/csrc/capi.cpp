@@ -1,18 +1,26 @@
 // extern "C" boundary (declared in include/torch_kdtree_amd.h).  Every entry point catches
 // C++ exceptions and turns them into a nonzero return code plus a thread-local message.
+#include <algorithm>
 #include <cstring>
 #include <exception>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include <hip/hip_runtime.h>
+#include <omp.h>
 
 #include "../../include/torch_kdtree_amd.h"
+#include "batch.hpp"
 #include "knn_gpu.hpp"
 #include "tree.hpp"
 
 struct tk_tree {
     tk::Tree* t;
+};
+
+struct tk_batch {
+    tk::BatchPlan* p;
 };
 
 namespace {
@@ -68,6 +76,44 @@ int tk_tree_build(const void* points, int64_t n_points, int dims, int dtype, int
         if (n_points > 0 && !points) throw std::runtime_error("null points");
         tk::Tree* t = tk::build_tree(points, n_points, dims, levels, dtype, n_threads);
         *out = new tk_tree{t};
+    });
+}
+
+int tk_tree_build_many(const void* const* points, const int64_t* n_points, int n_trees, int dims, int dtype,
+                       const int32_t* levels, int n_threads, tk_tree** out) {
+    return guarded([&] {
+        if (n_trees < 0 || (n_trees > 0 && (!points || !n_points || !levels || !out)))
+            throw std::runtime_error("invalid arguments");
+        check_dtype(dtype);
+        for (int i = 0; i < n_trees; ++i) out[i] = nullptr;
+        if (n_trees == 1) {
+            out[0] = new tk_tree{tk::build_tree(points[0], n_points[0], dims, levels[0], dtype, n_threads)};
+            return;
+        }
+        // largest first, one tree per thread (the build's own tasks stay on that thread)
+        std::vector<int> order(n_trees);
+        for (int i = 0; i < n_trees; ++i) order[i] = i;
+        std::sort(order.begin(), order.end(), [&](int a, int b) { return n_points[a] > n_points[b]; });
+        std::vector<std::string> errors(n_trees);
+        const int nt = n_threads > 0 ? n_threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+        for (int j = 0; j < n_trees; ++j) {
+            const int i = order[j];
+            try {
+                out[i] = new tk_tree{tk::build_tree(points[i], n_points[i], dims, levels[i], dtype, 1)};
+            } catch (const std::exception& e) {
+                errors[i] = e.what();
+            }
+        }
+        for (int i = 0; i < n_trees; ++i) {
+            if (!errors[i].empty()) {
+                for (int j = 0; j < n_trees; ++j) {
+                    tk_tree_free(out[j]);
+                    out[j] = nullptr;
+                }
+                throw std::runtime_error("tree " + std::to_string(i) + ": " + errors[i]);
+            }
+        }
     });
 }
 
@@ -182,6 +228,82 @@ int tk_knn_backward_gpu(const void* queries, const void* ref, int64_t n_queries,
 
 int tk_knn_backward_gpu_workspace(int64_t n_queries, int32_t n_ref, int k, size_t* bytes) {
     return guarded([&] { *bytes = tk::knn_backward_workspace(n_queries, n_ref, k); });
+}
+
+int tk_batch_create(const tk_tree* const* trees, int n_trees, const int64_t* n_queries, const int32_t* ks, int tie,
+                    int device, tk_batch** out) {
+    return guarded([&] {
+        if (!out) throw std::runtime_error("null output handle");
+        *out = nullptr;
+        if (n_trees < 1 || !trees || !n_queries || !ks) throw std::runtime_error("invalid arguments");
+        std::vector<const tk::Tree*> ts(n_trees);
+        for (int i = 0; i < n_trees; ++i) {
+            if (!trees[i] || !trees[i]->t) throw std::runtime_error("null tree handle");
+            ts[i] = trees[i]->t;
+        }
+        *out = new tk_batch{tk::batch_create(ts.data(), n_trees, n_queries, ks, tie, device)};
+    });
+}
+
+void tk_batch_free(tk_batch* batch) {
+    if (!batch) return;
+    tk::batch_free(batch->p);
+    delete batch;
+}
+
+int tk_batch_sizes(const tk_batch* batch, int64_t out[4]) {
+    return guarded([&] {
+        if (!batch || !batch->p) throw std::runtime_error("null batch handle");
+        out[0] = batch->p->total_q;
+        out[1] = batch->p->total_out;
+        out[2] = batch->p->total_ref;
+        out[3] = batch->p->n_clouds;
+    });
+}
+
+int tk_batch_offsets(const tk_batch* batch, int64_t* query_off, int64_t* out_off, int64_t* ref_off) {
+    return guarded([&] {
+        if (!batch || !batch->p) throw std::runtime_error("null batch handle");
+        const tk::BatchPlan& p = *batch->p;
+        for (int c = 0; c <= p.n_clouds; ++c) {
+            const bool end = c == p.n_clouds;
+            if (query_off) query_off[c] = end ? p.total_q : p.clouds[c].q_base;
+            if (out_off) out_off[c] = end ? p.total_out : p.clouds[c].out_base;
+            if (ref_off) ref_off[c] = end ? p.total_ref : p.clouds[c].rec_base;
+        }
+    });
+}
+
+int tk_batch_forward(tk_batch* batch, const void* queries, void* dist, int32_t* idx, int64_t* orig_idx, void* stream) {
+    return guarded([&] {
+        if (!batch || !batch->p) throw std::runtime_error("null batch handle");
+        tk::batch_forward(*batch->p, queries, dist, idx, orig_idx, static_cast<hipStream_t>(stream));
+    });
+}
+
+int tk_batch_backward(tk_batch* batch, const void* queries, const int32_t* idx, const void* grad_dist,
+                      void* grad_queries, void* grad_ref, void* stream) {
+    return guarded([&] {
+        if (!batch || !batch->p) throw std::runtime_error("null batch handle");
+        tk::batch_backward(*batch->p, queries, idx, grad_dist, grad_queries, grad_ref,
+                           static_cast<hipStream_t>(stream));
+    });
+}
+
+int tk_batch_capture(tk_batch* batch, const void* queries, void* dist, int32_t* idx, int64_t* orig_idx,
+                     const void* grad_dist, void* grad_queries, void* grad_ref, void* stream) {
+    return guarded([&] {
+        if (!batch || !batch->p) throw std::runtime_error("null batch handle");
+        tk::batch_capture(*batch->p, queries, dist, idx, orig_idx, grad_dist, grad_queries, grad_ref,
+                          static_cast<hipStream_t>(stream));
+    });
+}
+
+int tk_batch_launch(tk_batch* batch, void* stream) {
+    return guarded([&] {
+        if (!batch || !batch->p) throw std::runtime_error("null batch handle");
+        tk::batch_launch(*batch->p, static_cast<hipStream_t>(stream));
+    });
 }
 
 int tk_sort_pairs_gpu(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,

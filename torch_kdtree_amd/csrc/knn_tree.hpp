@@ -20,7 +20,10 @@
 
 namespace tk {
 
-template <typename T>
+// Device view of a tree.  kBatched: the arrays belong to an arena of many trees and this lane's
+// tree starts at the per-lane offsets below (set per query); otherwise the offsets are zero and
+// fold away.  Leaf offsets hold arena-wide record positions in both cases.
+template <typename T, bool kBatched = false>
 struct TreeView {
     const T* __restrict__ split;            // [n_nodes] heap order
     const unsigned char* __restrict__ recs; // [n_points] records, structured order
@@ -30,6 +33,19 @@ struct TreeView {
     const unsigned char* __restrict__ boxes;  // [n_nodes + n_leaves] point boxes, heap order
     int levels;
     int32_t n_nodes;
+    // batched only: where this tree's nodes, boxes, leaf offsets and records start in the arena
+    int32_t node_base = 0, box_base = 0, leaf_base = 0, rec_base = 0;
+
+    __device__ __forceinline__ T split_at(int32_t node) const { return split[kBatched ? node_base + node : node]; }
+    __device__ __forceinline__ int32_t leaf_begin(int32_t leaf) const {
+        return leaf_off[kBatched ? leaf_base + leaf : leaf];
+    }
+    __device__ __forceinline__ int32_t leaf_end(int32_t leaf) const {
+        return leaf_off[kBatched ? leaf_base + leaf + 1 : leaf + 1];
+    }
+    __device__ __forceinline__ int32_t box_of(int32_t heap_node) const {
+        return kBatched ? box_base + heap_node : heap_node;
+    }
 };
 
 // Threads per block of the traversal kernel; per-lane LDS staging slots are strided by it.
@@ -106,10 +122,10 @@ struct KBestLds {
     }
 };
 
-template <typename T, int D, typename List>
-__device__ __forceinline__ void scan_leaf(const TreeView<T>& tv, int32_t leaf, const Pt<T, D>& q, List& kb) {
-    const int32_t beg = tv.leaf_off[leaf];
-    const int32_t end = tv.leaf_off[leaf + 1];
+template <typename T, int D, typename List, bool kB>
+__device__ __forceinline__ void scan_leaf(const TreeView<T, kB>& tv, int32_t leaf, const Pt<T, D>& q, List& kb) {
+    const int32_t beg = tv.leaf_begin(leaf);
+    const int32_t end = tv.leaf_end(leaf);
     for (int32_t s = beg; s < end; ++s) {
         const Rec<T, D> r = load_rec<T, D>(tv.recs, s);
         const T dd = sq_dist<T, D>(q, r.c);
@@ -134,23 +150,23 @@ __device__ __forceinline__ uint32_t axis_levels(int a) {
 // policy: the cell bound with one rounding per term and the box distance, both non-strict, so
 // that a point at exactly the worst distance with a lower index is still found.
 // Returns 0: skip (the reference skips too), 1: enter, 2: skip a side the reference enters.
-template <typename T, int D, bool kLower>
-__device__ __forceinline__ int far_side(const TreeView<T>& tv, const Pt<T, D>& q, const Pt<T, D>& pp, int a, T m,
+template <typename T, int D, bool kLower, bool kB>
+__device__ __forceinline__ int far_side(const TreeView<T, kB>& tv, const Pt<T, D>& q, const Pt<T, D>& pp, int a, T m,
                                        int32_t far_child, T worst) {
     if constexpr (kLower) {
         if (!(clean_bound<T, D>(q, pp, a, m) <= worst)) return 0;
-        return aabb_bound<T, D>(tv.boxes, far_child, q) <= worst ? 1 : 0;
+        return aabb_bound<T, D>(tv.boxes, tv.box_of(far_child), q) <= worst ? 1 : 0;
     } else {
         if (!(proj_bound<T, D>(q, pp, a, m) < worst)) return 0;
-        return aabb_bound<T, D>(tv.boxes, far_child, q) < worst ? 1 : 2;
+        return aabb_bound<T, D>(tv.boxes, tv.box_of(far_child), q) < worst ? 1 : 2;
     }
 }
 
 // Whether a leaf can hold a point the list accepts (the reference scans it anyway; when the
 // distance to its box is >= worst it would reject every point).
-template <typename T, int D, bool kLower>
-__device__ __forceinline__ bool leaf_useful(const TreeView<T>& tv, const Pt<T, D>& q, int32_t leaf, T worst) {
-    const T b = aabb_bound<T, D>(tv.boxes, tv.n_nodes + leaf, q);
+template <typename T, int D, bool kLower, bool kB>
+__device__ __forceinline__ bool leaf_useful(const TreeView<T, kB>& tv, const Pt<T, D>& q, int32_t leaf, T worst) {
+    const T b = aabb_bound<T, D>(tv.boxes, tv.box_of(tv.n_nodes + leaf), q);
     return kLower ? b <= worst : b < worst;
 }
 
@@ -197,13 +213,13 @@ constexpr int kRecPad = 16;
 // by the stale worst is rejected by every later (smaller or equal) worst as well.
 // In pass 2 a candidate that the current list rejects is inserted as +inf, which leaves the list
 // unchanged: the insertion is then unconditional and updates the list in place.
-template <typename T, int D, int K, bool kLower, int CH, int LD>
-__device__ __forceinline__ void scan_leaf_staged(const TreeView<T>& tv, int32_t leaf, const Pt<T, D>& q,
+template <typename T, int D, int K, bool kLower, int CH, int LD, bool kB>
+__device__ __forceinline__ void scan_leaf_staged(const TreeView<T, kB>& tv, int32_t leaf, const Pt<T, D>& q,
                                                  KBest<T, K, kLower>& kb, T* __restrict__ st_d,
                                                  int32_t* __restrict__ st_o) {
     static_assert(CH % LD == 0 && CH <= 32, "a chunk is a whole number of load groups, at most 32 points");
-    const int32_t beg = tv.leaf_off[leaf];
-    const int32_t end = tv.leaf_off[leaf + 1];
+    const int32_t beg = tv.leaf_begin(leaf);
+    const int32_t end = tv.leaf_end(leaf);
     for (int32_t c0 = beg; c0 < end; c0 += CH) {
         const int n = min(CH, end - c0);
         const T w = kb.d[K - 1];
@@ -264,19 +280,19 @@ __device__ __forceinline__ bool pp_is_q(const Pt<T, D>& pp, const Pt<T, D>& q) {
 }
 
 // Walks the near path from `node` down to the leaf-parent level.
-template <typename T, int D>
-__device__ __forceinline__ void lane_descend(const TreeView<T>& tv, const Pt<T, D>& q, int32_t& node, int& level) {
+template <typename T, int D, bool kB>
+__device__ __forceinline__ void lane_descend(const TreeView<T, kB>& tv, const Pt<T, D>& q, int32_t& node, int& level) {
     int a = axis_of_level<D>(level);
     while (level < tv.levels - 1) {
-        node = 2 * node + (ax_get(q, a) < tv.split[node] ? 1 : 2);
+        node = 2 * node + (ax_get(q, a) < tv.split_at(node) ? 1 : 2);
         ++level;
         a = ax_next<D>(a);
     }
 }
 
 // Starts the search of the query in q.
-template <typename T, int D, int K, bool kLower>
-__device__ __forceinline__ void lane_start(const TreeView<T>& tv, int k, const Pt<T, D>& q, Pt<T, D>& pp,
+template <typename T, int D, int K, bool kLower, bool kB>
+__device__ __forceinline__ void lane_start(const TreeView<T, kB>& tv, int k, const Pt<T, D>& q, Pt<T, D>& pp,
                                            KBest<T, K, kLower>& kb, uint32_t& far_mask, int32_t& node,
                                            int& level) {
     pp = q;
@@ -289,14 +305,14 @@ __device__ __forceinline__ void lane_start(const TreeView<T>& tv, int k, const P
 
 // Scans the leaf pair of the current leaf-parent, then climbs to the next far side to enter and
 // walks down to its leaf-parent.  Returns true when the search is complete.
-template <typename T, int D, int K, bool kLower, int CH, int LD>
-__device__ __forceinline__ bool lane_step(const TreeView<T>& tv, const Pt<T, D>& q, Pt<T, D>& pp,
+template <typename T, int D, int K, bool kLower, int CH, int LD, bool kB>
+__device__ __forceinline__ bool lane_step(const TreeView<T, kB>& tv, const Pt<T, D>& q, Pt<T, D>& pp,
                                           KBest<T, K, kLower>& kb, uint32_t& far_mask, int32_t& node, int& level,
                                           T* __restrict__ st_d, int32_t* __restrict__ st_o) {
     const int L = tv.levels;
     const int a_lp = axis_of_level<D>(L - 1);  // split axis of every leaf-parent
     {
-        const T m = tv.split[node];
+        const T m = tv.split_at(node);
         const T qa = ax_get(q, a_lp);
         const bool lower = qa < m;
         const int32_t left_leaf = 2 * node + 1 - tv.n_nodes;
@@ -327,9 +343,9 @@ __device__ __forceinline__ bool lane_step(const TreeView<T>& tv, const Pt<T, D>&
                 const int j = 31 - __builtin_clz(near_lv);
                 const int32_t c = ((node + 1) >> (level - 1 - j)) - 1;
                 const int32_t fc = ((c - 1) ^ 1) + 1;
-                const T b = aabb_bound<T, D>(tv.boxes, fc, q);
+                const T b = aabb_bound<T, D>(tv.boxes, tv.box_of(fc), q);
                 if (kLower ? b <= kb.worst() : b < kb.worst()) {
-                    ax_set(pp, axis_of_level<D>(j), tv.split[(c - 1) >> 1]);
+                    ax_set(pp, axis_of_level<D>(j), tv.split_at((c - 1) >> 1));
                     far_mask = (far_mask & ((1u << j) - 1u)) | (1u << j);
                     node = fc;
                     level = j + 1;
@@ -353,7 +369,7 @@ __device__ __forceinline__ bool lane_step(const TreeView<T>& tv, const Pt<T, D>&
         if (j < 0) return true;
         const int32_t nj = ((node + 1) >> (level - j)) - 1;
         const int aj = axis_of_level<D>(j);
-        const T mj = tv.split[nj];
+        const T mj = tv.split_at(nj);
         const T qj = ax_get(q, aj);
         const int32_t far_child = 2 * nj + (qj < mj ? 2 : 1);
         ax_set(pp, aj, mj);
@@ -383,8 +399,8 @@ __device__ __forceinline__ bool lane_step(const TreeView<T>& tv, const Pt<T, D>&
 
 // Reference-order search with a generic list (used with the LDS list for large k): the same walk
 // as lane_start/lane_step, one node at a time.
-template <typename T, int D, bool kLower, typename List>
-__device__ __forceinline__ void knn_search_one(const TreeView<T>& tv, const Pt<T, D>& q, List& kb) {
+template <typename T, int D, bool kLower, typename List, bool kB>
+__device__ __forceinline__ void knn_search_one(const TreeView<T, kB>& tv, const Pt<T, D>& q, List& kb) {
     Pt<T, D> pp = q;
     const int L = tv.levels;
     const int32_t nn = tv.n_nodes;
@@ -393,12 +409,12 @@ __device__ __forceinline__ void knn_search_one(const TreeView<T>& tv, const Pt<T
     int a = 0;  // split axis of `node`: level mod D
     for (;;) {
         while (level < L - 1) {
-            node = 2 * node + (ax_get(q, a) < tv.split[node] ? 1 : 2);
+            node = 2 * node + (ax_get(q, a) < tv.split_at(node) ? 1 : 2);
             ++level;
             a = ax_next<D>(a);
         }
         {
-            const T m = tv.split[node];
+            const T m = tv.split_at(node);
             const T qa = ax_get(q, a);
             const bool lower = qa < m;
             const int32_t near_leaf = 2 * node + (lower ? 1 : 2) - nn;
@@ -417,7 +433,7 @@ __device__ __forceinline__ void knn_search_one(const TreeView<T>& tv, const Pt<T
             node = (node - 1) >> 1;
             --level;
             a = ax_prev<D>(a);
-            const T m2 = tv.split[node];
+            const T m2 = tv.split_at(node);
             const T q2 = ax_get(q, a);
             const bool lower2 = q2 < m2;
             if (child == 2 * node + (lower2 ? 1 : 2)) {
@@ -563,7 +579,7 @@ __global__ __launch_bounds__(kPresortBlock) void leaf_bucket_kernel(TreeView<T> 
     int32_t node = 0;
     int a = 0;
     for (int level = 0; level < tv.levels; ++level) {
-        node = 2 * node + (ax_get(q, a) < tv.split[node] ? 1 : 2);
+        node = 2 * node + (ax_get(q, a) < tv.split_at(node) ? 1 : 2);
         a = ax_next<D>(a);
     }
     const int32_t b = (node - tv.n_nodes) >> shift;

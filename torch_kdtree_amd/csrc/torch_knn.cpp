@@ -12,10 +12,12 @@
 // `device`, and query's `orig_idx_ptr`, `tie`, `presort`, `stream`, `n_threads`.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
 
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/torch_kdtree_amd.h"
 
@@ -108,6 +110,129 @@ struct KDTree {
     }
 };
 
+// Host trees of many clouds, built in parallel (for the batched mode).
+struct TreeSet {
+    std::vector<tk_tree*> trees;
+    int dims = 0;
+    int dtype = TK_F32;
+
+    TreeSet(const py::list& clouds, const std::vector<int32_t>& levels, int n_threads) {
+        const size_t n = clouds.size();
+        if (n == 0) throw std::runtime_error("no clouds");
+        if (levels.size() != n) throw std::runtime_error("one levels value per cloud is required");
+        std::vector<py::array> arrs;
+        std::vector<const void*> ptrs(n);
+        std::vector<int64_t> counts(n);
+        for (size_t i = 0; i < n; ++i) {
+            py::array a = py::array::ensure(clouds[i], py::array::c_style);
+            if (!a) throw std::runtime_error("clouds must be arrays");
+            const bool f32 = a.dtype().is(py::dtype::of<float>());
+            const bool f64 = a.dtype().is(py::dtype::of<double>());
+            if (!f32 && !f64) throw std::runtime_error("clouds must be float32 or float64");
+            if (a.ndim() != 2) throw std::runtime_error("clouds must have shape (N, d)");
+            if (i == 0) {
+                dims = (int)a.shape(1);
+                dtype = f32 ? TK_F32 : TK_F64;
+            } else if ((int)a.shape(1) != dims || (f32 ? TK_F32 : TK_F64) != dtype) {
+                throw std::runtime_error("all clouds must have the same dims and dtype");
+            }
+            ptrs[i] = a.data();
+            counts[i] = a.shape(0);
+            arrs.push_back(a);
+        }
+        trees.assign(n, nullptr);
+        int rc;
+        {
+            py::gil_scoped_release release;
+            rc = tk_tree_build_many(ptrs.data(), counts.data(), (int)n, dims, dtype, levels.data(), n_threads,
+                                    trees.data());
+        }
+        check(rc);
+    }
+    ~TreeSet() {
+        for (tk_tree* t : trees) tk_tree_free(t);
+    }
+    std::vector<int> info(int field) const {
+        std::vector<int> out;
+        for (tk_tree* t : trees) {
+            int32_t v[5];
+            check(tk_tree_info(t, v));
+            out.push_back(v[field]);
+        }
+        return out;
+    }
+};
+
+// Batched query plan over a TreeSet (tk_batch_*).
+struct BatchPlan {
+    tk_batch* b = nullptr;
+    int n_clouds = 0;
+
+    BatchPlan(const TreeSet& ts, const std::vector<int64_t>& n_queries, const std::vector<int32_t>& ks, int tie,
+              int device) {
+        n_clouds = (int)ts.trees.size();
+        if ((int)n_queries.size() != n_clouds || (int)ks.size() != n_clouds)
+            throw std::runtime_error("one query count and one k per cloud are required");
+        int rc;
+        {
+            py::gil_scoped_release release;
+            rc = tk_batch_create(const_cast<const tk_tree* const*>(ts.trees.data()), n_clouds, n_queries.data(),
+                                 ks.data(), tie, device, &b);
+        }
+        check(rc);
+    }
+    ~BatchPlan() { tk_batch_free(b); }
+    py::tuple sizes() const {
+        int64_t v[4];
+        check(tk_batch_sizes(b, v));
+        return py::make_tuple(v[0], v[1], v[2], v[3]);
+    }
+    py::tuple offsets() const {
+        py::array_t<int64_t> q(n_clouds + 1), o(n_clouds + 1), r(n_clouds + 1);
+        check(tk_batch_offsets(b, q.mutable_data(), o.mutable_data(), r.mutable_data()));
+        return py::make_tuple(q, o, r);
+    }
+    void forward(size_t q, size_t dist, size_t idx, size_t orig, size_t stream) {
+        int rc;
+        {
+            py::gil_scoped_release release;
+            rc = tk_batch_forward(b, reinterpret_cast<const void*>(q), reinterpret_cast<void*>(dist),
+                                  reinterpret_cast<int32_t*>(idx), reinterpret_cast<int64_t*>(orig),
+                                  reinterpret_cast<void*>(stream));
+        }
+        check(rc);
+    }
+    void backward(size_t q, size_t idx, size_t g, size_t dq, size_t dr, size_t stream) {
+        int rc;
+        {
+            py::gil_scoped_release release;
+            rc = tk_batch_backward(b, reinterpret_cast<const void*>(q), reinterpret_cast<const int32_t*>(idx),
+                                   reinterpret_cast<const void*>(g), reinterpret_cast<void*>(dq),
+                                   reinterpret_cast<void*>(dr), reinterpret_cast<void*>(stream));
+        }
+        check(rc);
+    }
+    void capture(size_t q, size_t dist, size_t idx, size_t orig, size_t g, size_t dq, size_t dr, size_t stream) {
+        int rc;
+        {
+            py::gil_scoped_release release;
+            rc = tk_batch_capture(b, reinterpret_cast<const void*>(q), reinterpret_cast<void*>(dist),
+                                  reinterpret_cast<int32_t*>(idx), reinterpret_cast<int64_t*>(orig),
+                                  reinterpret_cast<const void*>(g), reinterpret_cast<void*>(dq),
+                                  reinterpret_cast<void*>(dr), reinterpret_cast<void*>(stream));
+        }
+        check(rc);
+    }
+    void launch(size_t stream) {
+        int rc;
+        {
+            py::gil_scoped_release release;
+            rc = tk_batch_launch(b, reinterpret_cast<void*>(stream));
+        }
+        check(rc);
+    }
+};
+
 template <typename T, int D, bool kGpu>
 void bind_tree(py::module_& m, const char* name) {
     using C = KDTree<T, D, kGpu>;
@@ -154,6 +279,27 @@ PYBIND11_MODULE(torch_knn, m) {
     bind_tree<double, 8, false>(m, "KDTreeCPU8D");
     bind_tree<float, 8, true>(m, "KDTreeGPU8DF");
     bind_tree<double, 8, true>(m, "KDTreeGPU8D");
+
+    py::class_<TreeSet, std::shared_ptr<TreeSet>>(m, "TreeSet", "Host trees of many clouds, built in parallel")
+        .def(py::init<const py::list&, const std::vector<int32_t>&, int>(), py::arg("clouds"), py::arg("levels"),
+             py::arg("n_threads") = 0)
+        .def_property_readonly("levels", [](const TreeSet& t) { return t.info(2); })
+        .def_property_readonly("n_points", [](const TreeSet& t) { return t.info(3); })
+        .def_property_readonly("dims", [](const TreeSet& t) { return t.dims; })
+        .def("__len__", [](const TreeSet& t) { return t.trees.size(); });
+    py::class_<BatchPlan, std::shared_ptr<BatchPlan>>(m, "BatchPlan", "Batched KNN plan over a TreeSet")
+        .def(py::init<const TreeSet&, const std::vector<int64_t>&, const std::vector<int32_t>&, int, int>(),
+             py::arg("trees"), py::arg("n_queries"), py::arg("ks"), py::arg("tie") = 0, py::arg("device") = -1)
+        .def("sizes", &BatchPlan::sizes, "(total queries, total outputs, total reference points, clouds)")
+        .def("offsets", &BatchPlan::offsets, "(query row offsets, output offsets, reference row offsets)")
+        .def("forward", &BatchPlan::forward, py::arg("queries_ptr"), py::arg("dist_ptr"), py::arg("idx_ptr"),
+             py::arg("orig_idx_ptr") = 0, py::arg("stream") = 0)
+        .def("backward", &BatchPlan::backward, py::arg("queries_ptr"), py::arg("idx_ptr"), py::arg("grad_dist_ptr"),
+             py::arg("grad_queries_ptr") = 0, py::arg("grad_ref_ptr") = 0, py::arg("stream") = 0)
+        .def("capture", &BatchPlan::capture, py::arg("queries_ptr"), py::arg("dist_ptr"), py::arg("idx_ptr"),
+             py::arg("orig_idx_ptr") = 0, py::arg("grad_dist_ptr") = 0, py::arg("grad_queries_ptr") = 0,
+             py::arg("grad_ref_ptr") = 0, py::arg("stream") = 0)
+        .def("launch", &BatchPlan::launch, py::arg("stream") = 0);
 
     m.def("check_for_gpu", [] { return tk_gpu_available() != 0; }, "Check if the library was compiled with GPU support");
     m.def("device_count", [] {
