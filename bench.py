@@ -132,6 +132,72 @@ def _extra_default_levels(torch, tk, dev, ref, q, k, res_d, res_i, steps):
     return {"levels": tree.levels, "queries_per_s": q.shape[0] * steps / el, "ms_per_step": el * 1e3 / steps}
 
 
+def _extra_batch(np, torch, torch_knn, tk, workloads, dev, reps):
+    """Batching workload: 256 clouds, fwd + bwd, batched graph replay vs a per-cloud loop of the same kernels."""
+    clouds = workloads.batch_clouds()
+    refs = [c["ref"] for c in clouds]
+    qs = [c["queries"] for c in clouds]
+    ks = [c["k"] for c in clouds]
+    rng = np.random.default_rng(5)
+    ws = [rng.random((q.shape[0], k)).astype(np.float32) for q, k in zip(qs, ks)]
+    sum_q = sum(q.shape[0] for q in qs)
+    sum_n = sum(r.shape[0] for r in refs)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    t0 = time.perf_counter()
+    batch = tk.build_kd_tree_batch(refs, device=dev)
+    step = tk.BatchStep(batch, [torch.from_numpy(q) for q in qs], ks, [torch.from_numpy(w) for w in ws])
+    torch.cuda.synchronize()
+    plan_s = time.perf_counter() - t0
+    step.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step.replay()
+    torch.cuda.synchronize()
+    batched_ms = (time.perf_counter() - t0) * 1e3 / reps
+
+    # the same clouds as separate trees (same depths), one plain query + backward call per cloud
+    trees = [tk.build_kd_tree(torch.from_numpy(r), device=dev, levels=lv) for r, lv in zip(refs, batch.levels)]
+    bufs = []
+    for r, q, k, w in zip(refs, qs, ks, ws):
+        qd = torch.from_numpy(q).to(dev)
+        rd = torch.from_numpy(r).to(dev)
+        bufs.append((qd, rd, k, torch.empty((q.shape[0], k), device=dev),
+                     torch.empty((q.shape[0], k), dtype=torch.int32, device=dev),
+                     torch.empty((q.shape[0], k), dtype=torch.int64, device=dev), torch.from_numpy(w).to(dev),
+                     torch.empty_like(qd), torch.empty_like(rd)))
+
+    def loop():
+        for tree, (qd, rd, k, d, i, o, w, dq, dr) in zip(trees, bufs):
+            tree.kdtree.query(qd.data_ptr(), qd.shape[0], k, d.data_ptr(), i.data_ptr(), orig_idx_ptr=o.data_ptr(),
+                              stream=stream)
+            torch_knn.knn_backward(qd.data_ptr(), rd.data_ptr(), qd.shape[0], rd.shape[0], 3, "float32", k,
+                                   o.data_ptr(), w.data_ptr(), dq.data_ptr(), dr.data_ptr(), stream)
+
+    loop()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loop()
+    torch.cuda.synchronize()
+    loop_ms = (time.perf_counter() - t0) * 1e3 / reps
+    same = (torch.equal(step.dist, torch.cat([b[3].reshape(-1) for b in bufs]))
+            and torch.equal(step.orig, torch.cat([b[5].reshape(-1) for b in bufs]))
+            and torch.equal(step.grad_queries, torch.cat([b[7] for b in bufs]))
+            and torch.equal(step.grad_refs, torch.cat([b[8] for b in bufs])))
+    # algorithmic bytes per fwd+bwd: per query 12 + 8k (fwd: q, dist + idx) and 8k + 24 (bwd: g, idx, q, dq);
+    # per reference point 40 (tree records 16, r 12, dr 12)
+    alg = sum(q.shape[0] * (12 + 8 * k + 8 * k + 24) for q, k in zip(qs, ks)) + 40 * sum_n
+    return {"workload": "batch_clouds: 256 d=3 fp32 clouds (seed 3), N log-uniform [1e3, 5e5], Q = U(0.5,2) N, "
+                        "k in {1,4,16}; fwd + bwd, loss sum(w * dists)",
+            "clouds": len(clouds), "sum_refs": sum_n, "sum_queries": sum_q, "plan_and_capture_s": plan_s,
+            "batched_graph_ms": batched_ms, "batched_fwd_bwd_per_s": sum_q / (batched_ms * 1e-3),
+            "per_cloud_loop_ms": loop_ms, "per_cloud_fwd_bwd_per_s": sum_q / (loop_ms * 1e-3),
+            "batched_vs_per_cloud": loop_ms / batched_ms, "same_results": bool(same),
+            "batched_algorithmic_gbs": alg / (batched_ms * 1e-3) / 1e9}
+
+
 def _parse():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,7 +320,9 @@ def main():
         del res_d, res_i, q
         log("secondary: differentiation fwd+bwd")
         extras["fwd_bwd"] = _extra_fwd_bwd(np, torch, torch_knn, tk, workloads, dev, 10)
-        extras["batched_vs_per_cloud"] = None
+        log("secondary: batching (256 clouds)")
+        extras["batching"] = _extra_batch(np, torch, torch_knn, tk, workloads, dev, 3)
+        extras["batched_vs_per_cloud"] = extras["batching"]["batched_vs_per_cloud"]
 
     line = {
         "metric": "KNN queries/s at d=3,k=16 fp32 (+fwd+bwd/s); batched vs per-cloud, same GPU",

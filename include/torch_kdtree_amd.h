@@ -129,7 +129,7 @@ int tk_knn_bruteforce_gpu_workspace(int32_t n_ref, int64_t n_queries, int dims, 
 int tk_knn_backward_gpu(const void* queries, const void* ref, int64_t n_queries, int32_t n_ref, int dims, int dtype,
                         int k, const int64_t* idx, const void* grad_dist, void* grad_queries, void* grad_ref,
                         void* workspace, size_t workspace_bytes, void* stream);
-int tk_knn_backward_gpu_workspace(int64_t n_queries, int32_t n_ref, int k, size_t* bytes);
+int tk_knn_backward_gpu_workspace(int64_t n_queries, int32_t n_ref, int dims, int dtype, int k, size_t* bytes);
 
 /* Batched many-cloud mode (no counterpart in the reference, which queries one tree per call;
  * it replaces a loop of KDTree::query calls, src/interface.cpp:60-86, over many clouds).

@@ -6,52 +6,71 @@
 //   dL/dr_m = -sum_{(i,j): idx_ij = m} g_ij * 2 (q_i - r_m)
 // Each term is computed as g * (diff + diff) with diff = q - r rounded in T (the product
 // autograd forms for pow(diff, 2)), and nothing from the forward is stored: the differences
-// are recomputed from the coordinates.
+// are recomputed from the coordinates (once, in the dL/dq pass, which also keeps each term for
+// the dL/dr sums).
 //
 // dL/dq: one lane per query, terms added in neighbour order j = 0..k-1.
-// dL/dr: deterministic, no atomics.  The (neighbour index, flat position) pairs are sorted
-// by neighbour index with a stable LSD radix sort (radix_sort.hip), so each reference point's contributions
-// form a contiguous run in increasing flat position; one lane per run adds them in that
-// order.  The result is bit-identical from run to run.
+// dL/dr: deterministic, no atomics.  Queries are put in a canonical order: stably sorted by
+// (Morton cell over the reference points' bounding box, row), with canonical_morton_bits(Q, d)
+// bits per axis.  In that order each query's coordinates and upstream gradients are copied to
+// consecutive slots and its (reference index, slot) pairs are written; a stable radix sort of the
+// pairs by reference index then makes each reference point's contributions a contiguous run in
+// canonical order, and one lane per run subtracts them in that order.  Spatially close queries
+// are close in the canonical order, so a run's reads are close together.  The result is
+// bit-identical from run to run, and equal to the batched backward's on the same cloud.
+#include <algorithm>
 #include <memory>
 
 #include "gpu_common.hpp"
 #include "knn_gpu.hpp"
+#include "knn_tree.hpp"
 
 namespace tk {
 namespace {
 
+// dL/dq, and for dL/dr the canonical copies and pairs.  Lane t takes query order[t] (row t
+// without an order); with q_sorted it writes the query to slot t, and per neighbour j the
+// upstream gradient, the pair (reference index, t * k + j) to slot t * k + j.
 template <typename T, int D>
 __global__ __launch_bounds__(256) void grad_q_kernel(const T* __restrict__ q, const T* __restrict__ ref, int64_t nq,
                                                      int k, const int64_t* __restrict__ idx,
-                                                     const T* __restrict__ g, T* __restrict__ dq) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= nq) return;
+                                                     const T* __restrict__ g, const uint32_t* __restrict__ order,
+                                                     T* __restrict__ dq, T* __restrict__ q_sorted,
+                                                     T* __restrict__ g_sorted, uint32_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ vals) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= nq) return;
+    const int64_t i = order ? (int64_t)order[t] : t;
     T qi[D], acc[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         qi[a] = q[i * D + a];
         acc[a] = T(0);
     }
+    if (q_sorted) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) q_sorted[t * D + a] = qi[a];
+    }
     for (int j = 0; j < k; ++j) {
-        const int64_t m = idx[i * k + j];
-        const T gij = g[i * k + j];
+        const int64_t f = i * k + j;
+        const int64_t m = idx[f];
+        const T gij = g[f];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             const T diff = qi[a] - ref[m * D + a];
             acc[a] = acc[a] + gij * (diff + diff);
         }
+        if (q_sorted) {
+            const int64_t v = t * k + j;
+            g_sorted[v] = gij;
+            keys[v] = (uint32_t)m;
+            vals[v] = (uint32_t)v;
+        }
     }
+    if (dq) {
 #pragma unroll
-    for (int a = 0; a < D; ++a) dq[i * D + a] = acc[a];
-}
-
-__global__ __launch_bounds__(256) void pairs_kernel(const int64_t* __restrict__ idx, int64_t m,
-                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    const int64_t f = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (f >= m) return;
-    keys[f] = (uint32_t)idx[f];
-    vals[f] = (uint32_t)f;
+        for (int a = 0; a < D; ++a) dq[i * D + a] = acc[a];
+    }
 }
 
 template <typename T>
@@ -60,10 +79,12 @@ __global__ __launch_bounds__(256) void fill_kernel(T* __restrict__ p, int64_t n)
     if (i < n) p[i] = T(0);
 }
 
+// dL/dr: one lane per run of equal reference indices in the sorted pairs (in canonical order).
 template <typename T, int D>
-__global__ __launch_bounds__(256) void grad_r_kernel(const T* __restrict__ q, const T* __restrict__ ref, int64_t m,
-                                                     int k, const uint32_t* __restrict__ keys,
-                                                     const uint32_t* __restrict__ vals, const T* __restrict__ g,
+__global__ __launch_bounds__(256) void grad_r_kernel(const T* __restrict__ ref, int k, int64_t m,
+                                                     const uint32_t* __restrict__ keys,
+                                                     const uint32_t* __restrict__ vals,
+                                                     const T* __restrict__ q_sorted, const T* __restrict__ g_sorted,
                                                      T* __restrict__ dr) {
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (p >= m) return;
@@ -76,12 +97,12 @@ __global__ __launch_bounds__(256) void grad_r_kernel(const T* __restrict__ q, co
         acc[a] = T(0);
     }
     for (int64_t e = p; e < m && keys[e] == key; ++e) {
-        const uint32_t f = vals[e];
-        const int64_t i = f / (uint32_t)k;
-        const T gij = g[f];
+        const int64_t v = vals[e];
+        const int64_t t = v / k;
+        const T gij = g_sorted[v];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            const T diff = q[i * D + a] - r[a];
+            const T diff = q_sorted[t * D + a] - r[a];
             acc[a] = acc[a] - gij * (diff + diff);
         }
     }
@@ -91,27 +112,35 @@ __global__ __launch_bounds__(256) void grad_r_kernel(const T* __restrict__ q, co
 
 inline size_t a256(size_t b) { return (b + 255) & ~size_t(255); }
 
-int key_bits(int32_t n) {
+int key_bits(int64_t n) {
     int b = 1;
     while (b < 32 && (int64_t(1) << b) < n) ++b;
     return b;
 }
 
 struct BwLayout {
-    size_t keys, vals, keys_s, vals_s, temp, temp_bytes, total;
+    size_t qkey, qrow, qkey_s, order, bbox, q_sorted, g_sorted, keys, vals, keys_s, vals_s, temp, temp_bytes, total;
 };
 
-BwLayout bw_layout(int64_t nq, int32_t n, int k) {
+BwLayout bw_layout(int64_t nq, int32_t n, int k, int dims, int dtype) {
     (void)n;
     const int64_t m = nq * k;
+    const size_t ts = dtype_size(dtype);
     BwLayout l{};
     size_t off = 0;
+    l.qkey = off; off += a256(4 * (size_t)nq);
+    l.qrow = off; off += a256(4 * (size_t)nq);
+    l.qkey_s = off; off += a256(4 * (size_t)nq);
+    l.order = off; off += a256(4 * (size_t)nq);
+    l.bbox = off; off += a256(16 * (size_t)dims);
+    l.q_sorted = off; off += a256(ts * (size_t)dims * (size_t)nq);
+    l.g_sorted = off; off += a256(ts * (size_t)m);
     l.keys = off; off += a256(4 * (size_t)m);
     l.vals = off; off += a256(4 * (size_t)m);
     l.keys_s = off; off += a256(4 * (size_t)m);
     l.vals_s = off; off += a256(4 * (size_t)m);
     l.temp = off;
-    l.temp_bytes = radix_sort_workspace(m);
+    l.temp_bytes = std::max(radix_sort_workspace(m), radix_sort_workspace(nq));
     off += a256(l.temp_bytes);
     l.total = off;
     return l;
@@ -121,31 +150,54 @@ template <typename T, int D>
 void backward_typed(const T* q, const T* ref, int64_t nq, int32_t n, int k, const int64_t* idx, const T* g, T* dq,
                     T* dr, hipStream_t s, unsigned char* ws, const BwLayout& l) {
     const int64_t m = nq * k;
-    if (dq) {
-        hipLaunchKernelGGL((grad_q_kernel<T, D>), dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, q, ref, nq, k,
-                           idx, g, dq);
+    const bool pairs = dr && m > 0;
+    if (dr) {
+        hipLaunchKernelGGL((fill_kernel<T>), dim3((unsigned)(((int64_t)n * D + 255) / 256)), dim3(256), 0, s, dr,
+                           (int64_t)n * D);
         TK_HIP_CHECK(hipGetLastError());
     }
-    if (!dr) return;
-    hipLaunchKernelGGL((fill_kernel<T>), dim3((unsigned)(((int64_t)n * D + 255) / 256)), dim3(256), 0, s, dr,
-                       (int64_t)n * D);
+    if (nq == 0 || !(dq || pairs)) return;
+    const dim3 qgrid((unsigned)((nq + 255) / 256));
+    if (!pairs) {
+        hipLaunchKernelGGL((grad_q_kernel<T, D>), qgrid, dim3(256), 0, s, q, ref, nq, k, idx, g, nullptr, dq, nullptr,
+                           nullptr, nullptr, nullptr);
+        TK_HIP_CHECK(hipGetLastError());
+        return;
+    }
+    // canonical query order
+    uint32_t* qkey = reinterpret_cast<uint32_t*>(ws + l.qkey);
+    uint32_t* qrow = reinterpret_cast<uint32_t*>(ws + l.qrow);
+    uint32_t* qkey_s = reinterpret_cast<uint32_t*>(ws + l.qkey_s);
+    uint32_t* order = reinterpret_cast<uint32_t*>(ws + l.order);
+    auto* box = reinterpret_cast<OrderedT<T>*>(ws + l.bbox);
+    init_bbox(box, sizeof(T) == 4 ? kF32 : kF64, D, s);
+    const unsigned box_blocks = (unsigned)std::min<int64_t>(1024, ((int64_t)n + kBoxBlock - 1) / kBoxBlock);
+    hipLaunchKernelGGL((bbox_kernel<T, D>), dim3(box_blocks), dim3(kBoxBlock), 0, s, ref, (int64_t)n, box);
+    const int bits = canonical_morton_bits(nq, D);
+    hipLaunchKernelGGL((morton_bucket_kernel<T, D>), qgrid, dim3(256), 0, s, q, nq, box, bits, qkey, qrow);
     TK_HIP_CHECK(hipGetLastError());
-    if (m == 0) return;
+    radix_sort_pairs(qkey, qrow, qkey_s, order, nq, bits * D, ws + l.temp, l.temp_bytes, s);
+    // dL/dq, canonical copies and pairs; sort the pairs by reference index; dL/dr per run
+    T* q_sorted = reinterpret_cast<T*>(ws + l.q_sorted);
+    T* g_sorted = reinterpret_cast<T*>(ws + l.g_sorted);
     uint32_t* keys = reinterpret_cast<uint32_t*>(ws + l.keys);
     uint32_t* vals = reinterpret_cast<uint32_t*>(ws + l.vals);
     uint32_t* keys_s = reinterpret_cast<uint32_t*>(ws + l.keys_s);
     uint32_t* vals_s = reinterpret_cast<uint32_t*>(ws + l.vals_s);
-    hipLaunchKernelGGL(pairs_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, idx, m, keys, vals);
+    hipLaunchKernelGGL((grad_q_kernel<T, D>), qgrid, dim3(256), 0, s, q, ref, nq, k, idx, g, order, dq, q_sorted,
+                       g_sorted, keys, vals);
     TK_HIP_CHECK(hipGetLastError());
     radix_sort_pairs(keys, vals, keys_s, vals_s, m, key_bits(n), ws + l.temp, l.temp_bytes, s);
-    hipLaunchKernelGGL((grad_r_kernel<T, D>), dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, q, ref, m, k, keys_s,
-                       vals_s, g, dr);
+    hipLaunchKernelGGL((grad_r_kernel<T, D>), dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, ref, k, m, keys_s,
+                       vals_s, q_sorted, g_sorted, dr);
     TK_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace
 
-size_t knn_backward_workspace(int64_t nq, int32_t n, int k) { return bw_layout(nq, n, k).total; }
+size_t knn_backward_workspace(int64_t nq, int32_t n, int k, int dims, int dtype) {
+    return bw_layout(nq, n, k, dims, dtype).total;
+}
 
 void knn_backward_gpu(const void* queries, const void* ref, int64_t nq, int32_t n, int dims, int dtype, int k,
                       const int64_t* idx, const void* grad_dist, void* grad_q, void* grad_ref, hipStream_t stream,
@@ -153,7 +205,7 @@ void knn_backward_gpu(const void* queries, const void* ref, int64_t nq, int32_t 
     if (!dims_supported(dims)) throw std::runtime_error("unsupported dims");
     if (k < 1 || n < 1) throw std::runtime_error("invalid backward shape");
     if (nq * (int64_t)k > (int64_t(1) << 28)) throw std::runtime_error("backward: nr_query * k must be <= 2^28");
-    const BwLayout l = bw_layout(nq, n, k);
+    const BwLayout l = bw_layout(nq, n, k, dims, dtype);
     std::unique_ptr<StreamScratch> own;
     unsigned char* ws = static_cast<unsigned char*>(workspace);
     if (grad_ref) {

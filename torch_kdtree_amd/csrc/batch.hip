@@ -15,6 +15,11 @@ namespace {
 
 // Cloud of each query row: the last cloud whose first row is <= the row (clouds with no queries
 // share their first row with the next cloud and are never the last such cloud for a real row).
+__global__ __launch_bounds__(256) void batch_iota_kernel(int32_t* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = (int32_t)i;
+}
+
 __global__ __launch_bounds__(256) void batch_qcloud_kernel(const CloudDesc* __restrict__ clouds, int n_clouds,
                                                            int64_t total_q, int32_t* __restrict__ qcloud) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -34,16 +39,6 @@ int kbucket_of(int k) {
     int K = 1;
     while (K < k) K <<= 1;
     return K;
-}
-
-int morton_max_bits(int dims) {
-    switch (dims) {
-        case 1: return MortonBits<1>::kBits;
-        case 2: return MortonBits<2>::kBits;
-        case 3: return MortonBits<3>::kBits;
-        case 8: return MortonBits<8>::kBits;
-    }
-    throw std::runtime_error("unsupported dims");
 }
 
 inline size_t a256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -109,7 +104,13 @@ void enqueue_forward(BatchPlan& p, const void* q, void* dist, int32_t* idx, int6
 }
 
 void enqueue_backward(BatchPlan& p, const void* q, const int32_t* idx, const void* g, void* dq, void* dr,
-                      hipStream_t s) {
+                      bool reuse_order, hipStream_t s) {
+    if (!reuse_order && p.total_q > 0) {
+        if (p.dtype == kF32)
+            batch_presort_f32(p, static_cast<const float*>(q), s);
+        else
+            batch_presort_f64(p, static_cast<const double*>(q), s);
+    }
     if (p.dtype == kF32)
         batch_backward_f32(p, static_cast<const float*>(q), idx, static_cast<const float*>(g), static_cast<float*>(dq),
                            static_cast<float*>(dr), s);
@@ -122,8 +123,11 @@ void enqueue_backward(BatchPlan& p, const void* q, const int32_t* idx, const voi
 
 void batch_init_qcloud(const BatchPlan& p, hipStream_t s) {
     if (p.total_q == 0) return;
-    hipLaunchKernelGGL(batch_qcloud_kernel, dim3((unsigned)((p.total_q + 255) / 256)), dim3(256), 0, s, p.d_clouds,
-                       p.n_clouds, p.total_q, p.qcloud);
+    const dim3 grid((unsigned)((p.total_q + 255) / 256));
+    hipLaunchKernelGGL(batch_qcloud_kernel, grid, dim3(256), 0, s, p.d_clouds, p.n_clouds, p.total_q, p.qcloud);
+    TK_HIP_CHECK(hipGetLastError());
+    // a valid permutation before any forward has run (the backward walks queries in this order)
+    hipLaunchKernelGGL(batch_iota_kernel, grid, dim3(256), 0, s, p.order, p.total_q);
     TK_HIP_CHECK(hipGetLastError());
 }
 
@@ -183,27 +187,28 @@ BatchPlan* batch_create(const Tree* const* trees, int n_clouds, const int64_t* n
     p->total_q = qb;
     p->total_out = ob;
 
-    // Morton grids: about 8 queries per cell; buckets ordered by k bucket, then cloud
-    const int maxb = morton_max_bits(p->dims);
-    for (CloudDesc& cd : p->clouds) {
-        int b = 0;
-        while (b < maxb && (int64_t(8) << ((b + 1) * p->dims)) <= (int64_t)cd.n_queries) ++b;
-        cd.morton_bits = b;
-    }
-    int64_t nb = 0, pos = 0;
+    // Morton grids (the canonical rule); cells, sorted positions and canonical slots ordered by
+    // k bucket, then cloud
+    for (CloudDesc& cd : p->clouds) cd.morton_bits = canonical_morton_bits(cd.n_queries, p->dims);
+    int64_t nb = 0, pos = 0, slot = 0;
     for (int K = 1; K <= kBatchMaxK; K <<= 1) {
         BatchPlan::Range r{K, pos, 0};
         for (CloudDesc& cd : p->clouds) {
             if (cd.kbucket != K) continue;
             cd.bucket_base = (int32_t)nb;
+            cd.sorted_q_base = pos + r.count;
+            cd.sorted_out_base = slot;
             nb += int64_t(1) << (cd.morton_bits * p->dims);
             r.count += cd.n_queries;
+            slot += (int64_t)cd.n_queries * cd.k;
         }
         if (r.count > 0) p->ranges.push_back(r);
         pos += r.count;
     }
-    if (nb > (int64_t(1) << 24)) throw std::runtime_error("batch: too many presort buckets");
-    p->total_buckets = (int32_t)nb;
+    if (nb >= (int64_t(1) << 31)) throw std::runtime_error("batch: too many presort cells");
+    int kb = 1;
+    while (kb < 32 && (int64_t(1) << kb) < nb) ++kb;
+    p->key_bits = kb;
 
     // host staging of the arena
     const size_t ts = dtype_size(p->dtype);
@@ -211,9 +216,11 @@ BatchPlan* batch_create(const Tree* const* trees, int n_clouds, const int64_t* n
     const size_t bb = (size_t)box_bytes_of(p->dtype, p->dims);
     std::vector<unsigned char> h_recs((size_t)(rec + kRecPad) * rb, 0);
     std::vector<unsigned char> h_split((size_t)node * ts);
-    std::vector<int32_t> h_leaf((size_t)leaf), h_shuf((size_t)rec), h_inv((size_t)rec);
+    std::vector<int32_t> h_leaf((size_t)leaf), h_shuf((size_t)rec), h_inv((size_t)rec), h_rcl((size_t)rec);
     std::vector<unsigned char> h_boxes((size_t)box * bb, 0);
     for (int c = 0; c < n_clouds; ++c) {
+        const CloudDesc& cd = p->clouds[c];
+        std::fill(h_rcl.begin() + cd.rec_base, h_rcl.begin() + cd.rec_base + cd.n_points, c);
         if (p->dtype == kF32)
             stage_cloud_t<float>(*trees[c], p->clouds[c], h_recs.data(), h_split.data(), h_leaf.data(), h_shuf.data(),
                                  h_inv.data(), h_boxes.data());
@@ -228,6 +235,7 @@ BatchPlan* batch_create(const Tree* const* trees, int n_clouds, const int64_t* n
     dmalloc(&p->leaf_off, 4 * h_leaf.size());
     dmalloc(&p->shuffled, 4 * h_shuf.size());
     dmalloc(&p->inv_shuffled, 4 * h_inv.size());
+    dmalloc(&p->rec_cloud, 4 * h_rcl.size());
     dmalloc(&p->boxes, h_boxes.size());
     dmalloc(&p->d_clouds, sizeof(CloudDesc) * (size_t)n_clouds);
     TK_HIP_CHECK(hipMemcpy(p->recs, h_recs.data(), h_recs.size(), hipMemcpyHostToDevice));
@@ -235,22 +243,23 @@ BatchPlan* batch_create(const Tree* const* trees, int n_clouds, const int64_t* n
     TK_HIP_CHECK(hipMemcpy(p->leaf_off, h_leaf.data(), 4 * h_leaf.size(), hipMemcpyHostToDevice));
     TK_HIP_CHECK(hipMemcpy(p->shuffled, h_shuf.data(), 4 * h_shuf.size(), hipMemcpyHostToDevice));
     TK_HIP_CHECK(hipMemcpy(p->inv_shuffled, h_inv.data(), 4 * h_inv.size(), hipMemcpyHostToDevice));
+    TK_HIP_CHECK(hipMemcpy(p->rec_cloud, h_rcl.data(), 4 * h_rcl.size(), hipMemcpyHostToDevice));
     TK_HIP_CHECK(hipMemcpy(p->boxes, h_boxes.data(), h_boxes.size(), hipMemcpyHostToDevice));
     TK_HIP_CHECK(hipMemcpy(p->d_clouds, p->clouds.data(), sizeof(CloudDesc) * (size_t)n_clouds,
                            hipMemcpyHostToDevice));
     const size_t nq = (size_t)p->total_q, no = (size_t)p->total_out;
     dmalloc(&p->qcloud, 4 * nq);
-    dmalloc(&p->bucket, 4 * nq);
-    dmalloc(&p->rank, 4 * nq);
+    dmalloc(&p->key, 4 * nq);
+    dmalloc(&p->row, 4 * nq);
+    dmalloc(&p->key_s, 4 * nq);
     dmalloc(&p->order, 4 * nq);
-    dmalloc(&p->counts, 4 * (size_t)p->total_buckets);
-    dmalloc(&p->offsets, 4 * (size_t)p->total_buckets);
-    dmalloc(&p->sums, 4 * (size_t)scan_sums_count(p->total_buckets));
+    dmalloc(reinterpret_cast<unsigned char**>(&p->q_sorted), ts * (size_t)p->dims * nq);
+    dmalloc(reinterpret_cast<unsigned char**>(&p->g_sorted), ts * no);
     dmalloc(&p->keys, 4 * no);
     dmalloc(&p->vals, 4 * no);
     dmalloc(&p->keys_s, 4 * no);
     dmalloc(&p->vals_s, 4 * no);
-    p->radix_ws_bytes = radix_sort_workspace((int64_t)no);
+    p->radix_ws_bytes = std::max(radix_sort_workspace((int64_t)no), radix_sort_workspace((int64_t)nq));
     dmalloc(reinterpret_cast<unsigned char**>(&p->radix_ws), p->radix_ws_bytes);
     batch_init_qcloud(*p, nullptr);
     TK_HIP_CHECK(hipDeviceSynchronize());
@@ -269,15 +278,16 @@ void batch_free(BatchPlan* p) noexcept {
     dfree(p->leaf_off);
     dfree(p->shuffled);
     dfree(p->inv_shuffled);
+    dfree(p->rec_cloud);
     dfree(p->boxes);
     dfree(p->d_clouds);
     dfree(p->qcloud);
-    dfree(p->bucket);
-    dfree(p->rank);
+    dfree(p->key);
+    dfree(p->row);
+    dfree(p->key_s);
     dfree(p->order);
-    dfree(p->counts);
-    dfree(p->offsets);
-    dfree(p->sums);
+    dfree(p->q_sorted);
+    dfree(p->g_sorted);
     dfree(p->keys);
     dfree(p->vals);
     dfree(p->keys_s);
@@ -297,7 +307,7 @@ void batch_backward(BatchPlan& p, const void* queries, const int32_t* idx, const
                     void* grad_ref, hipStream_t s) {
     DeviceGuard guard(p.device);
     ProfScope prof("batch_backward", s);
-    enqueue_backward(p, queries, idx, grad_dist, grad_q, grad_ref, s);
+    enqueue_backward(p, queries, idx, grad_dist, grad_q, grad_ref, false, s);
 }
 
 void batch_capture(BatchPlan& p, const void* queries, void* dist, int32_t* idx, int64_t* orig, const void* grad_dist,
@@ -323,7 +333,8 @@ void batch_capture(BatchPlan& p, const void* queries, void* dist, int32_t* idx, 
     TK_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     try {
         enqueue_forward(p, queries, dist, idx, orig, cs);
-        if (grad_dist && (grad_q || grad_ref)) enqueue_backward(p, queries, idx, grad_dist, grad_q, grad_ref, cs);
+        if (grad_dist && (grad_q || grad_ref))
+            enqueue_backward(p, queries, idx, grad_dist, grad_q, grad_ref, true, cs);
     } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(cs, &g);

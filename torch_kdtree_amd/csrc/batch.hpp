@@ -6,12 +6,14 @@
 // kind (distances, structured int32 indices, original int64 indices), cloud c owning the
 // n_queries x k elements from out_base.  Per query the plan knows its cloud (qcloud).
 //
-// Forward: one Morton presort over all queries, with each cloud's buckets placed so that the
-// sorted order groups clouds by their k bucket (K = next power of two >= k); then one traversal
-// launch per k bucket over its contiguous range of the order.  Backward: dL/dq per query, and
-// dL/dr through one stable sort of (reference record, query/neighbour) pairs and per-run sums,
-// which gives exactly the summation order of the single-tree backward.  All scratch memory is
-// allocated with the plan, so a forward + backward sequence can be captured in a HIP graph.
+// Forward: one presort over all queries: stable sort by Morton cell over the cloud's root box
+// (the bounding box of its points) with canonical_morton_bits(Q_c, d) bits, each cloud's cells
+// placed so that the order groups clouds by their k bucket (K = next power of two >= k); then
+// one traversal launch per k bucket over its contiguous range of the order.  Within a cloud this
+// order is the single-tree backward's canonical order.  Backward: dL/dq per query, and dL/dr
+// through one stable sort of (reference record, canonical slot) pairs and per-run sums, which is
+// exactly the summation order of the single-tree backward.  All scratch memory is allocated with
+// the plan, so a forward + backward sequence can be captured in a HIP graph.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -36,8 +38,10 @@ struct CloudDesc {
     int64_t out_base;   // first output element
     int32_t n_queries;
     int32_t bucket_base;  // first Morton bucket of the cloud
-    int32_t morton_bits;  // bits per axis of the cloud's Morton grid
+    int32_t morton_bits;  // bits per axis of the cloud's Morton grid (canonical_morton_bits)
     int32_t kbucket;      // K: the register list size its queries run with
+    int64_t sorted_q_base;    // first position of the cloud's queries in the sorted order
+    int64_t sorted_out_base;  // first canonical slot of the cloud's (query, neighbour) pairs
 };
 
 constexpr int kBatchMaxK = 32;
@@ -63,18 +67,20 @@ struct BatchPlan {
     int32_t* inv_shuffled = nullptr;
     unsigned char* boxes = nullptr;
     CloudDesc* d_clouds = nullptr;
-    int32_t* qcloud = nullptr;
+    int32_t* qcloud = nullptr;     // [total_q] cloud of each query row
+    int32_t* rec_cloud = nullptr;  // [total_ref] cloud of each record
     // scratch
-    int32_t* bucket = nullptr;
-    int32_t* rank = nullptr;
-    int32_t* order = nullptr;
-    int32_t* counts = nullptr;
-    int32_t* offsets = nullptr;
-    int32_t* sums = nullptr;
-    uint32_t* keys = nullptr;
+    uint32_t* key = nullptr;    // [total_q] presort keys
+    uint32_t* row = nullptr;    // [total_q] query rows
+    uint32_t* key_s = nullptr;  // [total_q] sorted keys
+    int32_t* order = nullptr;   // [total_q] query rows in sorted order
+    void* q_sorted = nullptr;   // [total_q * dims] queries in sorted order
+    void* g_sorted = nullptr;   // [total_out] upstream gradients in canonical slots
+    uint32_t* keys = nullptr;   // [total_out] pairs (record, canonical slot)
     uint32_t* vals = nullptr;
     uint32_t* keys_s = nullptr;
     uint32_t* vals_s = nullptr;
+    int key_bits = 1;           // bits of the presort keys
     void* radix_ws = nullptr;
     size_t radix_ws_bytes = 0;
     // captured forward + backward
@@ -88,7 +94,8 @@ BatchPlan* batch_create(const Tree* const* trees, int n_clouds, const int64_t* n
                         int device);
 void batch_free(BatchPlan* p) noexcept;
 void batch_forward(BatchPlan& p, const void* queries, void* dist, int32_t* idx, int64_t* orig, hipStream_t s);
-// idx: the structured indices the forward wrote; either gradient may be null.
+// idx: the structured indices the forward wrote; either gradient may be null.  The canonical
+// order is recomputed from the queries.
 void batch_backward(BatchPlan& p, const void* queries, const int32_t* idx, const void* grad_dist, void* grad_q,
                     void* grad_ref, hipStream_t s);
 void batch_capture(BatchPlan& p, const void* queries, void* dist, int32_t* idx, int64_t* orig, const void* grad_dist,

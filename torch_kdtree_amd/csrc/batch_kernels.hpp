@@ -13,47 +13,34 @@ namespace {
 
 constexpr int kBatchBlock = 256;
 
-__global__ __launch_bounds__(kBatchBlock) void batch_fill_i32_kernel(int32_t* __restrict__ p, int64_t n, int32_t v) {
-    const int64_t i = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 template <typename T>
 __global__ __launch_bounds__(kBatchBlock) void batch_zero_kernel(T* __restrict__ p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
     if (i < n) p[i] = T(0);
 }
 
-// Morton bucket of every query over its cloud's root box (the box of the cloud's points; queries
-// outside it are clamped to the boundary cells, which holds queries with similar searches).
+// Presort key of every query: the cloud's first cell plus the Morton code of the query's cell
+// over the cloud's root box (the bounding box of its points), with the cloud's canonical grid.
 template <typename T, int D>
 __global__ __launch_bounds__(kBatchBlock) void batch_bucket_kernel(const unsigned char* __restrict__ boxes,
                                                                     const CloudDesc* __restrict__ clouds,
                                                                     const int32_t* __restrict__ qcloud,
                                                                     const T* __restrict__ queries, int64_t total_q,
-                                                                    int32_t* __restrict__ bucket,
-                                                                    int32_t* __restrict__ rank,
-                                                                    int32_t* __restrict__ counts) {
+                                                                    uint32_t* __restrict__ key,
+                                                                    uint32_t* __restrict__ row) {
     const int64_t i = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
     if (i >= total_q) return;
     const CloudDesc& cd = clouds[qcloud[i]];
     using L = BoxLayout<T, D>;
     const T* box = reinterpret_cast<const T*>(boxes + (size_t)cd.box_base * L::kBytes);
-    const int B = cd.morton_bits;
-    const float cells = (float)(1 << B);
-    uint32_t code = 0;
+    float lo[D], hi[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-        const float lo = (float)box[a];
-        const float hi = (float)box[L::kPad + a];
-        const float scale = hi > lo ? cells / (hi - lo) : 0.0f;
-        const float x = ((float)queries[i * D + a] - lo) * scale;
-        const uint32_t g = (uint32_t)fminf(fmaxf(x, 0.0f), cells - 1.0f);
-        for (int b = 0; b < B; ++b) code |= ((g >> b) & 1u) << (b * D + a);
+        lo[a] = (float)box[a];
+        hi[a] = (float)box[L::kPad + a];
     }
-    const int32_t bk = cd.bucket_base + (int32_t)code;
-    bucket[i] = bk;
-    rank[i] = atomicAdd(&counts[bk], 1);
+    key[i] = (uint32_t)cd.bucket_base + morton_cell<T, D>(lo, hi, cd.morton_bits, queries + i * D);
+    row[i] = (uint32_t)i;
 }
 
 // One lane per query of the k bucket K; `order` holds the presorted query rows.
@@ -106,85 +93,85 @@ __global__ __launch_bounds__(kTreeBlock) void knn_batch_kernel(TreeView<T, true>
     }
 }
 
-// dL/dq, one lane per query (the single-tree grad_q_kernel's arithmetic and order).
+// dL/dq, and for dL/dr the canonical copies and pairs.  Lane t takes query order[t] (the sorted
+// order of the forward, the canonical order within each cloud); with q_sorted it writes the query
+// to position t and, per neighbour j, the upstream gradient and the pair (arena record, slot) to
+// the canonical slot sorted_out_base + (t - sorted_q_base) * k + j.  dL/dq adds the terms in
+// neighbour order (the single-tree grad_q_kernel's arithmetic and order).
 template <typename T, int D>
-__global__ __launch_bounds__(kBatchBlock) void batch_grad_q_kernel(const unsigned char* __restrict__ recs,
-                                                                    const CloudDesc* __restrict__ clouds,
-                                                                    const int32_t* __restrict__ qcloud,
-                                                                    const T* __restrict__ queries, int64_t total_q,
-                                                                    const int32_t* __restrict__ idx,
-                                                                    const T* __restrict__ g, T* __restrict__ dq) {
-    const int64_t i = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
-    if (i >= total_q) return;
+__global__ __launch_bounds__(kBatchBlock) void batch_grad_q_pairs_kernel(
+    const unsigned char* __restrict__ recs, const CloudDesc* __restrict__ clouds, const int32_t* __restrict__ qcloud,
+    const int32_t* __restrict__ order, const T* __restrict__ queries, int64_t total_q, const int32_t* __restrict__ idx,
+    const T* __restrict__ g, T* __restrict__ dq, T* __restrict__ q_sorted, T* __restrict__ g_sorted,
+    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int64_t t = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
+    if (t >= total_q) return;
+    const int64_t i = order[t];
     const CloudDesc cd = clouds[qcloud[i]];
     const int64_t ob = cd.out_base + (i - cd.q_base) * cd.k;
+    const int64_t sb = cd.sorted_out_base + (t - cd.sorted_q_base) * cd.k;
     T qi[D], acc[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         qi[a] = queries[i * D + a];
         acc[a] = T(0);
     }
+    if (q_sorted) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) q_sorted[t * D + a] = qi[a];
+    }
     for (int j = 0; j < cd.k; ++j) {
-        const Rec<T, D> r = load_rec<T, D>(recs, cd.rec_base + idx[ob + j]);
+        const int32_t s = cd.rec_base + idx[ob + j];
+        const Rec<T, D> r = load_rec<T, D>(recs, s);
         const T gij = g[ob + j];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             const T diff = qi[a] - r.c[a];
             acc[a] = acc[a] + gij * (diff + diff);
         }
+        if (q_sorted) {
+            g_sorted[sb + j] = gij;
+            keys[sb + j] = (uint32_t)s;
+            vals[sb + j] = (uint32_t)(sb + j);
+        }
     }
+    if (dq) {
 #pragma unroll
-    for (int a = 0; a < D; ++a) dq[i * D + a] = acc[a];
-}
-
-// (arena record, query row << 5 | neighbour slot) per output element, in output order.
-__global__ __launch_bounds__(kBatchBlock) void batch_pairs_kernel(const CloudDesc* __restrict__ clouds,
-                                                                   const int32_t* __restrict__ qcloud, int64_t total_q,
-                                                                   const int32_t* __restrict__ idx,
-                                                                   uint32_t* __restrict__ keys,
-                                                                   uint32_t* __restrict__ vals) {
-    const int64_t i = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
-    if (i >= total_q) return;
-    const CloudDesc cd = clouds[qcloud[i]];
-    const int64_t ob = cd.out_base + (i - cd.q_base) * cd.k;
-    for (int j = 0; j < cd.k; ++j) {
-        keys[ob + j] = (uint32_t)(cd.rec_base + idx[ob + j]);
-        vals[ob + j] = ((uint32_t)i << 5) | (uint32_t)j;
+        for (int a = 0; a < D; ++a) dq[i * D + a] = acc[a];
     }
 }
 
-// dL/dr: one lane per run of equal records in the sorted pairs; the run is in output order.
+// dL/dr: one lane per run of equal records in the sorted pairs; the run is in canonical order.
 template <typename T, int D>
 __global__ __launch_bounds__(kBatchBlock) void batch_grad_r_kernel(const unsigned char* __restrict__ recs,
+                                                                    const int32_t* __restrict__ rec_cloud,
                                                                     const int32_t* __restrict__ shuffled,
-                                                                    const CloudDesc* __restrict__ clouds,
-                                                                    const int32_t* __restrict__ qcloud,
-                                                                    const T* __restrict__ queries, int64_t m,
+                                                                    const CloudDesc* __restrict__ clouds, int64_t m,
                                                                     const uint32_t* __restrict__ keys,
                                                                     const uint32_t* __restrict__ vals,
-                                                                    const T* __restrict__ g, T* __restrict__ dr) {
+                                                                    const T* __restrict__ q_sorted,
+                                                                    const T* __restrict__ g_sorted,
+                                                                    T* __restrict__ dr) {
     const int64_t p = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
     if (p >= m) return;
     const uint32_t key = keys[p];
     if (p > 0 && keys[p - 1] == key) return;  // not the head of its run
+    const CloudDesc cd = clouds[rec_cloud[key]];
     const Rec<T, D> r = load_rec<T, D>(recs, (int32_t)key);
     T acc[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) acc[a] = T(0);
-    int32_t rec_base = 0;
     for (int64_t e = p; e < m && keys[e] == key; ++e) {
-        const uint32_t v = vals[e];
-        const int64_t qi = (int64_t)(v >> 5);
-        const CloudDesc& cd = clouds[qcloud[qi]];
-        rec_base = cd.rec_base;
-        const T gij = g[cd.out_base + (qi - cd.q_base) * cd.k + (int64_t)(v & 31u)];
+        const int64_t v = vals[e];
+        const int64_t t = cd.sorted_q_base + (v - cd.sorted_out_base) / cd.k;
+        const T gij = g_sorted[v];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            const T diff = queries[qi * D + a] - r.c[a];
+            const T diff = q_sorted[t * D + a] - r.c[a];
             acc[a] = acc[a] - gij * (diff + diff);
         }
     }
-    const int64_t row = (int64_t)rec_base + shuffled[key];
+    const int64_t row = (int64_t)cd.rec_base + shuffled[key];
 #pragma unroll
     for (int a = 0; a < D; ++a) dr[row * D + a] = acc[a];
 }
@@ -199,13 +186,10 @@ inline dim3 batch_grid(int64_t n) { return dim3((unsigned)((n + kBatchBlock - 1)
 
 template <typename T>
 void batch_presort_t(const BatchPlan& p, const T* q, hipStream_t s) {
-    hipLaunchKernelGGL(batch_fill_i32_kernel, batch_grid(p.total_buckets), dim3(kBatchBlock), 0, s, p.counts,
-                       (int64_t)p.total_buckets, 0);
-    TK_HIP_CHECK(hipGetLastError());
 #define TK_DIMS(DD)                                                                                                  \
     case DD:                                                                                                         \
         hipLaunchKernelGGL((batch_bucket_kernel<T, DD>), batch_grid(p.total_q), dim3(kBatchBlock), 0, s, p.boxes,   \
-                           p.d_clouds, p.qcloud, q, p.total_q, p.bucket, p.rank, p.counts);                          \
+                           p.d_clouds, p.qcloud, q, p.total_q, p.key, p.row);                                        \
         break;
     switch (p.dims) {
         TK_DIMS(1)
@@ -216,10 +200,8 @@ void batch_presort_t(const BatchPlan& p, const T* q, hipStream_t s) {
     }
 #undef TK_DIMS
     TK_HIP_CHECK(hipGetLastError());
-    exclusive_scan_i32(p.counts, p.total_buckets, p.sums, p.offsets, s);
-    hipLaunchKernelGGL(bucket_scatter_kernel, batch_grid(p.total_q), dim3(kBatchBlock), 0, s, p.total_q, p.bucket,
-                       p.rank, p.offsets, p.order);
-    TK_HIP_CHECK(hipGetLastError());
+    radix_sort_pairs(p.key, p.row, p.key_s, reinterpret_cast<uint32_t*>(p.order), p.total_q, p.key_bits, p.radix_ws,
+                     p.radix_ws_bytes, s);
 }
 
 template <typename T, int D, bool kLower>
@@ -274,28 +256,29 @@ inline int batch_key_bits(int64_t n) {
 
 template <typename T>
 void batch_backward_t(const BatchPlan& p, const T* q, const int32_t* idx, const T* g, T* dq, T* dr, hipStream_t s) {
+    const bool pairs = dr && p.total_out > 0;
+    T* qs = pairs ? static_cast<T*>(p.q_sorted) : nullptr;
+    T* gs = pairs ? static_cast<T*>(p.g_sorted) : nullptr;
 #define TK_DIMS(DD)                                                                                                  \
     case DD:                                                                                                         \
-        if (dq && p.total_q > 0) {                                                                                   \
-            hipLaunchKernelGGL((batch_grad_q_kernel<T, DD>), batch_grid(p.total_q), dim3(kBatchBlock), 0, s, p.recs, \
-                               p.d_clouds, p.qcloud, q, p.total_q, idx, g, dq);                                      \
-            TK_HIP_CHECK(hipGetLastError());                                                                         \
-        }                                                                                                            \
         if (dr) {                                                                                                    \
             hipLaunchKernelGGL((batch_zero_kernel<T>), batch_grid(p.total_ref * DD), dim3(kBatchBlock), 0, s, dr,    \
                                p.total_ref * DD);                                                                    \
             TK_HIP_CHECK(hipGetLastError());                                                                         \
-            if (p.total_out > 0) {                                                                                   \
-                hipLaunchKernelGGL(batch_pairs_kernel, batch_grid(p.total_q), dim3(kBatchBlock), 0, s, p.d_clouds,   \
-                                   p.qcloud, p.total_q, idx, p.keys, p.vals);                                        \
-                TK_HIP_CHECK(hipGetLastError());                                                                     \
-                radix_sort_pairs(p.keys, p.vals, p.keys_s, p.vals_s, p.total_out, batch_key_bits(p.total_ref),       \
-                                 p.radix_ws, p.radix_ws_bytes, s);                                                   \
-                hipLaunchKernelGGL((batch_grad_r_kernel<T, DD>), batch_grid(p.total_out), dim3(kBatchBlock), 0, s,   \
-                                   p.recs, p.shuffled, p.d_clouds, p.qcloud, q, p.total_out, p.keys_s, p.vals_s, g,  \
-                                   dr);                                                                              \
-                TK_HIP_CHECK(hipGetLastError());                                                                     \
-            }                                                                                                        \
+        }                                                                                                            \
+        if ((dq || pairs) && p.total_q > 0) {                                                                        \
+            hipLaunchKernelGGL((batch_grad_q_pairs_kernel<T, DD>), batch_grid(p.total_q), dim3(kBatchBlock), 0, s,   \
+                               p.recs, p.d_clouds, p.qcloud, p.order, q, p.total_q, idx, g, dq, qs, gs, p.keys,      \
+                               p.vals);                                                                              \
+            TK_HIP_CHECK(hipGetLastError());                                                                         \
+        }                                                                                                            \
+        if (pairs) {                                                                                                 \
+            radix_sort_pairs(p.keys, p.vals, p.keys_s, p.vals_s, p.total_out, batch_key_bits(p.total_ref),           \
+                             p.radix_ws, p.radix_ws_bytes, s);                                                       \
+            hipLaunchKernelGGL((batch_grad_r_kernel<T, DD>), batch_grid(p.total_out), dim3(kBatchBlock), 0, s,       \
+                               p.recs, p.rec_cloud, p.shuffled, p.d_clouds, p.total_out, p.keys_s, p.vals_s, qs, gs, \
+                               dr);                                                                                  \
+            TK_HIP_CHECK(hipGetLastError());                                                                         \
         }                                                                                                            \
         break;
     switch (p.dims) {

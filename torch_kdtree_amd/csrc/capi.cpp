@@ -226,8 +226,11 @@ int tk_knn_backward_gpu(const void* queries, const void* ref, int64_t n_queries,
     });
 }
 
-int tk_knn_backward_gpu_workspace(int64_t n_queries, int32_t n_ref, int k, size_t* bytes) {
-    return guarded([&] { *bytes = tk::knn_backward_workspace(n_queries, n_ref, k); });
+int tk_knn_backward_gpu_workspace(int64_t n_queries, int32_t n_ref, int dims, int dtype, int k, size_t* bytes) {
+    return guarded([&] {
+        check_dtype(dtype);
+        *bytes = tk::knn_backward_workspace(n_queries, n_ref, k, dims, dtype);
+    });
 }
 
 int tk_batch_create(const tk_tree* const* trees, int n_trees, const int64_t* n_queries, const int32_t* ks, int tie,

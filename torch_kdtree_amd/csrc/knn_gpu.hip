@@ -214,6 +214,11 @@ __device__ __forceinline__ int wave_inclusive_scan(int v) {
     return v;
 }
 
+// Exclusive scan of n int32 counts (n <= kScanChunk * kScanChunk) in three kernels.
+constexpr int kScanBlock = 1024;
+constexpr int kScanItems = 4;
+constexpr int kScanChunk = kScanBlock * kScanItems;
+
 // Exclusive scan over a block of kScanBlock threads; `total` receives the block sum.
 __device__ __forceinline__ int block_exclusive_scan(int v, int* wsum, int& total) {
     const int lane = threadIdx.x & 63;
@@ -288,12 +293,6 @@ __global__ __launch_bounds__(kScanBlock) void scan_apply_kernel(const int32_t* _
     }
 }
 
-__global__ void bucket_scatter_kernel(int64_t nq, const int32_t* __restrict__ bucket, const int32_t* __restrict__ rank,
-                                      const int32_t* __restrict__ offsets, int32_t* __restrict__ order) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= nq) return;
-    order[offsets[bucket[i]] + rank[i]] = (int32_t)i;
-}
 
 // ------------------------------------------------------------------------------------------
 // Query driver
@@ -330,22 +329,28 @@ int presort_mode(const Tree& t, int64_t nq, const QueryOpts& o) {
 int bucket_shift(const Tree& t) { return std::max(0, t.levels - kMaxBucketBits); }
 
 struct PresortLayout {
-    size_t bucket, rank, order, counts, offsets, sums, bbox, total;
-    int32_t nb, nsums;
+    size_t key, row, key_s, order, bbox, radix, radix_bytes, total;
+    int key_bits;
 };
+
+int bits_for(int64_t n) {
+    int b = 1;
+    while (b < 32 && (int64_t(1) << b) < n) ++b;
+    return b;
+}
 
 PresortLayout presort_layout(const Tree& t, int64_t nq, int mode) {
     PresortLayout l{};
-    l.nb = mode == kPresortMorton ? morton_bucket_count(t.dims) : (t.n_leaves >> bucket_shift(t));
-    l.nsums = (l.nb + kScanChunk - 1) / kScanChunk;
+    l.key_bits = mode == kPresortMorton ? bits_for(morton_bucket_count(t.dims)) : bits_for(t.n_leaves >> bucket_shift(t));
     size_t off = 0;
-    l.bucket = off; off += align256(sizeof(int32_t) * (size_t)nq);
-    l.rank = off; off += align256(sizeof(int32_t) * (size_t)nq);
+    l.key = off; off += align256(sizeof(int32_t) * (size_t)nq);
+    l.row = off; off += align256(sizeof(int32_t) * (size_t)nq);
+    l.key_s = off; off += align256(sizeof(int32_t) * (size_t)nq);
     l.order = off; off += align256(sizeof(int32_t) * (size_t)nq);
-    l.counts = off; off += align256(sizeof(int32_t) * (size_t)l.nb);
-    l.offsets = off; off += align256(sizeof(int32_t) * (size_t)l.nb);
-    l.sums = off; off += align256(sizeof(int32_t) * (size_t)l.nsums);
     l.bbox = off; off += align256(2 * 8 * (size_t)t.dims);
+    l.radix = off;
+    l.radix_bytes = radix_sort_workspace(nq);
+    off += align256(l.radix_bytes);
     l.total = off;
     return l;
 }
@@ -359,6 +364,34 @@ __global__ void fill_u64_kernel(unsigned long long* __restrict__ p, int n, unsig
 }
 
 }  // namespace
+
+void init_bbox(void* bbox, int dtype, int dims, hipStream_t stream) {
+    // [min codes (all ones) | max codes (zero)] in the width of T
+    if (dtype == kF32) {
+        hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, stream, static_cast<int32_t*>(bbox), (int64_t)dims,
+                           (int32_t)-1);
+        hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, stream, static_cast<int32_t*>(bbox) + dims,
+                           (int64_t)dims, 0);
+    } else {
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, stream, static_cast<unsigned long long*>(bbox),
+                           2 * dims, ~0ull, 0ull, dims);
+    }
+    TK_HIP_CHECK(hipGetLastError());
+}
+
+int canonical_morton_bits(int64_t nq, int dims) {
+    int maxb = 0;
+    switch (dims) {
+        case 1: maxb = MortonBits<1>::kBits; break;
+        case 2: maxb = MortonBits<2>::kBits; break;
+        case 3: maxb = MortonBits<3>::kBits; break;
+        case 8: maxb = MortonBits<8>::kBits; break;
+        default: throw std::runtime_error("unsupported dims");
+    }
+    int b = 0;
+    while (b < maxb && (int64_t(8) << ((b + 1) * dims)) <= nq) ++b;
+    return b;
+}
 
 size_t query_gpu_workspace(const Tree& t, int64_t nq, const QueryOpts& o) {
     const int mode = presort_mode(t, nq, o);
@@ -391,42 +424,24 @@ void query_gpu(const Tree& t, const void* queries, int64_t nq, int k, void* dist
         } else if (workspace_bytes < l.total) {
             throw std::runtime_error("query workspace too small");
         }
-        int32_t* bucket = reinterpret_cast<int32_t*>(ws + l.bucket);
-        int32_t* rank = reinterpret_cast<int32_t*>(ws + l.rank);
+        uint32_t* key = reinterpret_cast<uint32_t*>(ws + l.key);
+        uint32_t* row = reinterpret_cast<uint32_t*>(ws + l.row);
+        uint32_t* key_s = reinterpret_cast<uint32_t*>(ws + l.key_s);
         int32_t* ord = reinterpret_cast<int32_t*>(ws + l.order);
-        int32_t* counts = reinterpret_cast<int32_t*>(ws + l.counts);
-        int32_t* offsets = reinterpret_cast<int32_t*>(ws + l.offsets);
-        int32_t* sums = reinterpret_cast<int32_t*>(ws + l.sums);
         void* bbox = ws + l.bbox;
-        hipLaunchKernelGGL(fill_i32_kernel, dim3((l.nb + 255) / 256), dim3(256), 0, stream, counts, (int64_t)l.nb, 0);
-        check_launch();
         if (mode == kPresortMorton) {
-            // bbox = [min codes (all ones) | max codes (zero)] in the width of T
-            if (t.dtype == kF32) {
-                hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, stream, static_cast<int32_t*>(bbox),
-                                   (int64_t)t.dims, (int32_t)-1);
-                hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, stream, static_cast<int32_t*>(bbox) + t.dims,
-                                   (int64_t)t.dims, 0);
-                check_launch();
-                launch_morton_bucket_f32(t, static_cast<const float*>(queries), nq, bbox, bucket, rank, counts, stream);
-            } else {
-                hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, stream,
-                                   static_cast<unsigned long long*>(bbox), 2 * t.dims, ~0ull, 0ull, t.dims);
-                check_launch();
-                launch_morton_bucket_f64(t, static_cast<const double*>(queries), nq, bbox, bucket, rank, counts,
-                                         stream);
-            }
+            init_bbox(bbox, t.dtype, t.dims, stream);
+            if (t.dtype == kF32)
+                launch_morton_bucket_f32(t, static_cast<const float*>(queries), nq, bbox, key, row, stream);
+            else
+                launch_morton_bucket_f64(t, static_cast<const double*>(queries), nq, bbox, key, row, stream);
         } else if (t.dtype == kF32) {
-            launch_leaf_bucket_f32(t, static_cast<const float*>(queries), nq, bucket_shift(t), bucket, rank, counts,
-                                   stream);
+            launch_leaf_bucket_f32(t, static_cast<const float*>(queries), nq, bucket_shift(t), key, row, stream);
         } else {
-            launch_leaf_bucket_f64(t, static_cast<const double*>(queries), nq, bucket_shift(t), bucket, rank, counts,
-                                   stream);
+            launch_leaf_bucket_f64(t, static_cast<const double*>(queries), nq, bucket_shift(t), key, row, stream);
         }
-        exclusive_scan_i32(counts, l.nb, sums, offsets, stream);
-        hipLaunchKernelGGL(bucket_scatter_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, stream, nq, bucket,
-                           rank, offsets, ord);
-        check_launch();
+        radix_sort_pairs(key, row, key_s, reinterpret_cast<uint32_t*>(ord), nq, l.key_bits, ws + l.radix,
+                         l.radix_bytes, stream);
         order = ord;
     }
     ProfScope pt("traverse", stream);

@@ -53,15 +53,20 @@ void launch_tree_search_f32(const Tree& t, const float* q, int64_t nq, const int
                             int32_t* idx, int64_t* orig, int tie, hipStream_t s);
 void launch_tree_search_f64(const Tree& t, const double* q, int64_t nq, const int32_t* order, int k, double* dist,
                             int32_t* idx, int64_t* orig, int tie, hipStream_t s);
-void launch_leaf_bucket_f32(const Tree& t, const float* q, int64_t nq, int shift, int32_t* bucket, int32_t* rank,
-                            int32_t* counts, hipStream_t s);
-void launch_leaf_bucket_f64(const Tree& t, const double* q, int64_t nq, int shift, int32_t* bucket, int32_t* rank,
-                            int32_t* counts, hipStream_t s);
-void launch_morton_bucket_f32(const Tree& t, const float* q, int64_t nq, void* bbox, int32_t* bucket, int32_t* rank,
-                              int32_t* counts, hipStream_t s);
-void launch_morton_bucket_f64(const Tree& t, const double* q, int64_t nq, void* bbox, int32_t* bucket,
-                              int32_t* rank, int32_t* counts, hipStream_t s);
+void launch_leaf_bucket_f32(const Tree& t, const float* q, int64_t nq, int shift, uint32_t* key, uint32_t* row,
+                            hipStream_t s);
+void launch_leaf_bucket_f64(const Tree& t, const double* q, int64_t nq, int shift, uint32_t* key, uint32_t* row,
+                            hipStream_t s);
+void launch_morton_bucket_f32(const Tree& t, const float* q, int64_t nq, void* bbox, uint32_t* key, uint32_t* row,
+                              hipStream_t s);
+void launch_morton_bucket_f64(const Tree& t, const double* q, int64_t nq, void* bbox, uint32_t* key, uint32_t* row,
+                              hipStream_t s);
 int morton_bucket_count(int dims);
+// Sets a 2*dims ordered-integer bounding box to (all ones | zero), the start of a min/max reduction.
+void init_bbox(void* bbox, int dtype, int dims, hipStream_t stream);
+// Bits per axis of the Morton grid of the backward's canonical query order: about 8 queries per
+// cell, at most the grid of the forward presort.
+int canonical_morton_bits(int64_t nq, int dims);
 
 // Brute force (bruteforce.hip).
 void bruteforce_gpu(const void* ref, int32_t n, const void* queries, int64_t nq, int dims, int dtype, int k,
@@ -72,7 +77,7 @@ size_t bruteforce_gpu_workspace(int32_t n, int64_t nq, int dims, int dtype, int 
 void knn_backward_gpu(const void* queries, const void* ref, int64_t nq, int32_t n, int dims, int dtype, int k,
                       const int64_t* idx, const void* grad_dist, void* grad_q, void* grad_ref, hipStream_t stream,
                       void* workspace, size_t workspace_bytes);
-size_t knn_backward_workspace(int64_t nq, int32_t n, int k);
+size_t knn_backward_workspace(int64_t nq, int32_t n, int k, int dims, int dtype);
 
 // Exclusive prefix sum of n int32 values (n <= 2^24) into out; sums needs scan_sums_count(n)
 // int32 of scratch.  Kernels only (no memset, no host synchronisation).

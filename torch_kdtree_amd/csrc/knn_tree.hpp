@@ -559,19 +559,20 @@ inline size_t lds_list_bytes(int k, size_t tsize, bool lower) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Leaf-bucket presort: each query descends to the leaf on its near side (no pruning) and is
-// counted into that leaf's bucket; an exclusive scan of the counts plus the per-query rank
-// gives a permutation that groups queries by leaf.  Leaves are numbered left to right, so
-// neighbouring buckets are neighbouring cells of space.
+// Query presorts.  Each presort kernel writes one integer key per query and the query's row; a
+// stable radix sort of (key, row) (radix_sort.hip) then gives the order, so the order is
+// deterministic: equal keys keep increasing rows.
+//
+// Leaf keys: each query descends to the leaf on its near side (no pruning); leaves are numbered
+// left to right, so neighbouring keys are neighbouring cells of space.
 // ---------------------------------------------------------------------------------------------
 constexpr int kPresortBlock = 256;
 
 template <typename T, int D>
 __global__ __launch_bounds__(kPresortBlock) void leaf_bucket_kernel(TreeView<T> tv, const T* __restrict__ queries,
                                                                      int64_t nq, int shift,
-                                                                     int32_t* __restrict__ bucket,
-                                                                     int32_t* __restrict__ rank,
-                                                                     int32_t* __restrict__ counts) {
+                                                                     uint32_t* __restrict__ key,
+                                                                     uint32_t* __restrict__ row) {
     const int64_t i = (int64_t)blockIdx.x * kPresortBlock + threadIdx.x;
     if (i >= nq) return;
     Pt<T, D> q;
@@ -582,22 +583,38 @@ __global__ __launch_bounds__(kPresortBlock) void leaf_bucket_kernel(TreeView<T> 
         node = 2 * node + (ax_get(q, a) < tv.split_at(node) ? 1 : 2);
         a = ax_next<D>(a);
     }
-    const int32_t b = (node - tv.n_nodes) >> shift;
-    bucket[i] = b;
-    rank[i] = atomicAdd(&counts[b], 1);
+    key[i] = (uint32_t)((node - tv.n_nodes) >> shift);
+    row[i] = (uint32_t)i;
 }
 
 // ---------------------------------------------------------------------------------------------
-// Morton presort: queries are bucketed by the Morton code of their cell in a 2^B-per-axis grid
-// over the queries' own bounding box (B*D <= 21), so queries outside the tree's cloud are
-// grouped by where they really are (two queries that fall into the same boundary leaf can need
-// very different amounts of work), and a wavefront's lanes end up with similar work.
+// Morton keys: the Morton code of a query's cell in a 2^B-per-axis grid over a box (B*D <= 21).
+// The forward presort uses the queries' own bounding box and the finest grid, so queries
+// outside the tree's cloud are grouped by where they really are.  The backward's canonical
+// order uses the reference points' bounding box and B from canonical_morton_bits(Q, D)
+// (knn_gpu.hpp), the same rule in the single-tree and the batched backward.
 // ---------------------------------------------------------------------------------------------
 template <int D>
 struct MortonBits {
     static constexpr int kBits = D == 1 ? 20 : D == 2 ? 10 : D == 3 ? 7 : 2;
     static constexpr int kBuckets = 1 << (kBits * D);
 };
+
+// Morton code of q's cell; lo/hi are the box, converted to float.  Queries outside the box fall
+// into the boundary cells.
+template <typename T, int D>
+__device__ __forceinline__ uint32_t morton_cell(const float* lo, const float* hi, int bits, const T* __restrict__ q) {
+    const float cells = (float)(1 << bits);
+    uint32_t code = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        const float scale = hi[a] > lo[a] ? cells / (hi[a] - lo[a]) : 0.0f;
+        const float x = ((float)q[a] - lo[a]) * scale;
+        const uint32_t g = (uint32_t)fminf(fmaxf(x, 0.0f), cells - 1.0f);
+        for (int b = 0; b < bits; ++b) code |= ((g >> b) & 1u) << (b * D + a);
+    }
+    return code;
+}
 
 // Order-preserving map of floating-point values onto unsigned integers (for atomicMin/Max).
 __device__ __forceinline__ uint32_t ordered_bits(float f) {
@@ -620,9 +637,10 @@ using OrderedT = typename std::conditional<sizeof(T) == 4, uint32_t, unsigned lo
 
 constexpr int kBoxBlock = 256;
 
-// box[0..D) = min, box[D..2D) = max, as ordered integers; initialised to (all ones, zero).
+// Bounding box of n points: box[0..D) = min, box[D..2D) = max, as ordered integers; the caller
+// initialises it to (all ones, zero).
 template <typename T, int D>
-__global__ __launch_bounds__(kBoxBlock) void bbox_kernel(const T* __restrict__ queries, int64_t nq,
+__global__ __launch_bounds__(kBoxBlock) void bbox_kernel(const T* __restrict__ pts, int64_t n,
                                                          OrderedT<T>* __restrict__ box) {
     T lo[D], hi[D];
 #pragma unroll
@@ -630,10 +648,10 @@ __global__ __launch_bounds__(kBoxBlock) void bbox_kernel(const T* __restrict__ q
         lo[a] = pos_inf<T>();
         hi[a] = -pos_inf<T>();
     }
-    for (int64_t i = (int64_t)blockIdx.x * kBoxBlock + threadIdx.x; i < nq; i += (int64_t)gridDim.x * kBoxBlock) {
+    for (int64_t i = (int64_t)blockIdx.x * kBoxBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBoxBlock) {
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            const T v = queries[i * D + a];
+            const T v = pts[i * D + a];
             lo[a] = fmin(lo[a], v);
             hi[a] = fmax(hi[a], v);
         }
@@ -655,41 +673,22 @@ __global__ __launch_bounds__(kBoxBlock) void bbox_kernel(const T* __restrict__ q
     }
 }
 
+// Morton key over a box held as ordered integers (from bbox_kernel).
 template <typename T, int D>
 __global__ __launch_bounds__(kPresortBlock) void morton_bucket_kernel(const T* __restrict__ queries, int64_t nq,
-                                                                       const OrderedT<T>* __restrict__ box,
-                                                                       int32_t* __restrict__ bucket,
-                                                                       int32_t* __restrict__ rank,
-                                                                       int32_t* __restrict__ counts) {
+                                                                       const OrderedT<T>* __restrict__ box, int bits,
+                                                                       uint32_t* __restrict__ key,
+                                                                       uint32_t* __restrict__ row) {
     const int64_t i = (int64_t)blockIdx.x * kPresortBlock + threadIdx.x;
     if (i >= nq) return;
-    constexpr int B = MortonBits<D>::kBits;
-    constexpr float kCells = (float)(1 << B);
-    uint32_t code = 0;
+    float lo[D], hi[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-        const float lo = (float)from_ordered(box[a]);
-        const float hi = (float)from_ordered(box[D + a]);
-        const float scale = hi > lo ? kCells / (hi - lo) : 0.0f;
-        const float x = ((float)queries[i * D + a] - lo) * scale;
-        const uint32_t g = (uint32_t)fminf(fmaxf(x, 0.0f), kCells - 1.0f);
-#pragma unroll
-        for (int b = 0; b < B; ++b) code |= ((g >> b) & 1u) << (b * D + a);
+        lo[a] = (float)from_ordered(box[a]);
+        hi[a] = (float)from_ordered(box[D + a]);
     }
-    bucket[i] = (int32_t)code;
-    rank[i] = atomicAdd(&counts[code], 1);
+    key[i] = morton_cell<T, D>(lo, hi, bits, queries + i * D);
+    row[i] = (uint32_t)i;
 }
-
-__global__ void bucket_scatter_kernel(int64_t nq, const int32_t* __restrict__ bucket, const int32_t* __restrict__ rank,
-                                      const int32_t* __restrict__ offsets, int32_t* __restrict__ order);
-
-// Exclusive scan of n int32 counts (n <= kScanChunk * kScanChunk).
-constexpr int kScanBlock = 1024;
-constexpr int kScanItems = 4;
-constexpr int kScanChunk = kScanBlock * kScanItems;
-__global__ void scan_block_sums_kernel(const int32_t* __restrict__ in, int32_t n, int32_t* __restrict__ sums);
-__global__ void scan_sums_kernel(int32_t* __restrict__ sums, int32_t nsums);
-__global__ void scan_apply_kernel(const int32_t* __restrict__ in, int32_t n, const int32_t* __restrict__ sums,
-                                  int32_t* __restrict__ out);
 
 }  // namespace tk

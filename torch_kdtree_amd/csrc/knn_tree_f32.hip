@@ -10,14 +10,14 @@ void launch_tree_search_f32(const Tree& t, const float* q, int64_t nq, const int
     launch_tree_search_t<float>(t, q, nq, order, k, dist, idx, orig, tie, s);
 }
 
-void launch_leaf_bucket_f32(const Tree& t, const float* q, int64_t nq, int shift, int32_t* bucket, int32_t* rank,
-                            int32_t* counts, hipStream_t s) {
-    launch_leaf_bucket_t<float>(t, q, nq, shift, bucket, rank, counts, s);
+void launch_leaf_bucket_f32(const Tree& t, const float* q, int64_t nq, int shift, uint32_t* key, uint32_t* row,
+                            hipStream_t s) {
+    launch_leaf_bucket_t<float>(t, q, nq, shift, key, row, s);
 }
 
-void launch_morton_bucket_f32(const Tree& t, const float* q, int64_t nq, void* bbox, int32_t* bucket, int32_t* rank,
-                              int32_t* counts, hipStream_t s) {
-    launch_morton_bucket_t<float>(t, q, nq, bbox, bucket, rank, counts, s);
+void launch_morton_bucket_f32(const Tree& t, const float* q, int64_t nq, void* bbox, uint32_t* key, uint32_t* row,
+                              hipStream_t s) {
+    launch_morton_bucket_t<float>(t, q, nq, bbox, key, row, s);
 }
 
 int morton_bucket_count(int dims) { return morton_buckets(dims); }

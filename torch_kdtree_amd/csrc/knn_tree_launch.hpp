@@ -66,18 +66,18 @@ void launch_tree_search_t(const Tree& t, const T* q, int64_t nq, const int32_t* 
 #undef TK_DIMS
 }
 
-// bbox: 2*dims ordered integers of sizeof(T) bytes each, initialised by the caller.
+// bbox: 2*dims ordered integers of sizeof(T) bytes each, initialised by the caller (init_bbox).
 template <typename T>
-void launch_morton_bucket_t(const Tree& t, const T* q, int64_t nq, void* bbox, int32_t* bucket, int32_t* rank,
-                            int32_t* counts, hipStream_t s) {
+void launch_morton_bucket_t(const Tree& t, const T* q, int64_t nq, void* bbox, uint32_t* key, uint32_t* row,
+                            hipStream_t s) {
     auto* box = static_cast<OrderedT<T>*>(bbox);
     const unsigned box_blocks = (unsigned)std::min<int64_t>(1024, (nq + kBoxBlock - 1) / kBoxBlock);
     const dim3 grid((unsigned)((nq + kPresortBlock - 1) / kPresortBlock));
 #define TK_DIMS(DD)                                                                                              \
     case DD:                                                                                                     \
         hipLaunchKernelGGL((bbox_kernel<T, DD>), dim3(box_blocks), dim3(kBoxBlock), 0, s, q, nq, box);          \
-        hipLaunchKernelGGL((morton_bucket_kernel<T, DD>), grid, dim3(kPresortBlock), 0, s, q, nq, box, bucket,   \
-                           rank, counts);                                                                        \
+        hipLaunchKernelGGL((morton_bucket_kernel<T, DD>), grid, dim3(kPresortBlock), 0, s, q, nq, box,           \
+                           MortonBits<DD>::kBits, key, row);                                                     \
         break;
     switch (t.dims) {
         TK_DIMS(1)
@@ -101,14 +101,14 @@ inline int morton_buckets(int dims) {
 }
 
 template <typename T>
-void launch_leaf_bucket_t(const Tree& t, const T* q, int64_t nq, int shift, int32_t* bucket, int32_t* rank,
-                          int32_t* counts, hipStream_t s) {
+void launch_leaf_bucket_t(const Tree& t, const T* q, int64_t nq, int shift, uint32_t* key, uint32_t* row,
+                          hipStream_t s) {
     const TreeView<T> tv = tree_view<T>(t);
     const dim3 grid((unsigned)((nq + kPresortBlock - 1) / kPresortBlock));
 #define TK_DIMS(DD)                                                                                              \
     case DD:                                                                                                     \
-        hipLaunchKernelGGL((leaf_bucket_kernel<T, DD>), grid, dim3(kPresortBlock), 0, s, tv, q, nq, shift, bucket, \
-                           rank, counts);                                                                        \
+        hipLaunchKernelGGL((leaf_bucket_kernel<T, DD>), grid, dim3(kPresortBlock), 0, s, tv, q, nq, shift, key,  \
+                           row);                                                                                 \
         break;
     switch (t.dims) {
         TK_DIMS(1)

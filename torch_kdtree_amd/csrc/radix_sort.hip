@@ -7,10 +7,12 @@
 //      the block's first pair with that digit (all smaller digits, then earlier blocks);
 //   3. radix_scatter_kernel: every block walks its tile in index order, 256 pairs at a time.
 //      Within a wavefront, lanes holding the same digit are found with one ballot per digit bit;
-//      a pair's position is its block offset, plus the pairs of its digit already placed from
-//      this tile, plus those in lower waves and lower lanes of the current 256.  The order of
-//      equal digits is therefore the input order: the sort is stable, and the result does not
-//      depend on scheduling.
+//      a pair's slot in the tile is its digit's start in the tile, plus the pairs of its digit
+//      already placed, plus those in lower waves and lower lanes of the current 256.  The tile,
+//      rearranged so in LDS, is then written out slot by slot: each digit's run goes to
+//      consecutive output positions from the block's offset for that digit.  The order of
+//      equal digits is the input order: the sort is stable, and the result does not depend on
+//      scheduling.
 // Nothing is cleared with a memset and nothing waits on the host, so the sort can be captured in
 // a HIP graph and replayed.
 #include <algorithm>
@@ -45,18 +47,37 @@ __global__ __launch_bounds__(kRadixBlock) void radix_hist_kernel(const uint32_t*
 __global__ __launch_bounds__(kRadixBlock) void radix_scatter_kernel(const uint32_t* __restrict__ keys_in,
                                                                      const uint32_t* __restrict__ vals_in, int64_t n,
                                                                      int shift, int32_t n_blocks,
+                                                                     const int32_t* __restrict__ hist,
                                                                      const int32_t* __restrict__ offsets,
                                                                      uint32_t* __restrict__ keys_out,
                                                                      uint32_t* __restrict__ vals_out) {
     constexpr int kWaves = kRadixBlock / 64;
-    __shared__ int32_t placed[kRadixBins];        // pairs of each digit placed from this tile so far
+    __shared__ uint32_t tile_k[kRadixTile];           // the tile in digit order (stable)
+    __shared__ uint32_t tile_v[kRadixTile];
+    __shared__ int32_t tile_start[kRadixBins + 1];    // first tile slot of each digit
+    __shared__ int32_t placed[kRadixBins];            // pairs of each digit placed so far
     __shared__ int32_t wave_cnt[kWaves][kRadixBins];  // pairs of each digit per wave in the current 256
-    __shared__ int32_t block_off[kRadixBins];
+    __shared__ int32_t scan_tmp[kWaves];
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int w = t >> 6;
+    // tile_start: exclusive scan of this tile's digit counts
+    {
+        const int c = hist[(int64_t)t * n_blocks + blockIdx.x];
+        int inc = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += y;
+        }
+        if (lane == 63) scan_tmp[w] = inc;
+        __syncthreads();
+        int base = 0;
+        for (int v = 0; v < w; ++v) base += scan_tmp[v];
+        tile_start[t] = base + inc - c;
+        if (t == kRadixBlock - 1) tile_start[kRadixBins] = base + inc;
+    }
     placed[t] = 0;
-    block_off[t] = offsets[(int64_t)t * n_blocks + blockIdx.x];
 #pragma unroll
     for (int v = 0; v < kWaves; ++v) wave_cnt[v][t] = 0;
     __syncthreads();
@@ -81,10 +102,10 @@ __global__ __launch_bounds__(kRadixBlock) void radix_scatter_kernel(const uint32
         if (valid && last_of_group) wave_cnt[w][dg] = rank + 1;
         __syncthreads();
         if (valid) {
-            int pos = block_off[dg] + placed[dg] + rank;
+            int pos = tile_start[dg] + placed[dg] + rank;
             for (int v = 0; v < w; ++v) pos += wave_cnt[v][dg];
-            keys_out[pos] = key;
-            vals_out[pos] = val;
+            tile_k[pos] = key;
+            tile_v[pos] = val;
         }
         __syncthreads();
         int add = 0;
@@ -95,6 +116,15 @@ __global__ __launch_bounds__(kRadixBlock) void radix_scatter_kernel(const uint32
         }
         placed[t] += add;
         __syncthreads();
+    }
+    // write out in tile order: each digit's slots go to consecutive output positions
+    const int n_tile = tile_start[kRadixBins];
+    for (int x = t; x < n_tile; x += kRadixBlock) {
+        const uint32_t key = tile_k[x];
+        const int dg = (int)((key >> shift) & (kRadixBins - 1));
+        const int64_t pos = (int64_t)offsets[(int64_t)dg * n_blocks + blockIdx.x] + (x - tile_start[dg]);
+        keys_out[pos] = key;
+        vals_out[pos] = tile_v[x];
     }
 }
 
@@ -165,7 +195,7 @@ void radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t
         TK_HIP_CHECK(hipGetLastError());
         exclusive_scan_i32(hist, l.n_hist, sums, offs, stream);
         hipLaunchKernelGGL(radix_scatter_kernel, dim3(l.n_blocks), dim3(kRadixBlock), 0, stream, ks, vs, n, 8 * p,
-                           l.n_blocks, offs, kd, vd);
+                           l.n_blocks, hist, offs, kd, vd);
         TK_HIP_CHECK(hipGetLastError());
         ks = kd;
         vs = vd;
