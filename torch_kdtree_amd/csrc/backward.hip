@@ -15,7 +15,7 @@
 // bits per axis.  In that order each query's coordinates and upstream gradients are copied to
 // consecutive slots and its (reference index, slot) pairs are written; a stable radix sort of the
 // pairs by reference index then makes each reference point's contributions a contiguous run in
-// canonical order, and one lane per run subtracts them in that order.  Spatially close queries
+// canonical order, and one lane per reference point subtracts its run in that order.  Spatially close queries
 // are close in the canonical order, so a run's reads are close together.  The result is
 // bit-identical from run to run, and equal to the batched backward's on the same cloud.
 #include <algorithm>
@@ -28,43 +28,69 @@
 namespace tk {
 namespace {
 
-// dL/dq, and for dL/dr the canonical copies and pairs.  Lane t takes query order[t] (row t
-// without an order); with q_sorted it writes the query to slot t, and per neighbour j the
-// upstream gradient, the pair (reference index, t * k + j) to slot t * k + j.
+constexpr int kBwUnroll = 4;  // loads issued ahead per lane
+
+// Inverse of the canonical order: rank[order[t]] = t.
+__global__ __launch_bounds__(256) void rank_kernel(const uint32_t* __restrict__ order, int64_t n,
+                                                   uint32_t* __restrict__ rank) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t < n) rank[order[t]] = (uint32_t)t;
+}
+
+// dL/dq, one lane per query row, terms added in neighbour order; with q_sorted (for dL/dr) it also
+// copies the query to its canonical position t = rank[i] and, per neighbour j, the upstream
+// gradient and the pair (reference index, slot) to the canonical slot t * k + j.
 template <typename T, int D>
 __global__ __launch_bounds__(256) void grad_q_kernel(const T* __restrict__ q, const T* __restrict__ ref, int64_t nq,
                                                      int k, const int64_t* __restrict__ idx,
-                                                     const T* __restrict__ g, const uint32_t* __restrict__ order,
+                                                     const T* __restrict__ g, const uint32_t* __restrict__ rank,
                                                      T* __restrict__ dq, T* __restrict__ q_sorted,
                                                      T* __restrict__ g_sorted, uint32_t* __restrict__ keys,
                                                      uint32_t* __restrict__ vals) {
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (t >= nq) return;
-    const int64_t i = order ? (int64_t)order[t] : t;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nq) return;
     T qi[D], acc[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         qi[a] = q[i * D + a];
         acc[a] = T(0);
     }
+    const int64_t t = q_sorted ? (int64_t)rank[i] : 0;
     if (q_sorted) {
 #pragma unroll
         for (int a = 0; a < D; ++a) q_sorted[t * D + a] = qi[a];
     }
-    for (int j = 0; j < k; ++j) {
-        const int64_t f = i * k + j;
-        const int64_t m = idx[f];
-        const T gij = g[f];
+    // loads four neighbours ahead; the terms are still added in neighbour order
+    for (int j0 = 0; j0 < k; j0 += kBwUnroll) {
+        int64_t m[kBwUnroll];
+        T gg[kBwUnroll], rr[kBwUnroll][D];
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            const T diff = qi[a] - ref[m * D + a];
-            acc[a] = acc[a] + gij * (diff + diff);
+        for (int u = 0; u < kBwUnroll; ++u) {
+            const bool ok = j0 + u < k;
+            m[u] = ok ? idx[i * k + j0 + u] : 0;
+            gg[u] = ok ? g[i * k + j0 + u] : T(0);
         }
-        if (q_sorted) {
-            const int64_t v = t * k + j;
-            g_sorted[v] = gij;
-            keys[v] = (uint32_t)m;
-            vals[v] = (uint32_t)v;
+#pragma unroll
+        for (int u = 0; u < kBwUnroll; ++u) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) rr[u][a] = j0 + u < k ? ref[m[u] * D + a] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kBwUnroll; ++u) {
+            const int j = j0 + u;
+            if (j < k) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    const T diff = qi[a] - rr[u][a];
+                    acc[a] = acc[a] + gg[u] * (diff + diff);
+                }
+                if (q_sorted) {
+                    const int64_t v = t * k + j;
+                    g_sorted[v] = gg[u];
+                    keys[v] = (uint32_t)m[u];
+                    vals[v] = (uint32_t)v;
+                }
+            }
         }
     }
     if (dq) {
@@ -79,35 +105,70 @@ __global__ __launch_bounds__(256) void fill_kernel(T* __restrict__ p, int64_t n)
     if (i < n) p[i] = T(0);
 }
 
-// dL/dr: one lane per run of equal reference indices in the sorted pairs (in canonical order).
-template <typename T, int D>
-__global__ __launch_bounds__(256) void grad_r_kernel(const T* __restrict__ ref, int k, int64_t m,
-                                                     const uint32_t* __restrict__ keys,
-                                                     const uint32_t* __restrict__ vals,
-                                                     const T* __restrict__ q_sorted, const T* __restrict__ g_sorted,
-                                                     T* __restrict__ dr) {
+__global__ __launch_bounds__(256) void zero_u32_kernel(uint32_t* __restrict__ a, uint32_t* __restrict__ b,
+                                                       int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        a[i] = 0;
+        b[i] = 0;
+    }
+}
+
+// Bounds of each key's run in the sorted pairs: [run_start[key], run_end[key]) (both zero for keys
+// that have no pairs).
+__global__ __launch_bounds__(256) void run_bounds_kernel(const uint32_t* __restrict__ keys, int64_t m,
+                                                         uint32_t* __restrict__ run_start,
+                                                         uint32_t* __restrict__ run_end) {
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (p >= m) return;
     const uint32_t key = keys[p];
-    if (p > 0 && keys[p - 1] == key) return;  // not the head of its run
+    if (p == 0 || keys[p - 1] != key) run_start[key] = (uint32_t)p;
+    if (p == m - 1 || keys[p + 1] != key) run_end[key] = (uint32_t)(p + 1);
+}
+
+// dL/dr, one lane per reference point: its run of sorted pairs, subtracted in canonical order.
+template <typename T, int D>
+__global__ __launch_bounds__(256) void grad_r_kernel(const T* __restrict__ ref, int32_t n, int k,
+                                                     const uint32_t* __restrict__ run_start,
+                                                     const uint32_t* __restrict__ run_end,
+                                                     const uint32_t* __restrict__ vals,
+                                                     const T* __restrict__ q_sorted, const T* __restrict__ g_sorted,
+                                                     T* __restrict__ dr) {
+    const int64_t mi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (mi >= n) return;
+    const uint32_t e0 = run_start[mi], e1 = run_end[mi];
     T r[D], acc[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-        r[a] = ref[(int64_t)key * D + a];
+        r[a] = ref[mi * D + a];
         acc[a] = T(0);
     }
-    for (int64_t e = p; e < m && keys[e] == key; ++e) {
-        const int64_t v = vals[e];
-        const int64_t t = v / k;
-        const T gij = g_sorted[v];
+    // loads four pairs ahead; the terms are still subtracted in run order
+    for (uint32_t e0u = e0; e0u < e1; e0u += kBwUnroll) {
+        int64_t v[kBwUnroll];
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            const T diff = q_sorted[t * D + a] - r[a];
-            acc[a] = acc[a] - gij * (diff + diff);
+        for (int u = 0; u < kBwUnroll; ++u) v[u] = e0u + u < e1 ? (int64_t)vals[e0u + u] : 0;
+        T gg[kBwUnroll], qq[kBwUnroll][D];
+#pragma unroll
+        for (int u = 0; u < kBwUnroll; ++u) {
+            const bool ok = e0u + u < e1;
+            gg[u] = ok ? g_sorted[v[u]] : T(0);
+#pragma unroll
+            for (int a = 0; a < D; ++a) qq[u][a] = ok ? q_sorted[(v[u] / k) * D + a] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kBwUnroll; ++u) {
+            if (e0u + u < e1) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    const T diff = qq[u][a] - r[a];
+                    acc[a] = acc[a] - gg[u] * (diff + diff);
+                }
+            }
         }
     }
 #pragma unroll
-    for (int a = 0; a < D; ++a) dr[(int64_t)key * D + a] = acc[a];
+    for (int a = 0; a < D; ++a) dr[mi * D + a] = acc[a];
 }
 
 inline size_t a256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -119,11 +180,11 @@ int key_bits(int64_t n) {
 }
 
 struct BwLayout {
-    size_t qkey, qrow, qkey_s, order, bbox, q_sorted, g_sorted, keys, vals, keys_s, vals_s, temp, temp_bytes, total;
+    size_t qkey, qrow, qkey_s, order, rank, run_start, run_end, bbox, q_sorted, g_sorted, keys, vals, keys_s, vals_s,
+        temp, temp_bytes, total;
 };
 
 BwLayout bw_layout(int64_t nq, int32_t n, int k, int dims, int dtype) {
-    (void)n;
     const int64_t m = nq * k;
     const size_t ts = dtype_size(dtype);
     BwLayout l{};
@@ -132,6 +193,9 @@ BwLayout bw_layout(int64_t nq, int32_t n, int k, int dims, int dtype) {
     l.qrow = off; off += a256(4 * (size_t)nq);
     l.qkey_s = off; off += a256(4 * (size_t)nq);
     l.order = off; off += a256(4 * (size_t)nq);
+    l.rank = off; off += a256(4 * (size_t)nq);
+    l.run_start = off; off += a256(4 * (size_t)n);
+    l.run_end = off; off += a256(4 * (size_t)n);
     l.bbox = off; off += a256(16 * (size_t)dims);
     l.q_sorted = off; off += a256(ts * (size_t)dims * (size_t)nq);
     l.g_sorted = off; off += a256(ts * (size_t)m);
@@ -151,24 +215,25 @@ void backward_typed(const T* q, const T* ref, int64_t nq, int32_t n, int k, cons
                     T* dr, hipStream_t s, unsigned char* ws, const BwLayout& l) {
     const int64_t m = nq * k;
     const bool pairs = dr && m > 0;
-    if (dr) {
+    const dim3 qgrid((unsigned)((nq + 255) / 256));
+    if (dr && !pairs) {
         hipLaunchKernelGGL((fill_kernel<T>), dim3((unsigned)(((int64_t)n * D + 255) / 256)), dim3(256), 0, s, dr,
                            (int64_t)n * D);
         TK_HIP_CHECK(hipGetLastError());
     }
     if (nq == 0 || !(dq || pairs)) return;
-    const dim3 qgrid((unsigned)((nq + 255) / 256));
     if (!pairs) {
         hipLaunchKernelGGL((grad_q_kernel<T, D>), qgrid, dim3(256), 0, s, q, ref, nq, k, idx, g, nullptr, dq, nullptr,
                            nullptr, nullptr, nullptr);
         TK_HIP_CHECK(hipGetLastError());
         return;
     }
-    // canonical query order
+    // canonical query order and its inverse
     uint32_t* qkey = reinterpret_cast<uint32_t*>(ws + l.qkey);
     uint32_t* qrow = reinterpret_cast<uint32_t*>(ws + l.qrow);
     uint32_t* qkey_s = reinterpret_cast<uint32_t*>(ws + l.qkey_s);
     uint32_t* order = reinterpret_cast<uint32_t*>(ws + l.order);
+    uint32_t* rank = reinterpret_cast<uint32_t*>(ws + l.rank);
     auto* box = reinterpret_cast<OrderedT<T>*>(ws + l.bbox);
     init_bbox(box, sizeof(T) == 4 ? kF32 : kF64, D, s);
     const unsigned box_blocks = (unsigned)std::min<int64_t>(1024, ((int64_t)n + kBoxBlock - 1) / kBoxBlock);
@@ -177,19 +242,27 @@ void backward_typed(const T* q, const T* ref, int64_t nq, int32_t n, int k, cons
     hipLaunchKernelGGL((morton_bucket_kernel<T, D>), qgrid, dim3(256), 0, s, q, nq, box, bits, qkey, qrow);
     TK_HIP_CHECK(hipGetLastError());
     radix_sort_pairs(qkey, qrow, qkey_s, order, nq, bits * D, ws + l.temp, l.temp_bytes, s);
-    // dL/dq, canonical copies and pairs; sort the pairs by reference index; dL/dr per run
+    hipLaunchKernelGGL(rank_kernel, qgrid, dim3(256), 0, s, order, nq, rank);
+    TK_HIP_CHECK(hipGetLastError());
+    // dL/dq, canonical copies and pairs; pairs sorted by reference index; dL/dr per reference point
     T* q_sorted = reinterpret_cast<T*>(ws + l.q_sorted);
     T* g_sorted = reinterpret_cast<T*>(ws + l.g_sorted);
     uint32_t* keys = reinterpret_cast<uint32_t*>(ws + l.keys);
     uint32_t* vals = reinterpret_cast<uint32_t*>(ws + l.vals);
     uint32_t* keys_s = reinterpret_cast<uint32_t*>(ws + l.keys_s);
     uint32_t* vals_s = reinterpret_cast<uint32_t*>(ws + l.vals_s);
-    hipLaunchKernelGGL((grad_q_kernel<T, D>), qgrid, dim3(256), 0, s, q, ref, nq, k, idx, g, order, dq, q_sorted,
+    uint32_t* run_start = reinterpret_cast<uint32_t*>(ws + l.run_start);
+    uint32_t* run_end = reinterpret_cast<uint32_t*>(ws + l.run_end);
+    hipLaunchKernelGGL((grad_q_kernel<T, D>), qgrid, dim3(256), 0, s, q, ref, nq, k, idx, g, rank, dq, q_sorted,
                        g_sorted, keys, vals);
     TK_HIP_CHECK(hipGetLastError());
     radix_sort_pairs(keys, vals, keys_s, vals_s, m, key_bits(n), ws + l.temp, l.temp_bytes, s);
-    hipLaunchKernelGGL((grad_r_kernel<T, D>), dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, ref, k, m, keys_s,
-                       vals_s, q_sorted, g_sorted, dr);
+    const dim3 rgrid((unsigned)(((int64_t)n + 255) / 256));
+    hipLaunchKernelGGL(zero_u32_kernel, rgrid, dim3(256), 0, s, run_start, run_end, (int64_t)n);
+    hipLaunchKernelGGL(run_bounds_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, keys_s, m, run_start,
+                       run_end);
+    hipLaunchKernelGGL((grad_r_kernel<T, D>), rgrid, dim3(256), 0, s, ref, n, k, run_start, run_end, vals_s, q_sorted,
+                       g_sorted, dr);
     TK_HIP_CHECK(hipGetLastError());
 }
 

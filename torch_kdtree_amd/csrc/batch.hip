@@ -92,20 +92,21 @@ void stage_cloud_t(const Tree& t, const CloudDesc& cd, unsigned char* recs, void
 
 int box_bytes_of(int dtype, int dims) { return box_bytes_for(dtype, dims); }
 
-void enqueue_forward(BatchPlan& p, const void* q, void* dist, int32_t* idx, int64_t* orig, hipStream_t s) {
+void enqueue_forward(BatchPlan& p, const void* q, void* dist, int32_t* idx, int64_t* orig, bool emit,
+                     hipStream_t s) {
     if (p.total_q == 0) return;
     if (p.dtype == kF32) {
         batch_presort_f32(p, static_cast<const float*>(q), s);
-        batch_traverse_f32(p, static_cast<const float*>(q), static_cast<float*>(dist), idx, orig, s);
+        batch_traverse_f32(p, static_cast<const float*>(q), static_cast<float*>(dist), idx, orig, emit, s);
     } else {
         batch_presort_f64(p, static_cast<const double*>(q), s);
-        batch_traverse_f64(p, static_cast<const double*>(q), static_cast<double*>(dist), idx, orig, s);
+        batch_traverse_f64(p, static_cast<const double*>(q), static_cast<double*>(dist), idx, orig, emit, s);
     }
 }
 
 void enqueue_backward(BatchPlan& p, const void* q, const int32_t* idx, const void* g, void* dq, void* dr,
-                      bool reuse_order, hipStream_t s) {
-    if (!reuse_order && p.total_q > 0) {
+                      bool from_forward, hipStream_t s) {
+    if (!from_forward && p.total_q > 0) {
         if (p.dtype == kF32)
             batch_presort_f32(p, static_cast<const float*>(q), s);
         else
@@ -113,10 +114,10 @@ void enqueue_backward(BatchPlan& p, const void* q, const int32_t* idx, const voi
     }
     if (p.dtype == kF32)
         batch_backward_f32(p, static_cast<const float*>(q), idx, static_cast<const float*>(g), static_cast<float*>(dq),
-                           static_cast<float*>(dr), s);
+                           static_cast<float*>(dr), from_forward, s);
     else
         batch_backward_f64(p, static_cast<const double*>(q), idx, static_cast<const double*>(g),
-                           static_cast<double*>(dq), static_cast<double*>(dr), s);
+                           static_cast<double*>(dq), static_cast<double*>(dr), from_forward, s);
 }
 
 }  // namespace
@@ -126,8 +127,9 @@ void batch_init_qcloud(const BatchPlan& p, hipStream_t s) {
     const dim3 grid((unsigned)((p.total_q + 255) / 256));
     hipLaunchKernelGGL(batch_qcloud_kernel, grid, dim3(256), 0, s, p.d_clouds, p.n_clouds, p.total_q, p.qcloud);
     TK_HIP_CHECK(hipGetLastError());
-    // a valid permutation before any forward has run (the backward walks queries in this order)
+    // a valid permutation and its inverse before any forward has run
     hipLaunchKernelGGL(batch_iota_kernel, grid, dim3(256), 0, s, p.order, p.total_q);
+    hipLaunchKernelGGL(batch_iota_kernel, grid, dim3(256), 0, s, reinterpret_cast<int32_t*>(p.rank), p.total_q);
     TK_HIP_CHECK(hipGetLastError());
 }
 
@@ -253,8 +255,10 @@ BatchPlan* batch_create(const Tree* const* trees, int n_clouds, const int64_t* n
     dmalloc(&p->row, 4 * nq);
     dmalloc(&p->key_s, 4 * nq);
     dmalloc(&p->order, 4 * nq);
+    dmalloc(&p->rank, 4 * nq);
+    dmalloc(&p->run_start, 4 * (size_t)p->total_ref);
+    dmalloc(&p->run_end, 4 * (size_t)p->total_ref);
     dmalloc(reinterpret_cast<unsigned char**>(&p->q_sorted), ts * (size_t)p->dims * nq);
-    dmalloc(reinterpret_cast<unsigned char**>(&p->g_sorted), ts * no);
     dmalloc(&p->keys, 4 * no);
     dmalloc(&p->vals, 4 * no);
     dmalloc(&p->keys_s, 4 * no);
@@ -286,8 +290,10 @@ void batch_free(BatchPlan* p) noexcept {
     dfree(p->row);
     dfree(p->key_s);
     dfree(p->order);
+    dfree(p->rank);
+    dfree(p->run_start);
+    dfree(p->run_end);
     dfree(p->q_sorted);
-    dfree(p->g_sorted);
     dfree(p->keys);
     dfree(p->vals);
     dfree(p->keys_s);
@@ -300,7 +306,7 @@ void batch_free(BatchPlan* p) noexcept {
 void batch_forward(BatchPlan& p, const void* queries, void* dist, int32_t* idx, int64_t* orig, hipStream_t s) {
     DeviceGuard guard(p.device);
     ProfScope prof("batch_forward", s);
-    enqueue_forward(p, queries, dist, idx, orig, s);
+    enqueue_forward(p, queries, dist, idx, orig, false, s);
 }
 
 void batch_backward(BatchPlan& p, const void* queries, const int32_t* idx, const void* grad_dist, void* grad_q,
@@ -332,9 +338,9 @@ void batch_capture(BatchPlan& p, const void* queries, void* dist, int32_t* idx, 
     TK_HIP_CHECK(hipStreamSynchronize(s));
     TK_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     try {
-        enqueue_forward(p, queries, dist, idx, orig, cs);
-        if (grad_dist && (grad_q || grad_ref))
-            enqueue_backward(p, queries, idx, grad_dist, grad_q, grad_ref, true, cs);
+        const bool train = grad_dist && (grad_q || grad_ref);
+        enqueue_forward(p, queries, dist, idx, orig, train, cs);
+        if (train) enqueue_backward(p, queries, idx, grad_dist, grad_q, grad_ref, true, cs);
     } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(cs, &g);

@@ -74,8 +74,10 @@ struct BatchPlan {
     uint32_t* row = nullptr;    // [total_q] query rows
     uint32_t* key_s = nullptr;  // [total_q] sorted keys
     int32_t* order = nullptr;   // [total_q] query rows in sorted order
+    uint32_t* rank = nullptr;   // [total_q] position of each query row in the sorted order
+    uint32_t* run_start = nullptr;  // [total_ref] runs of each record in the sorted pairs
+    uint32_t* run_end = nullptr;
     void* q_sorted = nullptr;   // [total_q * dims] queries in sorted order
-    void* g_sorted = nullptr;   // [total_out] upstream gradients in canonical slots
     uint32_t* keys = nullptr;   // [total_out] pairs (record, canonical slot)
     uint32_t* vals = nullptr;
     uint32_t* keys_s = nullptr;
@@ -106,12 +108,15 @@ void batch_launch(BatchPlan& p, hipStream_t s);
 void batch_init_qcloud(const BatchPlan& p, hipStream_t s);
 void batch_presort_f32(const BatchPlan& p, const float* q, hipStream_t s);
 void batch_presort_f64(const BatchPlan& p, const double* q, hipStream_t s);
-void batch_traverse_f32(const BatchPlan& p, const float* q, float* dist, int32_t* idx, int64_t* orig, hipStream_t s);
-void batch_traverse_f64(const BatchPlan& p, const double* q, double* dist, int32_t* idx, int64_t* orig,
+// emit: also write the training outputs (the queries at their sorted positions and the
+// (record, canonical slot) pairs) for a following batch_backward_*(from_forward = true).
+void batch_traverse_f32(const BatchPlan& p, const float* q, float* dist, int32_t* idx, int64_t* orig, bool emit,
+                        hipStream_t s);
+void batch_traverse_f64(const BatchPlan& p, const double* q, double* dist, int32_t* idx, int64_t* orig, bool emit,
                         hipStream_t s);
 void batch_backward_f32(const BatchPlan& p, const float* q, const int32_t* idx, const float* g, float* dq, float* dr,
-                        hipStream_t s);
+                        bool from_forward, hipStream_t s);
 void batch_backward_f64(const BatchPlan& p, const double* q, const int32_t* idx, const double* g, double* dq,
-                        double* dr, hipStream_t s);
+                        double* dr, bool from_forward, hipStream_t s);
 
 }  // namespace tk

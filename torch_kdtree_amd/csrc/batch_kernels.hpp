@@ -12,6 +12,7 @@ namespace tk {
 namespace {
 
 constexpr int kBatchBlock = 256;
+constexpr int kBwUnroll = 4;  // backward loads issued ahead per lane
 
 template <typename T>
 __global__ __launch_bounds__(kBatchBlock) void batch_zero_kernel(T* __restrict__ p, int64_t n) {
@@ -51,7 +52,10 @@ __global__ __launch_bounds__(kTreeBlock) void knn_batch_kernel(TreeView<T, true>
                                                                 const T* __restrict__ queries,
                                                                 const int32_t* __restrict__ order, int64_t begin,
                                                                 int64_t count, T* __restrict__ out_d,
-                                                                int32_t* __restrict__ out_i, int64_t* __restrict__ out_o) {
+                                                                int32_t* __restrict__ out_i, int64_t* __restrict__ out_o,
+                                                                uint32_t* __restrict__ pair_keys,
+                                                                uint32_t* __restrict__ pair_vals,
+                                                                T* __restrict__ q_sorted) {
     constexpr int CH = StageBytes<T, K, kLower>::kChunk;
     constexpr int LD = StageBytes<T, K, kLower>::kLoads;
     static_assert(LD <= kRecPad, "a load group may read at most kRecPad records past a leaf's end");
@@ -82,40 +86,57 @@ __global__ __launch_bounds__(kTreeBlock) void knn_batch_kernel(TreeView<T, true>
     // keys: reference policy = arena record position, lower-index policy = the cloud's original index
     const int pad = K - k;
     const int64_t ob = cd.out_base + (qi - cd.q_base) * k;
+    // training outputs: this query at its sorted position, its pairs at their canonical slots
+    const int64_t tpos = begin + t;
+    const int64_t sb = cd.sorted_out_base + (tpos - cd.sorted_q_base) * k;
+    if (q_sorted) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) q_sorted[tpos * D + a] = q[a];
+    }
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         if (j >= pad) {
             const int32_t x = kb.x[j];
+            const int32_t loc = kLower ? load_i32_at(arena.inv_shuffled, cd.rec_base + x) : x - cd.rec_base;
             out_d[ob + j - pad] = kb.d[j];
-            out_i[ob + j - pad] = kLower ? load_i32_at(arena.inv_shuffled, cd.rec_base + x) : x - cd.rec_base;
+            out_i[ob + j - pad] = loc;
             if (out_o) out_o[ob + j - pad] = kLower ? x : load_i32_at(arena.shuffled, x);
+            if (pair_keys) {
+                pair_keys[sb + j - pad] = (uint32_t)(cd.rec_base + loc);
+                pair_vals[sb + j - pad] = (uint32_t)(sb + j - pad);
+            }
         }
     }
 }
 
-// dL/dq, and for dL/dr the canonical copies and pairs.  Lane t takes query order[t] (the sorted
-// order of the forward, the canonical order within each cloud); with q_sorted it writes the query
-// to position t and, per neighbour j, the upstream gradient and the pair (arena record, slot) to
-// the canonical slot sorted_out_base + (t - sorted_q_base) * k + j.  dL/dq adds the terms in
-// neighbour order (the single-tree grad_q_kernel's arithmetic and order).
+__global__ __launch_bounds__(kBatchBlock) void batch_rank_kernel(const int32_t* __restrict__ order, int64_t n,
+                                                                  uint32_t* __restrict__ rank) {
+    const int64_t t = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
+    if (t < n) rank[order[t]] = (uint32_t)t;
+}
+
+// dL/dq, one lane per query row, terms added in neighbour order (the single-tree grad_q_kernel's
+// arithmetic and order).  With q_sorted (for dL/dr) it also copies the query to its sorted
+// position t = rank[i] and, per neighbour j, the pair (arena record, slot) to the canonical slot
+// sorted_out_base + (t - sorted_q_base) * k + j.
 template <typename T, int D>
 __global__ __launch_bounds__(kBatchBlock) void batch_grad_q_pairs_kernel(
     const unsigned char* __restrict__ recs, const CloudDesc* __restrict__ clouds, const int32_t* __restrict__ qcloud,
-    const int32_t* __restrict__ order, const T* __restrict__ queries, int64_t total_q, const int32_t* __restrict__ idx,
-    const T* __restrict__ g, T* __restrict__ dq, T* __restrict__ q_sorted, T* __restrict__ g_sorted,
-    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    const int64_t t = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
-    if (t >= total_q) return;
-    const int64_t i = order[t];
+    const uint32_t* __restrict__ rank, const T* __restrict__ queries, int64_t total_q, const int32_t* __restrict__ idx,
+    const T* __restrict__ g, T* __restrict__ dq, T* __restrict__ q_sorted, uint32_t* __restrict__ keys,
+    uint32_t* __restrict__ vals) {
+    const int64_t i = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
+    if (i >= total_q) return;
     const CloudDesc cd = clouds[qcloud[i]];
     const int64_t ob = cd.out_base + (i - cd.q_base) * cd.k;
-    const int64_t sb = cd.sorted_out_base + (t - cd.sorted_q_base) * cd.k;
     T qi[D], acc[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         qi[a] = queries[i * D + a];
         acc[a] = T(0);
     }
+    const int64_t t = q_sorted ? (int64_t)rank[i] : 0;
+    const int64_t sb = cd.sorted_out_base + (t - cd.sorted_q_base) * cd.k;
     if (q_sorted) {
 #pragma unroll
         for (int a = 0; a < D; ++a) q_sorted[t * D + a] = qi[a];
@@ -130,7 +151,6 @@ __global__ __launch_bounds__(kBatchBlock) void batch_grad_q_pairs_kernel(
             acc[a] = acc[a] + gij * (diff + diff);
         }
         if (q_sorted) {
-            g_sorted[sb + j] = gij;
             keys[sb + j] = (uint32_t)s;
             vals[sb + j] = (uint32_t)(sb + j);
         }
@@ -141,37 +161,127 @@ __global__ __launch_bounds__(kBatchBlock) void batch_grad_q_pairs_kernel(
     }
 }
 
-// dL/dr: one lane per run of equal records in the sorted pairs; the run is in canonical order.
+// dL/dq after a forward that wrote the training outputs: one lane per sorted position t (query
+// order[t], read from q_sorted), terms added in neighbour order.
+template <typename T, int D>
+__global__ __launch_bounds__(kBatchBlock) void batch_grad_q_sorted_kernel(
+    const unsigned char* __restrict__ recs, const CloudDesc* __restrict__ clouds, const int32_t* __restrict__ qcloud,
+    const int32_t* __restrict__ order, const T* __restrict__ q_sorted, int64_t total_q,
+    const int32_t* __restrict__ idx, const T* __restrict__ g, T* __restrict__ dq) {
+    const int64_t t = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
+    if (t >= total_q) return;
+    const int64_t i = order[t];
+    const CloudDesc cd = clouds[qcloud[i]];
+    const int64_t ob = cd.out_base + (i - cd.q_base) * cd.k;
+    T qi[D], acc[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        qi[a] = q_sorted[t * D + a];
+        acc[a] = T(0);
+    }
+    // loads four neighbours ahead; the terms are still added in neighbour order
+    for (int j0 = 0; j0 < cd.k; j0 += kBwUnroll) {
+        int32_t id[kBwUnroll];
+        T gg[kBwUnroll];
+#pragma unroll
+        for (int u = 0; u < kBwUnroll; ++u) {
+            const bool ok = j0 + u < cd.k;
+            id[u] = ok ? idx[ob + j0 + u] : 0;
+            gg[u] = ok ? g[ob + j0 + u] : T(0);
+        }
+        Rec<T, D> r[kBwUnroll];
+#pragma unroll
+        for (int u = 0; u < kBwUnroll; ++u) r[u] = load_rec<T, D>(recs, cd.rec_base + id[u]);
+#pragma unroll
+        for (int u = 0; u < kBwUnroll; ++u) {
+            if (j0 + u < cd.k) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    const T diff = qi[a] - r[u].c[a];
+                    acc[a] = acc[a] + gg[u] * (diff + diff);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) dq[i * D + a] = acc[a];
+}
+
+__global__ __launch_bounds__(kBatchBlock) void batch_zero_runs_kernel(uint32_t* __restrict__ a,
+                                                                       uint32_t* __restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
+    if (i < n) {
+        a[i] = 0;
+        b[i] = 0;
+    }
+}
+
+// Bounds of each record's run in the sorted pairs (zero for records without pairs).
+__global__ __launch_bounds__(kBatchBlock) void batch_run_bounds_kernel(const uint32_t* __restrict__ keys, int64_t m,
+                                                                        uint32_t* __restrict__ run_start,
+                                                                        uint32_t* __restrict__ run_end) {
+    const int64_t p = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
+    if (p >= m) return;
+    const uint32_t key = keys[p];
+    if (p == 0 || keys[p - 1] != key) run_start[key] = (uint32_t)p;
+    if (p == m - 1 || keys[p + 1] != key) run_end[key] = (uint32_t)(p + 1);
+}
+
+// dL/dr, one lane per record: its run of sorted pairs (canonical order), written to the record's
+// original row.  A slot's upstream gradient is read from the caller's layout through the order.
 template <typename T, int D>
 __global__ __launch_bounds__(kBatchBlock) void batch_grad_r_kernel(const unsigned char* __restrict__ recs,
                                                                     const int32_t* __restrict__ rec_cloud,
                                                                     const int32_t* __restrict__ shuffled,
-                                                                    const CloudDesc* __restrict__ clouds, int64_t m,
-                                                                    const uint32_t* __restrict__ keys,
+                                                                    const CloudDesc* __restrict__ clouds,
+                                                                    int64_t total_ref,
+                                                                    const uint32_t* __restrict__ run_start,
+                                                                    const uint32_t* __restrict__ run_end,
                                                                     const uint32_t* __restrict__ vals,
                                                                     const T* __restrict__ q_sorted,
-                                                                    const T* __restrict__ g_sorted,
-                                                                    T* __restrict__ dr) {
-    const int64_t p = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
-    if (p >= m) return;
-    const uint32_t key = keys[p];
-    if (p > 0 && keys[p - 1] == key) return;  // not the head of its run
-    const CloudDesc cd = clouds[rec_cloud[key]];
-    const Rec<T, D> r = load_rec<T, D>(recs, (int32_t)key);
+                                                                    const int32_t* __restrict__ order,
+                                                                    const T* __restrict__ g, T* __restrict__ dr) {
+    const int64_t s = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
+    if (s >= total_ref) return;
+    const CloudDesc cd = clouds[rec_cloud[s]];
+    const Rec<T, D> r = load_rec<T, D>(recs, (int32_t)s);
+    const uint32_t e0 = run_start[s], e1 = run_end[s];
     T acc[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) acc[a] = T(0);
-    for (int64_t e = p; e < m && keys[e] == key; ++e) {
-        const int64_t v = vals[e];
-        const int64_t t = cd.sorted_q_base + (v - cd.sorted_out_base) / cd.k;
-        const T gij = g_sorted[v];
+    // loads four pairs ahead; the terms are still subtracted in run order
+    for (uint32_t e0u = e0; e0u < e1; e0u += kBwUnroll) {
+        int64_t t[kBwUnroll], jj[kBwUnroll];
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            const T diff = q_sorted[t * D + a] - r.c[a];
-            acc[a] = acc[a] - gij * (diff + diff);
+        for (int u = 0; u < kBwUnroll; ++u) {
+            const bool ok = e0u + u < e1;
+            const int64_t off = ok ? (int64_t)vals[e0u + u] - cd.sorted_out_base : 0;
+            t[u] = cd.sorted_q_base + off / cd.k;
+            jj[u] = off % cd.k;
+        }
+        int64_t i[kBwUnroll];
+#pragma unroll
+        for (int u = 0; u < kBwUnroll; ++u) i[u] = e0u + u < e1 ? (int64_t)order[t[u]] : cd.q_base;
+        T gg[kBwUnroll], qq[kBwUnroll][D];
+#pragma unroll
+        for (int u = 0; u < kBwUnroll; ++u) {
+            const bool ok = e0u + u < e1;
+            gg[u] = ok ? g[cd.out_base + (i[u] - cd.q_base) * cd.k + jj[u]] : T(0);
+#pragma unroll
+            for (int a = 0; a < D; ++a) qq[u][a] = ok ? q_sorted[t[u] * D + a] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kBwUnroll; ++u) {
+            if (e0u + u < e1) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    const T diff = qq[u][a] - r.c[a];
+                    acc[a] = acc[a] - gg[u] * (diff + diff);
+                }
+            }
         }
     }
-    const int64_t row = (int64_t)cd.rec_base + shuffled[key];
+    const int64_t row = (int64_t)cd.rec_base + shuffled[s];
 #pragma unroll
     for (int a = 0; a < D; ++a) dr[row * D + a] = acc[a];
 }
@@ -202,10 +312,16 @@ void batch_presort_t(const BatchPlan& p, const T* q, hipStream_t s) {
     TK_HIP_CHECK(hipGetLastError());
     radix_sort_pairs(p.key, p.row, p.key_s, reinterpret_cast<uint32_t*>(p.order), p.total_q, p.key_bits, p.radix_ws,
                      p.radix_ws_bytes, s);
+    hipLaunchKernelGGL(batch_rank_kernel, batch_grid(p.total_q), dim3(kBatchBlock), 0, s, p.order, p.total_q, p.rank);
+    TK_HIP_CHECK(hipGetLastError());
 }
 
 template <typename T, int D, bool kLower>
-void batch_traverse_td(const BatchPlan& p, const T* q, T* dist, int32_t* idx, int64_t* orig, hipStream_t s) {
+void batch_traverse_td(const BatchPlan& p, const T* q, T* dist, int32_t* idx, int64_t* orig, bool emit,
+                       hipStream_t s) {
+    uint32_t* pk = emit ? p.keys : nullptr;
+    uint32_t* pv = emit ? p.vals : nullptr;
+    T* qs = emit ? static_cast<T*>(p.q_sorted) : nullptr;
     const TreeView<T, true> arena = batch_arena_view<T>(p);
     for (const BatchPlan::Range& r : p.ranges) {
         if (r.count == 0) continue;
@@ -213,7 +329,7 @@ void batch_traverse_td(const BatchPlan& p, const T* q, T* dist, int32_t* idx, in
 #define TK_LAUNCH_K(KK)                                                                                              \
     case KK:                                                                                                         \
         hipLaunchKernelGGL((knn_batch_kernel<T, D, KK, kLower>), grid, dim3(kTreeBlock), 0, s, arena, p.d_clouds,   \
-                           p.qcloud, q, p.order, r.begin, r.count, dist, idx, orig);                                 \
+                           p.qcloud, q, p.order, r.begin, r.count, dist, idx, orig, pk, pv, qs);                     \
         break;
         switch (r.K) {
             TK_LAUNCH_K(1)
@@ -230,13 +346,14 @@ void batch_traverse_td(const BatchPlan& p, const T* q, T* dist, int32_t* idx, in
 }
 
 template <typename T>
-void batch_traverse_t(const BatchPlan& p, const T* q, T* dist, int32_t* idx, int64_t* orig, hipStream_t s) {
+void batch_traverse_t(const BatchPlan& p, const T* q, T* dist, int32_t* idx, int64_t* orig, bool emit,
+                      hipStream_t s) {
 #define TK_DIMS(DD)                                                         \
     case DD:                                                                \
         if (p.tie == kTieLowerIndex)                                        \
-            batch_traverse_td<T, DD, true>(p, q, dist, idx, orig, s);       \
+            batch_traverse_td<T, DD, true>(p, q, dist, idx, orig, emit, s); \
         else                                                                \
-            batch_traverse_td<T, DD, false>(p, q, dist, idx, orig, s);      \
+            batch_traverse_td<T, DD, false>(p, q, dist, idx, orig, emit, s);\
         break;
     switch (p.dims) {
         TK_DIMS(1)
@@ -254,30 +371,41 @@ inline int batch_key_bits(int64_t n) {
     return b;
 }
 
+// from_forward: the forward wrote the training outputs (pairs, q_sorted) for these queries and
+// the order is current; otherwise the order is recomputed by the caller and the pairs are made
+// here.
 template <typename T>
-void batch_backward_t(const BatchPlan& p, const T* q, const int32_t* idx, const T* g, T* dq, T* dr, hipStream_t s) {
+void batch_backward_t(const BatchPlan& p, const T* q, const int32_t* idx, const T* g, T* dq, T* dr, bool from_forward,
+                      hipStream_t s) {
     const bool pairs = dr && p.total_out > 0;
-    T* qs = pairs ? static_cast<T*>(p.q_sorted) : nullptr;
-    T* gs = pairs ? static_cast<T*>(p.g_sorted) : nullptr;
+    T* qs = static_cast<T*>(p.q_sorted);
 #define TK_DIMS(DD)                                                                                                  \
     case DD:                                                                                                         \
-        if (dr) {                                                                                                    \
+        if (dr && !pairs) {                                                                                          \
             hipLaunchKernelGGL((batch_zero_kernel<T>), batch_grid(p.total_ref * DD), dim3(kBatchBlock), 0, s, dr,    \
                                p.total_ref * DD);                                                                    \
             TK_HIP_CHECK(hipGetLastError());                                                                         \
         }                                                                                                            \
-        if ((dq || pairs) && p.total_q > 0) {                                                                        \
+        if (p.total_q > 0 && from_forward && dq) {                                                                   \
+            hipLaunchKernelGGL((batch_grad_q_sorted_kernel<T, DD>), batch_grid(p.total_q), dim3(kBatchBlock), 0, s,  \
+                               p.recs, p.d_clouds, p.qcloud, p.order, qs, p.total_q, idx, g, dq);                    \
+            TK_HIP_CHECK(hipGetLastError());                                                                         \
+        } else if (p.total_q > 0 && !from_forward && (dq || pairs)) {                                                \
             hipLaunchKernelGGL((batch_grad_q_pairs_kernel<T, DD>), batch_grid(p.total_q), dim3(kBatchBlock), 0, s,   \
-                               p.recs, p.d_clouds, p.qcloud, p.order, q, p.total_q, idx, g, dq, qs, gs, p.keys,      \
-                               p.vals);                                                                              \
+                               p.recs, p.d_clouds, p.qcloud, p.rank, q, p.total_q, idx, g, dq,                       \
+                               pairs ? qs : nullptr, p.keys, p.vals);                                                \
             TK_HIP_CHECK(hipGetLastError());                                                                         \
         }                                                                                                            \
         if (pairs) {                                                                                                 \
             radix_sort_pairs(p.keys, p.vals, p.keys_s, p.vals_s, p.total_out, batch_key_bits(p.total_ref),           \
                              p.radix_ws, p.radix_ws_bytes, s);                                                       \
-            hipLaunchKernelGGL((batch_grad_r_kernel<T, DD>), batch_grid(p.total_out), dim3(kBatchBlock), 0, s,       \
-                               p.recs, p.rec_cloud, p.shuffled, p.d_clouds, p.total_out, p.keys_s, p.vals_s, qs, gs, \
-                               dr);                                                                                  \
+            hipLaunchKernelGGL(batch_zero_runs_kernel, batch_grid(p.total_ref), dim3(kBatchBlock), 0, s,             \
+                               p.run_start, p.run_end, p.total_ref);                                                 \
+            hipLaunchKernelGGL(batch_run_bounds_kernel, batch_grid(p.total_out), dim3(kBatchBlock), 0, s, p.keys_s,  \
+                               p.total_out, p.run_start, p.run_end);                                                 \
+            hipLaunchKernelGGL((batch_grad_r_kernel<T, DD>), batch_grid(p.total_ref), dim3(kBatchBlock), 0, s,       \
+                               p.recs, p.rec_cloud, p.shuffled, p.d_clouds, p.total_ref, p.run_start, p.run_end,     \
+                               p.vals_s, qs, p.order, g, dr);                                                        \
             TK_HIP_CHECK(hipGetLastError());                                                                         \
         }                                                                                                            \
         break;

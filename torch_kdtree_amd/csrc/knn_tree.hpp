@@ -638,10 +638,13 @@ using OrderedT = typename std::conditional<sizeof(T) == 4, uint32_t, unsigned lo
 constexpr int kBoxBlock = 256;
 
 // Bounding box of n points: box[0..D) = min, box[D..2D) = max, as ordered integers; the caller
-// initialises it to (all ones, zero).
+// initialises it to (all ones, zero).  Reduced per wavefront, then per block in LDS, then one
+// atomic per block and value.
 template <typename T, int D>
 __global__ __launch_bounds__(kBoxBlock) void bbox_kernel(const T* __restrict__ pts, int64_t n,
                                                          OrderedT<T>* __restrict__ box) {
+    constexpr int kWaves = kBoxBlock / 64;
+    __shared__ T s_lo[kWaves][D], s_hi[kWaves][D];
     T lo[D], hi[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
@@ -664,12 +667,24 @@ __global__ __launch_bounds__(kBoxBlock) void bbox_kernel(const T* __restrict__ p
             hi[a] = fmax(hi[a], __shfl_xor(hi[a], off, 64));
         }
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            atomicMin(&box[a], ordered_bits(lo[a]));
-            atomicMax(&box[D + a], ordered_bits(hi[a]));
+            s_lo[w][a] = lo[a];
+            s_hi[w][a] = hi[a];
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < D) {
+        const int a = threadIdx.x;
+        T l = s_lo[0][a], h = s_hi[0][a];
+        for (int v = 1; v < kWaves; ++v) {
+            l = fmin(l, s_lo[v][a]);
+            h = fmax(h, s_hi[v][a]);
+        }
+        atomicMin(&box[a], ordered_bits(l));
+        atomicMax(&box[D + a], ordered_bits(h));
     }
 }
 
