@@ -132,6 +132,52 @@ def _extra_default_levels(torch, tk, dev, ref, q, k, res_d, res_i, steps):
     return {"levels": tree.levels, "queries_per_s": q.shape[0] * steps / el, "ms_per_step": el * 1e3 / steps}
 
 
+def _extra_stress(np, torch, torch_knn, tk, workloads, dev, reps):
+    """Higher-dimension and fp64 stress: d=8 fwd + bwd at k=32, and the d=3 fp64 forward at k=16."""
+    out = {}
+    w = workloads.d8_mixture()
+    ref = torch.from_numpy(w["ref"]).to(dev).requires_grad_(True)
+    q = torch.from_numpy(w["queries"]).to(dev).requires_grad_(True)
+    wt = torch.rand((q.shape[0], w["k"]), device=dev, generator=torch.Generator(device=dev).manual_seed(6))
+    tree = tk.build_kd_tree(ref, device=dev, levels="auto")
+
+    def step():
+        d, _ = tree.query(q, nr_nns_searches=w["k"])
+        return torch.autograd.grad((wt * d).sum(), [q, ref])
+
+    step()
+    torch.cuda.synchronize()
+    torch_knn.profile_reset()
+    torch_knn.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    torch_knn.profile_enable(False)
+    tr, n = torch_knn.profile_read("traverse")
+    bw, m = torch_knn.profile_read("backward")
+    torch_knn.profile_reset()
+    out["d8_fwd_bwd"] = {"workload": "d8_mixture: d=8 fp32, 2,000,000 refs and queries from a 64-component "
+                                     "anisotropic mixture, k=32, loss sum(w*dists), grads w.r.t. queries and refs",
+                         "levels": tree.levels, "fwd_bwd_per_s": q.shape[0] * reps / el,
+                         "ms_per_step": el * 1e3 / reps, "traverse_ms": tr / max(n, 1), "backward_ms": bw / max(m, 1)}
+    del tree, ref, q, wt
+    w = workloads.fp64_normal()
+    tree = tk.build_kd_tree(torch.from_numpy(w["ref"]), device=dev, levels="auto")
+    q = torch.from_numpy(w["queries"]).to(dev)
+    tree.query(q, nr_nns_searches=w["k"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        tree.query(q, nr_nns_searches=w["k"])
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    out["fp64_fwd"] = {"workload": "fp64_normal: d=3 fp64, 500,000 refs and queries N(0,1)*1e3, k=16",
+                       "levels": tree.levels, "queries_per_s": q.shape[0] * reps / el, "ms_per_step": el * 1e3 / reps}
+    return out
+
+
 def _extra_batch(np, torch, torch_knn, tk, workloads, dev, reps):
     """Batching workload: 256 clouds, fwd + bwd, batched graph replay vs a per-cloud loop of the same kernels."""
     clouds = workloads.batch_clouds()
@@ -320,6 +366,8 @@ def main():
         del res_d, res_i, q
         log("secondary: differentiation fwd+bwd")
         extras["fwd_bwd"] = _extra_fwd_bwd(np, torch, torch_knn, tk, workloads, dev, 10)
+        log("secondary: d=8 and fp64 stress")
+        extras.update(_extra_stress(np, torch, torch_knn, tk, workloads, dev, 5))
         log("secondary: batching (256 clouds)")
         extras["batching"] = _extra_batch(np, torch, torch_knn, tk, workloads, dev, 3)
         extras["batched_vs_per_cloud"] = extras["batching"]["batched_vs_per_cloud"]

@@ -76,6 +76,17 @@ int tk_tree_build(const void* points, int64_t n_points, int dims, int dtype, int
 int tk_tree_build_many(const void* const* points, const int64_t* n_points, int n_trees, int dims, int dtype,
                        const int32_t* levels, int n_threads, tk_tree** out);
 
+/* Tree serialization (no counterpart in the reference, whose trees exist only in memory; it
+ * extends the read-only getters of src/interface.cpp:45-58 so that a static cloud is built
+ * once).  The image is versioned and checksummed: split values, leaf offsets, structured points
+ * and shuffled indices.  tk_tree_serialize with buf == NULL stores the image size in *bytes;
+ * otherwise *bytes is the buffer size on entry and the image size on return.  Loading checks
+ * the checksum and the structure before returning a host tree (upload it with tk_tree_upload). */
+int tk_tree_serialize(const tk_tree* tree, void* buf, size_t* bytes);
+int tk_tree_deserialize(const void* buf, size_t bytes, tk_tree** out);
+int tk_tree_save(const tk_tree* tree, const char* path);
+int tk_tree_load(const char* path, tk_tree** out);
+
 /* Replaces copyPartitionToGPU (src/kdtree_g.cu:63-98), called from the constructor at
  * src/interface.cpp:38-41.  Copies the tree to GPU `device` (-1 = current device). */
 int tk_tree_upload(tk_tree* tree, int device);

@@ -268,3 +268,72 @@ def test_c_abi_build_query_cpu_and_errors():
     assert lib.tk_tree_build(pts.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(1000), 5, 0, 7, 0,
                              ctypes.byref(bad)) != 0
     assert b"dimensionality" in lib.tk_last_error()
+
+
+# --- tree serialization --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("dtype,d", [(np.float32, 3), (np.float64, 2), (np.float32, 8), (np.float64, 1)])
+def test_tree_save_load_roundtrip(tmp_path, dtype, d):
+    rng = np.random.default_rng(900 + d)
+    pts = np.round(rng.standard_normal((3000, d)) * 4).astype(dtype)  # duplicates included
+    tree = tk.build_kd_tree(torch.from_numpy(pts), device="cpu", levels=7)
+    path = tmp_path / "tree.bin"
+    tree.save(path)
+    loaded = tk.load_kd_tree(path, device="cpu")
+    assert loaded.levels == tree.levels and loaded.dims == d and loaded.dtype == tree.dtype
+    np.testing.assert_array_equal(loaded.kdtree.get_structured_points(), tree.kdtree.get_structured_points())
+    np.testing.assert_array_equal(loaded.kdtree.get_shuffled_inds(), tree.kdtree.get_shuffled_inds())
+    for a, b in zip(loaded.kdtree.get_nodes(), tree.kdtree.get_nodes()):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(loaded.points_ref_bak.numpy(), pts)
+    q = torch.from_numpy(rng.standard_normal((500, d)).astype(dtype) * 4)
+    d0, i0 = tree.query(q, nr_nns_searches=5)
+    d1, i1 = loaded.query(q, nr_nns_searches=5)
+    assert torch.equal(d0, d1) and torch.equal(i0, i1)
+
+
+def test_tree_load_rejects_damaged_files(tmp_path):
+    pts = np.random.default_rng(3).random((1000, 3)).astype(np.float32)
+    path = tmp_path / "tree.bin"
+    tk.build_kd_tree(torch.from_numpy(pts), device="cpu", levels=5).save(path)
+    good = path.read_bytes()
+    for name, data in [("flipped", good[:100] + bytes([good[100] ^ 1]) + good[101:]),
+                       ("truncated", good[:-9]),
+                       ("empty", b""),
+                       ("magic", b"XKDTREE\0" + good[8:])]:
+        bad = tmp_path / ("bad_" + name)
+        bad.write_bytes(data)
+        with pytest.raises(RuntimeError):
+            tk.load_kd_tree(bad, device="cpu")
+    with pytest.raises(RuntimeError):
+        tk.load_kd_tree(tmp_path / "missing.bin", device="cpu")
+
+
+def test_c_abi_serialize_roundtrip():
+    lib = ctypes.CDLL(os.path.join(ROOT, "torch_kdtree_amd", "libtorch_kdtree_amd.so"))
+    vp = ctypes.c_void_p
+    lib.tk_last_error.restype = ctypes.c_char_p
+    lib.tk_tree_build.argtypes = [vp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(vp)]
+    lib.tk_tree_serialize.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_size_t)]
+    lib.tk_tree_deserialize.argtypes = [vp, ctypes.c_size_t, ctypes.POINTER(vp)]
+    lib.tk_tree_get_shuffled_inds.argtypes = [vp, vp]
+    lib.tk_tree_free.argtypes = [vp]
+    pts = np.random.default_rng(5).random((777, 2)).astype(np.float64)
+    h = vp()
+    assert lib.tk_tree_build(pts.ctypes.data, 777, 2, 1, 6, 0, ctypes.byref(h)) == 0, lib.tk_last_error()
+    n = ctypes.c_size_t(0)
+    assert lib.tk_tree_serialize(h, None, ctypes.byref(n)) == 0
+    buf = ctypes.create_string_buffer(n.value)
+    small = ctypes.c_size_t(n.value - 1)
+    assert lib.tk_tree_serialize(h, buf, ctypes.byref(small)) != 0  # too small a buffer is an error
+    assert lib.tk_tree_serialize(h, buf, ctypes.byref(n)) == 0
+    h2 = vp()
+    assert lib.tk_tree_deserialize(buf, n.value, ctypes.byref(h2)) == 0, lib.tk_last_error()
+    a = np.empty(777, np.int32)
+    b = np.empty(777, np.int32)
+    lib.tk_tree_get_shuffled_inds(h, a.ctypes.data)
+    lib.tk_tree_get_shuffled_inds(h2, b.ctypes.data)
+    np.testing.assert_array_equal(a, b)
+    lib.tk_tree_free(h)
+    lib.tk_tree_free(h2)

@@ -156,3 +156,16 @@ def test_repeatable(cuda):
     b = gpu_query(tree, qry, 16, cuda)
     for x, y in zip(a, b):
         assert_same(x, y, "second run")
+
+
+def test_saved_tree_loads_onto_the_gpu(cuda, tmp_path):
+    rng = np.random.default_rng(4242)
+    pts = rng.random((20000, 3)).astype(np.float32)
+    tree = tk.build_kd_tree(torch.from_numpy(pts), device=cuda, levels="auto")
+    path = tmp_path / "gpu_tree.bin"
+    tree.save(path)
+    loaded = tk.load_kd_tree(path, device=cuda)
+    q = torch.from_numpy(rng.random((5000, 3)).astype(np.float32)).to(cuda)
+    d0, i0 = tree.query(q, nr_nns_searches=8)
+    d1, i1 = loaded.query(q, nr_nns_searches=8)
+    assert torch.equal(d0, d1) and torch.equal(i0, i1)

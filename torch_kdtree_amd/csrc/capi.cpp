@@ -117,6 +117,41 @@ int tk_tree_build_many(const void* const* points, const int64_t* n_points, int n
     });
 }
 
+int tk_tree_serialize(const tk_tree* tree, void* buf, size_t* bytes) {
+    return guarded([&] {
+        if (!tree || !tree->t || !bytes) throw std::runtime_error("invalid arguments");
+        const std::vector<unsigned char> img = tk::serialize_tree(*tree->t);
+        if (buf) {
+            if (*bytes < img.size()) throw std::runtime_error("serialization buffer too small");
+            std::memcpy(buf, img.data(), img.size());
+        }
+        *bytes = img.size();
+    });
+}
+
+int tk_tree_deserialize(const void* buf, size_t bytes, tk_tree** out) {
+    return guarded([&] {
+        if (!out || (!buf && bytes)) throw std::runtime_error("invalid arguments");
+        *out = nullptr;
+        *out = new tk_tree{tk::deserialize_tree(static_cast<const unsigned char*>(buf), bytes)};
+    });
+}
+
+int tk_tree_save(const tk_tree* tree, const char* path) {
+    return guarded([&] {
+        if (!tree || !tree->t || !path) throw std::runtime_error("invalid arguments");
+        tk::save_tree(*tree->t, path);
+    });
+}
+
+int tk_tree_load(const char* path, tk_tree** out) {
+    return guarded([&] {
+        if (!out || !path) throw std::runtime_error("invalid arguments");
+        *out = nullptr;
+        *out = new tk_tree{tk::load_tree(path)};
+    });
+}
+
 int tk_tree_upload(tk_tree* tree, int device) {
     return guarded([&] {
         if (!tree || !tree->t) throw std::runtime_error("null tree handle");

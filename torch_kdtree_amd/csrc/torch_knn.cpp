@@ -58,7 +58,35 @@ struct KDTree {
             throw std::runtime_error(msg);
         }
     }
+    // Adopts a loaded host tree (uploading it for GPU classes).
+    KDTree(tk_tree* loaded, int device) : tree(loaded) {
+        int32_t info[5];
+        check(tk_tree_info(tree, info));
+        levels = info[2];
+        if (kGpu) {
+            int rc;
+            {
+                py::gil_scoped_release release;
+                rc = tk_tree_upload(tree, device);
+            }
+            if (rc != 0) {
+                const std::string msg = tk_last_error();
+                tk_tree_free(tree);
+                tree = nullptr;
+                throw std::runtime_error(msg);
+            }
+        }
+    }
     ~KDTree() { tk_tree_free(tree); }
+
+    void save(const std::string& path) const {
+        int rc;
+        {
+            py::gil_scoped_release release;
+            rc = tk_tree_save(tree, path.c_str());
+        }
+        check(rc);
+    }
 
     int32_t n_points() const {
         int32_t info[5];
@@ -239,6 +267,7 @@ void bind_tree(py::module_& m, const char* name) {
     py::class_<C, std::shared_ptr<C>>(m, name)
         .def(py::init<const py::array_t<T, py::array::c_style | py::array::forcecast>&, int, int>(),
              py::arg("points_ref"), py::arg("levels"), py::kw_only(), py::arg("device") = -1)
+        .def("save", &C::save, py::arg("path"), "Writes the tree to a file (load it with torch_knn.load_tree)")
         .def("get_shuffled_inds", &C::get_shuffled_inds,
              "Returns the shuffled indices to translate from local to global indices")
         .def("get_structured_points", &C::get_structured_points,
@@ -300,6 +329,42 @@ PYBIND11_MODULE(torch_knn, m) {
              py::arg("orig_idx_ptr") = 0, py::arg("grad_dist_ptr") = 0, py::arg("grad_queries_ptr") = 0,
              py::arg("grad_ref_ptr") = 0, py::arg("stream") = 0)
         .def("launch", &BatchPlan::launch, py::arg("stream") = 0);
+
+    m.def(
+        "load_tree",
+        [](const std::string& path, bool gpu, int device) -> py::object {
+            tk_tree* t = nullptr;
+            int rc;
+            {
+                py::gil_scoped_release release;
+                rc = tk_tree_load(path.c_str(), &t);
+            }
+            check(rc);
+            int32_t info[5];
+            if (tk_tree_info(t, info) != 0) {
+                tk_tree_free(t);
+                throw std::runtime_error(tk_last_error());
+            }
+            const int dims = info[0], dtype = info[1];
+#define TK_MAKE(TT, DD)                                                                              \
+    if (dims == DD && dtype == dtype_code<TT>()) {                                                   \
+        if (gpu) return py::cast(std::make_shared<KDTree<TT, DD, true>>(t, device));                 \
+        return py::cast(std::make_shared<KDTree<TT, DD, false>>(t, device));                         \
+    }
+            TK_MAKE(float, 1)
+            TK_MAKE(float, 2)
+            TK_MAKE(float, 3)
+            TK_MAKE(float, 8)
+            TK_MAKE(double, 1)
+            TK_MAKE(double, 2)
+            TK_MAKE(double, 3)
+            TK_MAKE(double, 8)
+#undef TK_MAKE
+            tk_tree_free(t);
+            throw std::runtime_error("tree file: unsupported dims or dtype");
+        },
+        py::arg("path"), py::arg("gpu") = false, py::arg("device") = -1,
+        "Loads a tree written by save(); returns the KDTree{CPU,GPU}{d}D[F] object of its dims and dtype");
 
     m.def("check_for_gpu", [] { return tk_gpu_available() != 0; }, "Check if the library was compiled with GPU support");
     m.def("device_count", [] {

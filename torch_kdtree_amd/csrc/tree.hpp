@@ -61,6 +61,16 @@ inline bool dims_supported(int d) { return d == 1 || d == 2 || d == 3 || d == 8;
 // Builds the tree on the host.  Throws std::runtime_error with the reference's messages.
 Tree* build_tree(const void* points, int64_t n_points, int dims, int levels, int dtype, int n_threads);
 
+// Derived arrays of a tree whose split values, leaf offsets, points and indices are set: the
+// split axes and the point boxes.
+void finish_tree(Tree& t);
+
+// Serialization (tree_io.cpp): a versioned, checksummed byte image of the tree's arrays.
+std::vector<unsigned char> serialize_tree(const Tree& t);
+Tree* deserialize_tree(const unsigned char* buf, size_t n);
+void save_tree(const Tree& t, const char* path);
+Tree* load_tree(const char* path);
+
 // CPU KNN search with the reference's semantics (iterative near-first DFS, upper-bound
 // insertion; src/kdtree.cpp:16-194).
 void query_cpu(const Tree& t, const void* queries, int64_t n_queries, int k, void* dist, int32_t* idx,

@@ -124,7 +124,30 @@ void build_typed(Tree& t, const T* src, int n_threads) {
     compute_aabb<T, D>(t);
 }
 
+template <typename T>
+void compute_aabb_t(Tree& t) {
+    switch (t.dims) {
+        case 1: compute_aabb<T, 1>(t); break;
+        case 2: compute_aabb<T, 2>(t); break;
+        case 3: compute_aabb<T, 3>(t); break;
+        case 8: compute_aabb<T, 8>(t); break;
+        default: throw std::runtime_error("unsupported dims");
+    }
+}
+
 }  // namespace
+
+void finish_tree(Tree& t) {
+    t.axis.assign(t.n_nodes, 0);
+    for (int32_t i = 0, lv = 0; i < t.n_nodes; ++i) {
+        if (i + 1 >= (int32_t(1) << (lv + 1))) ++lv;  // node i is at depth floor(log2(i + 1))
+        t.axis[i] = (uint8_t)(lv % t.dims);
+    }
+    if (t.dtype == kF32)
+        compute_aabb_t<float>(t);
+    else
+        compute_aabb_t<double>(t);
+}
 
 Tree* build_tree(const void* points, int64_t n_points, int dims, int levels, int dtype, int n_threads) {
     if (!dims_supported(dims))

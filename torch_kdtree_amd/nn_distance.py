@@ -86,8 +86,9 @@ class _KnnSquaredDistances(torch.autograd.Function):
 
 class TorchKDTree:
     def __init__(self, points_ref: torch.Tensor, device: torch.device, levels: int, squared_distances: bool,
-                 tie_break: str = "reference", presort="auto"):
-        """Builds the KDTree. See :ref:`build_kd_tree` for more details."""
+                 tie_break: str = "reference", presort="auto", _kdtree=None):
+        """Builds the KDTree. See :ref:`build_kd_tree` for more details.  (``_kdtree``: an already
+        built ``torch_knn`` tree of ``points_ref``, see ``load_kd_tree``.)"""
         assert device.type in ['cpu', 'cuda']
         assert points_ref.shape[0] < 2**31, "Only 32 bit signed indexing implemented"
 
@@ -111,7 +112,12 @@ class TorchKDTree:
         if device.type == "cuda":
             kwargs["device"] = device.index if device.index is not None else torch.cuda.current_device()
             device = torch.device("cuda", kwargs["device"])
-        self.kdtree = tree_cls(points_ref.detach().cpu().numpy(), levels, **kwargs)
+        if _kdtree is not None:
+            if type(_kdtree).__name__ != kdtree_str:
+                raise RuntimeError("loaded tree is a %s, expected %s" % (type(_kdtree).__name__, kdtree_str))
+            self.kdtree = _kdtree
+        else:
+            self.kdtree = tree_cls(points_ref.detach().cpu().numpy(), levels, **kwargs)
         self.levels = levels
 
         self.structured_points = torch.from_numpy(self.kdtree.get_structured_points())
@@ -132,6 +138,11 @@ class TorchKDTree:
         if self.use_gpu:
             self.structured_points = self.structured_points.to(self.device)
             self.shuffled_ind = self.shuffled_ind.to(self.device)
+
+    def save(self, path: str) -> None:
+        """Writes the tree (structured points, indices, nodes and leaves) to ``path``; ``load_kd_tree``
+        reads it back without rebuilding."""
+        self.kdtree.save(str(path))
 
     def query(self, points_query: torch.Tensor, nr_nns_searches: int = 1,
               result_dists: torch.Tensor = None, result_idx: torch.Tensor = None):
@@ -265,3 +276,22 @@ def build_kd_tree(points_ref: Union[torch.Tensor, np.ndarray], device: torch.dev
     assert device.type in ['cuda', 'cpu']
 
     return TorchKDTree(points_ref, device, levels, squared_distances, tie_break=tie_break, presort=presort)
+
+
+def load_kd_tree(path: str, device: torch.device = None, squared_distances=True, tie_break: str = "reference",
+                 presort="auto") -> TorchKDTree:
+    """Loads a tree written by ``TorchKDTree.save`` onto ``device`` (default: the CPU), without
+    rebuilding it.  The reference points, in their original order, are recovered from the file."""
+    if device is None:
+        device = torch.device("cpu")
+    if issubclass(type(device), str):
+        device = torch.device(device)
+    assert device.type in ['cuda', 'cpu']
+    gpu = device.type == 'cuda'
+    dev_index = (device.index if device.index is not None else torch.cuda.current_device()) if gpu else -1
+    kd = torch_knn.load_tree(str(path), gpu, dev_index)
+    structured = torch.from_numpy(kd.get_structured_points())
+    shuffled = torch.from_numpy(kd.get_shuffled_inds()).long()
+    points = torch.empty_like(structured)
+    points[shuffled] = structured
+    return TorchKDTree(points, device, kd.levels, squared_distances, tie_break=tie_break, presort=presort, _kdtree=kd)
