@@ -255,6 +255,9 @@ def _parse():
     ap.add_argument("--presort", default="auto", choices=["auto", "morton", "leaf", "off"])
     ap.add_argument("--cpu-sample", type=int, default=1 << 22, help="queries timed on the CPU (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    ap.add_argument("--section", default="all", choices=["all", "batching", "fwd_bwd", "stress", "bruteforce"],
+                    help="run one secondary measurement alone (for profiling) and print its JSON; 'all' = the "
+                         "benchmark line")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the secondary measurements (brute force, fwd+bwd, reference default depth)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -281,6 +284,18 @@ def main():
         print("[bench] " + msg, file=sys.stderr, flush=True)
 
     dev = torch.device("cuda", 0)
+    if args.section != "all":
+        reps = max(1, args.steps)
+        if args.section == "batching":
+            out = _extra_batch(np, torch, torch_knn, tk, workloads, dev, reps)
+        elif args.section == "fwd_bwd":
+            out = _extra_fwd_bwd(np, torch, torch_knn, tk, workloads, dev, reps)
+        elif args.section == "stress":
+            out = _extra_stress(np, torch, torch_knn, tk, workloads, dev, reps)
+        else:
+            out = _extra_bruteforce(np, torch, torch_knn, dev, workloads.plain_gpu())
+        print(json.dumps({"section": args.section, "steps": reps, **out}), flush=True)
+        return
     log("generating plain_gpu workload")
     w = workloads.plain_gpu()
     ref, qry, k = w["ref"], w["queries"], w["k"]
