@@ -244,6 +244,43 @@ def _extra_batch(np, torch, torch_knn, tk, workloads, dev, reps):
             "batched_algorithmic_gbs": alg / (batched_ms * 1e-3) / 1e9}
 
 
+def _extra_tree_build(np, torch, tk, workloads, dev, w, cpu_build_s, reps):
+    """Tree build on the GPU (builder="gpu") against the host build: the 4M-point plain tree and the
+    256 trees of the batching workload (one GPU build for all of them)."""
+    ref = torch.from_numpy(w["ref"])
+    if cpu_build_s is None:
+        t0 = time.perf_counter()
+        tk.build_kd_tree(ref, device=dev, levels=17)
+        cpu_build_s = time.perf_counter() - t0
+    tk.build_kd_tree(ref, device=dev, levels=17, builder="gpu")  # first call: allocation warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        tk.build_kd_tree(ref, device=dev, levels=17, builder="gpu")
+    torch.cuda.synchronize()
+    gpu_s = (time.perf_counter() - t0) / reps
+    refs = [torch.from_numpy(c["ref"]) for c in workloads.batch_clouds()]
+    t0 = time.perf_counter()
+    cpu_batch = tk.build_kd_tree_batch(refs, device=dev)
+    cpu_batch_s = time.perf_counter() - t0
+    levels = cpu_batch.levels
+    del cpu_batch
+    tk.build_kd_tree_batch(refs, device=dev, levels=levels, builder="gpu")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        tk.build_kd_tree_batch(refs, device=dev, levels=levels, builder="gpu")
+    torch.cuda.synchronize()
+    gpu_batch_s = (time.perf_counter() - t0) / reps
+    return {"plain_gpu_tree": {"points": int(ref.shape[0]), "levels": 17, "host_build_s": cpu_build_s,
+                               "gpu_build_s": gpu_s, "speedup": cpu_build_s / gpu_s},
+            "batch_clouds_trees": {"trees": len(refs), "points": int(sum(r.shape[0] for r in refs)),
+                                   "host_build_s": cpu_batch_s, "gpu_build_s": gpu_batch_s,
+                                   "speedup": cpu_batch_s / gpu_batch_s},
+            "note": "host build = the reference's algorithm, OpenMP over subtrees (one tree) or over trees (256); "
+                    "GPU build = one level-by-level radix-sort build, host copies and upload included"}
+
+
 def _parse():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,7 +292,8 @@ def _parse():
     ap.add_argument("--presort", default="auto", choices=["auto", "morton", "leaf", "off"])
     ap.add_argument("--cpu-sample", type=int, default=1 << 22, help="queries timed on the CPU (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
-    ap.add_argument("--section", default="all", choices=["all", "batching", "fwd_bwd", "stress", "bruteforce"],
+    ap.add_argument("--section", default="all",
+                    choices=["all", "batching", "fwd_bwd", "stress", "bruteforce", "build"],
                     help="run one secondary measurement alone (for profiling) and print its JSON; 'all' = the "
                          "benchmark line")
     ap.add_argument("--no-extras", action="store_true",
@@ -292,6 +330,8 @@ def main():
             out = _extra_fwd_bwd(np, torch, torch_knn, tk, workloads, dev, reps)
         elif args.section == "stress":
             out = _extra_stress(np, torch, torch_knn, tk, workloads, dev, reps)
+        elif args.section == "build":
+            out = _extra_tree_build(np, torch, tk, workloads, dev, workloads.plain_gpu(), None, reps)
         else:
             out = _extra_bruteforce(np, torch, torch_knn, dev, workloads.plain_gpu())
         print(json.dumps({"section": args.section, "steps": reps, **out}), flush=True)
@@ -386,6 +426,8 @@ def main():
         log("secondary: batching (256 clouds)")
         extras["batching"] = _extra_batch(np, torch, torch_knn, tk, workloads, dev, 3)
         extras["batched_vs_per_cloud"] = extras["batching"]["batched_vs_per_cloud"]
+        log("secondary: GPU tree build")
+        extras["tree_build"] = _extra_tree_build(np, torch, tk, workloads, dev, w, build_s, 2)
 
     line = {
         "metric": "KNN queries/s at d=3,k=16 fp32 (+fwd+bwd/s); batched vs per-cloud, same GPU",

@@ -76,6 +76,22 @@ int tk_tree_build(const void* points, int64_t n_points, int dims, int dtype, int
 int tk_tree_build_many(const void* const* points, const int64_t* n_points, int n_trees, int dims, int dtype,
                        const int32_t* levels, int n_threads, tk_tree** out);
 
+/* GPU tree build (no counterpart in the reference, whose createKDTree, src/kdtree.hpp:494-534,
+ * runs on the host only).  points: device array [n_points*dims] on `device` (-1 = current);
+ * the kernels run on `stream` (a hipStream_t; NULL = default stream).  Same node sizes, axes,
+ * split values and leaf ranges as tk_tree_build whenever no node has two points with equal
+ * coordinates on its split axis; with such ties the tied points may be ordered (and split)
+ * differently, because the reference orders them by an unstable std::sort.  Fails when a
+ * coordinate it sorts on is NaN.  The tree is uploaded to `device`.  At most 2^28 points. */
+int tk_tree_build_gpu(const void* points, int64_t n_points, int dims, int dtype, int levels, int device, void* stream,
+                      tk_tree** out);
+
+/* Builds n_trees trees in one GPU build: tree i from the n_points[i] rows that follow the rows of
+ * trees 0..i-1 in the device array `points`, with levels[i] levels.  The trees are returned on
+ * the host (not uploaded), as tk_tree_build_many returns them.  At most 2^28 points in all. */
+int tk_tree_build_many_gpu(const void* points, const int64_t* n_points, int n_trees, int dims, int dtype,
+                           const int32_t* levels, int device, void* stream, tk_tree** out);
+
 /* Tree serialization (no counterpart in the reference, whose trees exist only in memory; it
  * extends the read-only getters of src/interface.cpp:45-58 so that a static cloud is built
  * once).  The image is versioned and checksummed: split values, leaf offsets, structured points

@@ -182,6 +182,10 @@ def test_levels_and_point_count_errors():
     tk.build_kd_tree(torch.randn(128, 3), device="cpu", levels=8)
     with pytest.raises(ValueError):
         tk.build_kd_tree(pts, device="cpu", tie_break="random")
+    with pytest.raises(RuntimeError, match="needs a GPU"):
+        tk.build_kd_tree(pts, device="cpu", builder="gpu")
+    with pytest.raises(ValueError, match="builder"):
+        tk.build_kd_tree(pts, device="cpu", builder="fast")
 
 
 def test_unsupported_dtype_raises():

@@ -20,6 +20,8 @@ from .nn_distance import _TIE_POLICIES, _dtype_name, _stream_handle, default_lev
 
 __all__ = ["KDTreeBatch", "BatchStep", "build_kd_tree_batch"]
 
+_DTYPE_CODES = {torch.float32: 0, torch.float64: 1}
+
 
 def _as_host_array(x) -> np.ndarray:
     if isinstance(x, torch.Tensor):
@@ -62,10 +64,12 @@ class KDTreeBatch:
     levels : "auto" (a depth chosen per cloud for the GPU), None (the reference's default rule per
         cloud), an int for all clouds, or one value per cloud
     tie_break : "reference" or "lower_index", as for ``build_kd_tree``
+    builder : "cpu" (default: the reference's build, in parallel over clouds) or "gpu" (one GPU
+        build of all trees; see ``build_kd_tree`` for how it can differ on tied coordinates)
     """
 
     def __init__(self, clouds: Sequence, device=None, levels: Union[str, int, None, List] = "auto",
-                 tie_break: str = "reference", n_threads: int = 0):
+                 tie_break: str = "reference", n_threads: int = 0, builder: str = "cpu"):
         if tie_break not in _TIE_POLICIES:
             raise ValueError("tie_break must be one of %s" % sorted(_TIE_POLICIES))
         if device is None:
@@ -74,12 +78,21 @@ class KDTreeBatch:
         if self.device.type != "cuda":
             raise RuntimeError("the batched mode runs on a GPU device")
         self.tie_break = tie_break
-        arrays = [_as_host_array(c) for c in clouds]
-        if not arrays:
+        if builder not in ("cpu", "gpu"):
+            raise ValueError("builder must be 'cpu' or 'gpu'")
+        if not len(clouds):
             raise ValueError("no clouds")
-        self.dims = arrays[0].shape[1]
-        self.dtype = torch.from_numpy(arrays[0][:0]).dtype
-        n = [a.shape[0] for a in arrays]
+        if builder == "gpu":
+            tensors = [c.detach() if isinstance(c, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(c))
+                       for c in clouds]
+            self.dims = tensors[0].shape[1]
+            self.dtype = tensors[0].dtype
+            n = [t.shape[0] for t in tensors]
+        else:
+            arrays = [_as_host_array(c) for c in clouds]
+            self.dims = arrays[0].shape[1]
+            self.dtype = torch.from_numpy(arrays[0][:0]).dtype
+            n = [a.shape[0] for a in arrays]
         if isinstance(levels, (list, tuple)):
             lv = [int(x) for x in levels]
         elif levels == "auto":
@@ -90,7 +103,16 @@ class KDTreeBatch:
             lv = [int(levels)] * len(n)
         if len(lv) != len(n) or not all(1 <= x <= MAX_LEVELS for x in lv):
             raise ValueError("levels must be in [1, %d], one per cloud" % MAX_LEVELS)
-        self.trees = torch_knn.TreeSet(arrays, lv, n_threads)
+        if builder == "gpu":
+            for t in tensors:
+                if t.dim() != 2 or t.shape[1] != self.dims or t.dtype != self.dtype:
+                    raise RuntimeError("all clouds must have the same dims and dtype")
+            pts = torch.cat([t.to(self.device) for t in tensors]).contiguous()
+            self.trees = torch_knn.TreeSet.from_device(pts.data_ptr(), n, lv, self.dims, _DTYPE_CODES[self.dtype],
+                                                       self.device.index if self.device.index is not None else
+                                                       torch.cuda.current_device(), _stream_handle(self.device))
+        else:
+            self.trees = torch_knn.TreeSet(arrays, lv, n_threads)
         self.levels = list(self.trees.levels)
         self.nr_ref_points = n
         self._clouds = [c if isinstance(c, torch.Tensor) else None for c in clouds]
@@ -195,6 +217,7 @@ class BatchStep:
         return list(torch.split(flat, self.batch.nr_ref_points))
 
 
-def build_kd_tree_batch(clouds: Sequence, device=None, levels="auto", tie_break: str = "reference") -> KDTreeBatch:
+def build_kd_tree_batch(clouds: Sequence, device=None, levels="auto", tie_break: str = "reference",
+                        builder: str = "cpu") -> KDTreeBatch:
     """Builds the trees of many clouds for batched queries (see ``KDTreeBatch``)."""
-    return KDTreeBatch(clouds, device=device, levels=levels, tie_break=tie_break)
+    return KDTreeBatch(clouds, device=device, levels=levels, tie_break=tie_break, builder=builder)

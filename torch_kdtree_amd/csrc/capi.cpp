@@ -117,6 +117,31 @@ int tk_tree_build_many(const void* const* points, const int64_t* n_points, int n
     });
 }
 
+int tk_tree_build_gpu(const void* points, int64_t n_points, int dims, int dtype, int levels, int device, void* stream,
+                      tk_tree** out) {
+    return guarded([&] {
+        if (!out) throw std::runtime_error("invalid arguments");
+        *out = nullptr;
+        check_dtype(dtype);
+        const int32_t lv = levels;
+        std::vector<tk::Tree*> t = tk::build_trees_gpu(points, &n_points, &lv, 1, dims, dtype, device, stream);
+        std::unique_ptr<tk::Tree> owned(t[0]);
+        tk::upload_tree(*owned, device);
+        *out = new tk_tree{owned.release()};
+    });
+}
+
+int tk_tree_build_many_gpu(const void* points, const int64_t* n_points, int n_trees, int dims, int dtype,
+                           const int32_t* levels, int device, void* stream, tk_tree** out) {
+    return guarded([&] {
+        if (n_trees < 1 || !n_points || !levels || !out) throw std::runtime_error("invalid arguments");
+        check_dtype(dtype);
+        for (int i = 0; i < n_trees; ++i) out[i] = nullptr;
+        std::vector<tk::Tree*> t = tk::build_trees_gpu(points, n_points, levels, n_trees, dims, dtype, device, stream);
+        for (int i = 0; i < n_trees; ++i) out[i] = new tk_tree{t[i]};
+    });
+}
+
 int tk_tree_serialize(const tk_tree* tree, void* buf, size_t* bytes) {
     return guarded([&] {
         if (!tree || !tree->t || !bytes) throw std::runtime_error("invalid arguments");

@@ -58,12 +58,12 @@ struct KDTree {
             throw std::runtime_error(msg);
         }
     }
-    // Adopts a loaded host tree (uploading it for GPU classes).
-    KDTree(tk_tree* loaded, int device) : tree(loaded) {
+    // Adopts a built or loaded tree (uploading it for GPU classes unless it is on the GPU already).
+    KDTree(tk_tree* loaded, int device, bool uploaded = false) : tree(loaded) {
         int32_t info[5];
         check(tk_tree_info(tree, info));
         levels = info[2];
-        if (kGpu) {
+        if (kGpu && !uploaded) {
             int rc;
             {
                 py::gil_scoped_release release;
@@ -144,6 +144,9 @@ struct TreeSet {
     int dims = 0;
     int dtype = TK_F32;
 
+    TreeSet() = default;
+    TreeSet(const TreeSet&) = delete;
+    TreeSet& operator=(const TreeSet&) = delete;
     TreeSet(const py::list& clouds, const std::vector<int32_t>& levels, int n_threads) {
         const size_t n = clouds.size();
         if (n == 0) throw std::runtime_error("no clouds");
@@ -179,6 +182,27 @@ struct TreeSet {
     }
     ~TreeSet() {
         for (tk_tree* t : trees) tk_tree_free(t);
+    }
+    // Trees built in one GPU build from a concatenated device array (tk_tree_build_many_gpu).
+    static std::shared_ptr<TreeSet> from_device(size_t points_ptr, const std::vector<int64_t>& n_points,
+                                                const std::vector<int32_t>& levels, int dims_, int dtype_, int device,
+                                                size_t stream) {
+        const size_t n = n_points.size();
+        if (n == 0) throw std::runtime_error("no clouds");
+        if (levels.size() != n) throw std::runtime_error("one levels value per cloud is required");
+        auto ts = std::make_shared<TreeSet>();
+        ts->dims = dims_;
+        ts->dtype = dtype_;
+        ts->trees.assign(n, nullptr);
+        int rc;
+        {
+            py::gil_scoped_release release;
+            rc = tk_tree_build_many_gpu(reinterpret_cast<const void*>(points_ptr), n_points.data(), (int)n, dims_,
+                                        dtype_, levels.data(), device, reinterpret_cast<void*>(stream),
+                                        ts->trees.data());
+        }
+        check(rc);
+        return ts;
     }
     std::vector<int> info(int field) const {
         std::vector<int> out;
@@ -315,6 +339,9 @@ PYBIND11_MODULE(torch_knn, m) {
         .def_property_readonly("levels", [](const TreeSet& t) { return t.info(2); })
         .def_property_readonly("n_points", [](const TreeSet& t) { return t.info(3); })
         .def_property_readonly("dims", [](const TreeSet& t) { return t.dims; })
+        .def_static("from_device", &TreeSet::from_device, py::arg("points_ptr"), py::arg("n_points"),
+                    py::arg("levels"), py::arg("dims"), py::arg("dtype"), py::arg("device") = -1,
+                    py::arg("stream") = 0, "Builds the trees of a concatenated device array on the GPU")
         .def("__len__", [](const TreeSet& t) { return t.trees.size(); });
     py::class_<BatchPlan, std::shared_ptr<BatchPlan>>(m, "BatchPlan", "Batched KNN plan over a TreeSet")
         .def(py::init<const TreeSet&, const std::vector<int64_t>&, const std::vector<int32_t>&, int, int>(),
@@ -365,6 +392,37 @@ PYBIND11_MODULE(torch_knn, m) {
         },
         py::arg("path"), py::arg("gpu") = false, py::arg("device") = -1,
         "Loads a tree written by save(); returns the KDTree{CPU,GPU}{d}D[F] object of its dims and dtype");
+
+    m.def(
+        "build_tree_gpu",
+        [](size_t points_ptr, int64_t n_points, int dims, const std::string& dtype, int levels, int device,
+           size_t stream) -> py::object {
+            const int dt = dtype_from_str(dtype);
+            tk_tree* t = nullptr;
+            int rc;
+            {
+                py::gil_scoped_release release;
+                rc = tk_tree_build_gpu(reinterpret_cast<const void*>(points_ptr), n_points, dims, dt, levels, device,
+                                       reinterpret_cast<void*>(stream), &t);
+            }
+            check(rc);
+#define TK_MAKE(TT, DD) \
+    if (dims == DD && dt == dtype_code<TT>()) return py::cast(std::make_shared<KDTree<TT, DD, true>>(t, device, true));
+            TK_MAKE(float, 1)
+            TK_MAKE(float, 2)
+            TK_MAKE(float, 3)
+            TK_MAKE(float, 8)
+            TK_MAKE(double, 1)
+            TK_MAKE(double, 2)
+            TK_MAKE(double, 3)
+            TK_MAKE(double, 8)
+#undef TK_MAKE
+            tk_tree_free(t);
+            throw std::runtime_error("unsupported dims or dtype");
+        },
+        py::arg("points_ptr"), py::arg("n_points"), py::arg("dims"), py::arg("dtype"), py::arg("levels"),
+        py::arg("device") = -1, py::arg("stream") = 0,
+        "Builds a tree on the GPU from a device array; returns the KDTreeGPU{d}D[F] object");
 
     m.def("check_for_gpu", [] { return tk_gpu_available() != 0; }, "Check if the library was compiled with GPU support");
     m.def("device_count", [] {

@@ -61,6 +61,14 @@ inline bool dims_supported(int d) { return d == 1 || d == 2 || d == 3 || d == 8;
 // Builds the tree on the host.  Throws std::runtime_error with the reference's messages.
 Tree* build_tree(const void* points, int64_t n_points, int dims, int levels, int dtype, int n_threads);
 
+// Builds trees on the GPU (tree_build_gpu.hip): tree c from points[base_c .. base_c + n_points[c])
+// of the concatenated device array `points` ([sum n_points, dims] values of dtype, on `device`),
+// with levels[c] levels; the work runs on `stream` (a hipStream_t).  Returns host trees (not
+// uploaded).  Equal to build_tree whenever no node has two points with equal coordinates on its
+// split axis; otherwise equal up to the order of such points.
+std::vector<Tree*> build_trees_gpu(const void* points, const int64_t* n_points, const int32_t* levels, int n_trees,
+                                   int dims, int dtype, int device, void* stream);
+
 // Derived arrays of a tree whose split values, leaf offsets, points and indices are set: the
 // split axes and the point boxes.
 void finish_tree(Tree& t);

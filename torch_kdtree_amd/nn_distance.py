@@ -86,7 +86,7 @@ class _KnnSquaredDistances(torch.autograd.Function):
 
 class TorchKDTree:
     def __init__(self, points_ref: torch.Tensor, device: torch.device, levels: int, squared_distances: bool,
-                 tie_break: str = "reference", presort="auto", _kdtree=None):
+                 tie_break: str = "reference", presort="auto", _kdtree=None, builder: str = "cpu"):
         """Builds the KDTree. See :ref:`build_kd_tree` for more details.  (``_kdtree``: an already
         built ``torch_knn`` tree of ``points_ref``, see ``load_kd_tree``.)"""
         assert device.type in ['cpu', 'cuda']
@@ -116,8 +116,16 @@ class TorchKDTree:
             if type(_kdtree).__name__ != kdtree_str:
                 raise RuntimeError("loaded tree is a %s, expected %s" % (type(_kdtree).__name__, kdtree_str))
             self.kdtree = _kdtree
-        else:
+        elif builder == "gpu":
+            if device.type != "cuda":
+                raise RuntimeError("builder='gpu' needs a GPU device")
+            pts = points_ref.detach().to(device).contiguous()
+            self.kdtree = torch_knn.build_tree_gpu(pts.data_ptr(), pts.shape[0], self.dims, _dtype_name(self.dtype),
+                                                   levels, kwargs["device"], _stream_handle(device))
+        elif builder == "cpu":
             self.kdtree = tree_cls(points_ref.detach().cpu().numpy(), levels, **kwargs)
+        else:
+            raise ValueError("builder must be 'cpu' or 'gpu'")
         self.levels = levels
 
         self.structured_points = torch.from_numpy(self.kdtree.get_structured_points())
@@ -226,10 +234,12 @@ class TorchKDTree:
 
 
 def build_kd_tree(points_ref: Union[torch.Tensor, np.ndarray], device: torch.device = None,
-                  squared_distances=True, levels: int = None, tie_break: str = "reference", presort="auto"):
+                  squared_distances=True, levels: int = None, tie_break: str = "reference", presort="auto",
+                  builder: str = "cpu"):
     """Builds the KD-Tree for subsequent queries.
 
-    The tree is always built on the CPU and then transferred to the GPU if necessary.
+    By default the tree is built on the CPU (the reference's build, exactly) and then transferred
+    to the GPU if necessary.
 
     Parameters
     ----------
@@ -248,6 +258,11 @@ def build_kd_tree(points_ref: Union[torch.Tensor, np.ndarray], device: torch.dev
     presort : "auto", "on"/"morton", "leaf" or "off"
         How GPU queries are ordered before the traversal (Morton buckets of their positions, or
         buckets by tree leaf); the order changes speed only, never results.
+    builder : "cpu" (default) or "gpu"
+        "gpu" builds a GPU tree on the device.  It gives the same tree as the CPU build whenever no
+        node has two points with equal coordinates on its split axis; points with such ties may be
+        ordered and split differently (the reference orders them by an unstable sort), so
+        structured indices can differ while the neighbour distances do not.
 
     Returns
     -------
@@ -275,7 +290,8 @@ def build_kd_tree(points_ref: Union[torch.Tensor, np.ndarray], device: torch.dev
         "You requested the KD-Tree on the GPU, but the library was compiled with CPU support only"
     assert device.type in ['cuda', 'cpu']
 
-    return TorchKDTree(points_ref, device, levels, squared_distances, tie_break=tie_break, presort=presort)
+    return TorchKDTree(points_ref, device, levels, squared_distances, tie_break=tie_break, presort=presort,
+                       builder=builder)
 
 
 def load_kd_tree(path: str, device: torch.device = None, squared_distances=True, tie_break: str = "reference",
