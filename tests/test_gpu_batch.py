@@ -83,9 +83,13 @@ def test_batch_forward_matches_oracle_and_single_tree(cuda, tie, dtype, d):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_batch_backward_matches_single_tree(cuda, dtype):
-    rng = np.random.default_rng(77 + (dtype == np.float64))
-    clouds, queries, ks = _clouds(rng, 7, 3, dtype, sizes=(20, 3000))
+@pytest.mark.parametrize("n_clouds,sizes", [(7, (20, 3000)), (40, (1800, 3000))])
+def test_batch_backward_matches_single_tree(cuda, dtype, n_clouds, sizes):
+    """With 40 clouds of at most 3,000 points (over 65,536 points in all) the pair sort keys carry
+    the cloud in their high bits and sort by the record within the cloud (two passes instead of
+    three); with 7 small clouds they are arena records."""
+    rng = np.random.default_rng(77 + (dtype == np.float64) + n_clouds)
+    clouds, queries, ks = _clouds(rng, n_clouds, 3, dtype, sizes=sizes)
     batch = tk.build_kd_tree_batch(clouds, device=cuda)
     plan = batch.plan([q.shape[0] for q in queries], ks)
     total_q, total_out, total_ref, _ = plan.sizes()

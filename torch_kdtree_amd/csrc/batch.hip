@@ -211,6 +211,27 @@ BatchPlan* batch_create(const Tree* const* trees, int n_clouds, const int64_t* n
     int kb = 1;
     while (kb < 32 && (int64_t(1) << kb) < nb) ++kb;
     p->key_bits = kb;
+    // pair keys: the record within its cloud when that saves a sort pass (8-bit digits)
+    {
+        int64_t max_n = 1;
+        for (const CloudDesc& cd : p->clouds) max_n = std::max<int64_t>(max_n, cd.n_points);
+        auto bits = [](int64_t n) {
+            int b = 1;
+            while (b < 32 && (int64_t(1) << b) < n) ++b;
+            return b;
+        };
+        const int global_bits = bits(std::max<int64_t>(p->total_ref, 1));
+        const int local_bits = bits(max_n);
+        const int sort_bits = 8 * ((local_bits + 7) / 8);
+        if (sort_bits < 32 && sort_bits / 8 < (global_bits + 7) / 8 &&
+            (int64_t)n_clouds <= (int64_t(1) << (32 - sort_bits))) {
+            p->pair_bits = sort_bits;
+            p->pair_tag_shift = sort_bits;
+        } else {
+            p->pair_bits = global_bits;
+            p->pair_tag_shift = 0;
+        }
+    }
 
     // host staging of the arena
     const size_t ts = dtype_size(p->dtype);

@@ -83,6 +83,8 @@ struct BatchPlan {
     uint32_t* keys_s = nullptr;
     uint32_t* vals_s = nullptr;
     int key_bits = 1;           // bits of the presort keys
+    int pair_bits = 1;          // bits the dL/dr pair sort orders by
+    int pair_tag_shift = 0;     // pair keys: 0 = arena record; else cloud << shift | record within the cloud
     void* radix_ws = nullptr;
     size_t radix_ws_bytes = 0;
     // captured forward + backward

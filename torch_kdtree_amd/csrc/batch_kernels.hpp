@@ -44,6 +44,14 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bucket_kernel(const unsigne
     row[i] = (uint32_t)i;
 }
 
+// Key of a (record, slot) pair for the dL/dr sort: the arena record position, or, with a tag shift,
+// the cloud index above the tag shift and the record's position within its cloud below it (the
+// sort then needs only the bits below the shift; equal low bits of different clouds stay apart,
+// because each cloud's slots are one contiguous range and the sort is stable).
+__device__ __forceinline__ uint32_t pair_key(int32_t cloud, int32_t local, int32_t rec_base, int tag_shift) {
+    return tag_shift ? (((uint32_t)cloud << tag_shift) | (uint32_t)local) : (uint32_t)(rec_base + local);
+}
+
 // One lane per query of the k bucket K; `order` holds the presorted query rows.
 template <typename T, int D, int K, bool kLower>
 __global__ __launch_bounds__(kTreeBlock) void knn_batch_kernel(TreeView<T, true> arena,
@@ -55,7 +63,7 @@ __global__ __launch_bounds__(kTreeBlock) void knn_batch_kernel(TreeView<T, true>
                                                                 int32_t* __restrict__ out_i, int64_t* __restrict__ out_o,
                                                                 uint32_t* __restrict__ pair_keys,
                                                                 uint32_t* __restrict__ pair_vals,
-                                                                T* __restrict__ q_sorted) {
+                                                                T* __restrict__ q_sorted, int tag_shift) {
     constexpr int CH = StageBytes<T, K, kLower>::kChunk;
     constexpr int LD = StageBytes<T, K, kLower>::kLoads;
     static_assert(LD <= kRecPad, "a load group may read at most kRecPad records past a leaf's end");
@@ -64,7 +72,8 @@ __global__ __launch_bounds__(kTreeBlock) void knn_batch_kernel(TreeView<T, true>
     const int64_t t = (int64_t)blockIdx.x * kTreeBlock + threadIdx.x;
     if (t >= count) return;
     const int64_t qi = order[begin + t];
-    const CloudDesc cd = clouds[qcloud[qi]];
+    const int32_t cloud = qcloud[qi];
+    const CloudDesc cd = clouds[cloud];
     TreeView<T, true> tv = arena;
     tv.node_base = cd.node_base;
     tv.box_base = cd.box_base;
@@ -102,7 +111,7 @@ __global__ __launch_bounds__(kTreeBlock) void knn_batch_kernel(TreeView<T, true>
             out_i[ob + j - pad] = loc;
             if (out_o) out_o[ob + j - pad] = kLower ? x : load_i32_at(arena.shuffled, x);
             if (pair_keys) {
-                pair_keys[sb + j - pad] = (uint32_t)(cd.rec_base + loc);
+                pair_keys[sb + j - pad] = pair_key(cloud, loc, cd.rec_base, tag_shift);
                 pair_vals[sb + j - pad] = (uint32_t)(sb + j - pad);
             }
         }
@@ -124,10 +133,11 @@ __global__ __launch_bounds__(kBatchBlock) void batch_grad_q_pairs_kernel(
     const unsigned char* __restrict__ recs, const CloudDesc* __restrict__ clouds, const int32_t* __restrict__ qcloud,
     const uint32_t* __restrict__ rank, const T* __restrict__ queries, int64_t total_q, const int32_t* __restrict__ idx,
     const T* __restrict__ g, T* __restrict__ dq, T* __restrict__ q_sorted, uint32_t* __restrict__ keys,
-    uint32_t* __restrict__ vals) {
+    uint32_t* __restrict__ vals, int tag_shift) {
     const int64_t i = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
     if (i >= total_q) return;
-    const CloudDesc cd = clouds[qcloud[i]];
+    const int32_t cloud = qcloud[i];
+    const CloudDesc cd = clouds[cloud];
     const int64_t ob = cd.out_base + (i - cd.q_base) * cd.k;
     T qi[D], acc[D];
 #pragma unroll
@@ -151,7 +161,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_grad_q_pairs_kernel(
             acc[a] = acc[a] + gij * (diff + diff);
         }
         if (q_sorted) {
-            keys[sb + j] = (uint32_t)s;
+            keys[sb + j] = pair_key(cloud, idx[ob + j], cd.rec_base, tag_shift);
             vals[sb + j] = (uint32_t)(sb + j);
         }
     }
@@ -216,15 +226,23 @@ __global__ __launch_bounds__(kBatchBlock) void batch_zero_runs_kernel(uint32_t* 
     }
 }
 
-// Bounds of each record's run in the sorted pairs (zero for records without pairs).
+// Bounds of each record's run in the sorted pairs (zero for records without pairs).  A run is a
+// maximal range of equal keys (whole keys, tag included).
 __global__ __launch_bounds__(kBatchBlock) void batch_run_bounds_kernel(const uint32_t* __restrict__ keys, int64_t m,
+                                                                        const CloudDesc* __restrict__ clouds,
+                                                                        int tag_shift,
                                                                         uint32_t* __restrict__ run_start,
                                                                         uint32_t* __restrict__ run_end) {
     const int64_t p = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
     if (p >= m) return;
     const uint32_t key = keys[p];
-    if (p == 0 || keys[p - 1] != key) run_start[key] = (uint32_t)p;
-    if (p == m - 1 || keys[p + 1] != key) run_end[key] = (uint32_t)(p + 1);
+    const bool first = p == 0 || keys[p - 1] != key;
+    const bool last = p == m - 1 || keys[p + 1] != key;
+    if (!first && !last) return;
+    const uint32_t rec = tag_shift ? (uint32_t)clouds[key >> tag_shift].rec_base + (key & ((1u << tag_shift) - 1u))
+                                   : key;
+    if (first) run_start[rec] = (uint32_t)p;
+    if (last) run_end[rec] = (uint32_t)(p + 1);
 }
 
 // dL/dr, one lane per record: its run of sorted pairs (canonical order), written to the record's
@@ -329,7 +347,7 @@ void batch_traverse_td(const BatchPlan& p, const T* q, T* dist, int32_t* idx, in
 #define TK_LAUNCH_K(KK)                                                                                              \
     case KK:                                                                                                         \
         hipLaunchKernelGGL((knn_batch_kernel<T, D, KK, kLower>), grid, dim3(kTreeBlock), 0, s, arena, p.d_clouds,   \
-                           p.qcloud, q, p.order, r.begin, r.count, dist, idx, orig, pk, pv, qs);                     \
+                           p.qcloud, q, p.order, r.begin, r.count, dist, idx, orig, pk, pv, qs, p.pair_tag_shift);   \
         break;
         switch (r.K) {
             TK_LAUNCH_K(1)
@@ -393,16 +411,16 @@ void batch_backward_t(const BatchPlan& p, const T* q, const int32_t* idx, const 
         } else if (p.total_q > 0 && !from_forward && (dq || pairs)) {                                                \
             hipLaunchKernelGGL((batch_grad_q_pairs_kernel<T, DD>), batch_grid(p.total_q), dim3(kBatchBlock), 0, s,   \
                                p.recs, p.d_clouds, p.qcloud, p.rank, q, p.total_q, idx, g, dq,                       \
-                               pairs ? qs : nullptr, p.keys, p.vals);                                                \
+                               pairs ? qs : nullptr, p.keys, p.vals, p.pair_tag_shift);                              \
             TK_HIP_CHECK(hipGetLastError());                                                                         \
         }                                                                                                            \
         if (pairs) {                                                                                                 \
-            radix_sort_pairs(p.keys, p.vals, p.keys_s, p.vals_s, p.total_out, batch_key_bits(p.total_ref),           \
+            radix_sort_pairs(p.keys, p.vals, p.keys_s, p.vals_s, p.total_out, p.pair_bits,                           \
                              p.radix_ws, p.radix_ws_bytes, s);                                                       \
             hipLaunchKernelGGL(batch_zero_runs_kernel, batch_grid(p.total_ref), dim3(kBatchBlock), 0, s,             \
                                p.run_start, p.run_end, p.total_ref);                                                 \
             hipLaunchKernelGGL(batch_run_bounds_kernel, batch_grid(p.total_out), dim3(kBatchBlock), 0, s, p.keys_s,  \
-                               p.total_out, p.run_start, p.run_end);                                                 \
+                               p.total_out, p.d_clouds, p.pair_tag_shift, p.run_start, p.run_end);                   \
             hipLaunchKernelGGL((batch_grad_r_kernel<T, DD>), batch_grid(p.total_ref), dim3(kBatchBlock), 0, s,       \
                                p.recs, p.rec_cloud, p.shuffled, p.d_clouds, p.total_ref, p.run_start, p.run_end,     \
                                p.vals_s, qs, p.order, g, dr);                                                        \
