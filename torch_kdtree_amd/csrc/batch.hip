@@ -92,15 +92,17 @@ void stage_cloud_t(const Tree& t, const CloudDesc& cd, unsigned char* recs, void
 
 int box_bytes_of(int dtype, int dims) { return box_bytes_for(dtype, dims); }
 
-void enqueue_forward(BatchPlan& p, const void* q, void* dist, int32_t* idx, int64_t* orig, bool emit,
-                     hipStream_t s) {
+void enqueue_forward(BatchPlan& p, const void* q, void* dist, int32_t* idx, int64_t* orig, bool emit, const void* g,
+                     void* dq, hipStream_t s) {
     if (p.total_q == 0) return;
     if (p.dtype == kF32) {
         batch_presort_f32(p, static_cast<const float*>(q), s);
-        batch_traverse_f32(p, static_cast<const float*>(q), static_cast<float*>(dist), idx, orig, emit, s);
+        batch_traverse_f32(p, static_cast<const float*>(q), static_cast<float*>(dist), idx, orig, emit,
+                           static_cast<const float*>(g), static_cast<float*>(dq), s);
     } else {
         batch_presort_f64(p, static_cast<const double*>(q), s);
-        batch_traverse_f64(p, static_cast<const double*>(q), static_cast<double*>(dist), idx, orig, emit, s);
+        batch_traverse_f64(p, static_cast<const double*>(q), static_cast<double*>(dist), idx, orig, emit,
+                           static_cast<const double*>(g), static_cast<double*>(dq), s);
     }
 }
 
@@ -327,7 +329,7 @@ void batch_free(BatchPlan* p) noexcept {
 void batch_forward(BatchPlan& p, const void* queries, void* dist, int32_t* idx, int64_t* orig, hipStream_t s) {
     DeviceGuard guard(p.device);
     ProfScope prof("batch_forward", s);
-    enqueue_forward(p, queries, dist, idx, orig, false, s);
+    enqueue_forward(p, queries, dist, idx, orig, false, nullptr, nullptr, s);
 }
 
 void batch_backward(BatchPlan& p, const void* queries, const int32_t* idx, const void* grad_dist, void* grad_q,
@@ -359,9 +361,12 @@ void batch_capture(BatchPlan& p, const void* queries, void* dist, int32_t* idx, 
     TK_HIP_CHECK(hipStreamSynchronize(s));
     TK_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     try {
+        // a captured training step: the upstream gradient is in place before each replay, so the
+        // forward also writes dL/dq, and the backward only dL/dr
         const bool train = grad_dist && (grad_q || grad_ref);
-        enqueue_forward(p, queries, dist, idx, orig, train, cs);
-        if (train) enqueue_backward(p, queries, idx, grad_dist, grad_q, grad_ref, true, cs);
+        enqueue_forward(p, queries, dist, idx, orig, train && grad_ref, train ? grad_dist : nullptr,
+                        train ? grad_q : nullptr, cs);
+        if (train && grad_ref) enqueue_backward(p, queries, idx, grad_dist, nullptr, grad_ref, true, cs);
     } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(cs, &g);

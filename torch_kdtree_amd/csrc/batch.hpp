@@ -112,10 +112,11 @@ void batch_presort_f32(const BatchPlan& p, const float* q, hipStream_t s);
 void batch_presort_f64(const BatchPlan& p, const double* q, hipStream_t s);
 // emit: also write the training outputs (the queries at their sorted positions and the
 // (record, canonical slot) pairs) for a following batch_backward_*(from_forward = true).
+// dq (with g, the upstream gradient of the distances, already in place): also write dL/dq.
 void batch_traverse_f32(const BatchPlan& p, const float* q, float* dist, int32_t* idx, int64_t* orig, bool emit,
-                        hipStream_t s);
+                        const float* g, float* dq, hipStream_t s);
 void batch_traverse_f64(const BatchPlan& p, const double* q, double* dist, int32_t* idx, int64_t* orig, bool emit,
-                        hipStream_t s);
+                        const double* g, double* dq, hipStream_t s);
 void batch_backward_f32(const BatchPlan& p, const float* q, const int32_t* idx, const float* g, float* dq, float* dr,
                         bool from_forward, hipStream_t s);
 void batch_backward_f64(const BatchPlan& p, const double* q, const int32_t* idx, const double* g, double* dq,

@@ -6,8 +6,8 @@ namespace tk {
 void batch_presort_f64(const BatchPlan& p, const double* q, hipStream_t s) { batch_presort_t<double>(p, q, s); }
 
 void batch_traverse_f64(const BatchPlan& p, const double* q, double* dist, int32_t* idx, int64_t* orig, bool emit,
-                        hipStream_t s) {
-    batch_traverse_t<double>(p, q, dist, idx, orig, emit, s);
+                        const double* g, double* dq, hipStream_t s) {
+    batch_traverse_t<double>(p, q, dist, idx, orig, emit, g, dq, s);
 }
 
 void batch_backward_f64(const BatchPlan& p, const double* q, const int32_t* idx, const double* g, double* dq, double* dr,

@@ -63,7 +63,9 @@ __global__ __launch_bounds__(kTreeBlock) void knn_batch_kernel(TreeView<T, true>
                                                                 int32_t* __restrict__ out_i, int64_t* __restrict__ out_o,
                                                                 uint32_t* __restrict__ pair_keys,
                                                                 uint32_t* __restrict__ pair_vals,
-                                                                T* __restrict__ q_sorted, int tag_shift) {
+                                                                T* __restrict__ q_sorted, int tag_shift,
+                                                                const T* __restrict__ fuse_g,
+                                                                T* __restrict__ fuse_dq) {
     constexpr int CH = StageBytes<T, K, kLower>::kChunk;
     constexpr int LD = StageBytes<T, K, kLower>::kLoads;
     static_assert(LD <= kRecPad, "a load group may read at most kRecPad records past a leaf's end");
@@ -115,6 +117,29 @@ __global__ __launch_bounds__(kTreeBlock) void knn_batch_kernel(TreeView<T, true>
                 pair_vals[sb + j - pad] = (uint32_t)(sb + j - pad);
             }
         }
+    }
+    // dL/dq of a training step whose upstream gradient is already in place (a captured step):
+    // the terms of batch_grad_q_sorted_kernel, in neighbour order, without a second pass
+    if (fuse_dq) {
+        T acc[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) acc[a] = T(0);
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            if (j >= pad) {
+                const int32_t x = kb.x[j];
+                const int32_t loc = kLower ? load_i32_at(arena.inv_shuffled, cd.rec_base + x) : x - cd.rec_base;
+                const Rec<T, D> r = load_rec<T, D>(arena.recs, cd.rec_base + loc);
+                const T gij = fuse_g[ob + j - pad];
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    const T diff = q[a] - r.c[a];
+                    acc[a] = acc[a] + gij * (diff + diff);
+                }
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) fuse_dq[qi * D + a] = acc[a];
     }
 }
 
@@ -336,7 +361,7 @@ void batch_presort_t(const BatchPlan& p, const T* q, hipStream_t s) {
 
 template <typename T, int D, bool kLower>
 void batch_traverse_td(const BatchPlan& p, const T* q, T* dist, int32_t* idx, int64_t* orig, bool emit,
-                       hipStream_t s) {
+                       const T* g, T* dq, hipStream_t s) {
     uint32_t* pk = emit ? p.keys : nullptr;
     uint32_t* pv = emit ? p.vals : nullptr;
     T* qs = emit ? static_cast<T*>(p.q_sorted) : nullptr;
@@ -347,7 +372,8 @@ void batch_traverse_td(const BatchPlan& p, const T* q, T* dist, int32_t* idx, in
 #define TK_LAUNCH_K(KK)                                                                                              \
     case KK:                                                                                                         \
         hipLaunchKernelGGL((knn_batch_kernel<T, D, KK, kLower>), grid, dim3(kTreeBlock), 0, s, arena, p.d_clouds,   \
-                           p.qcloud, q, p.order, r.begin, r.count, dist, idx, orig, pk, pv, qs, p.pair_tag_shift);   \
+                           p.qcloud, q, p.order, r.begin, r.count, dist, idx, orig, pk, pv, qs, p.pair_tag_shift,    \
+                           g, dq);                                                                                   \
         break;
         switch (r.K) {
             TK_LAUNCH_K(1)
@@ -365,13 +391,13 @@ void batch_traverse_td(const BatchPlan& p, const T* q, T* dist, int32_t* idx, in
 
 template <typename T>
 void batch_traverse_t(const BatchPlan& p, const T* q, T* dist, int32_t* idx, int64_t* orig, bool emit,
-                      hipStream_t s) {
+                      const T* g, T* dq, hipStream_t s) {
 #define TK_DIMS(DD)                                                         \
     case DD:                                                                \
         if (p.tie == kTieLowerIndex)                                        \
-            batch_traverse_td<T, DD, true>(p, q, dist, idx, orig, emit, s); \
+            batch_traverse_td<T, DD, true>(p, q, dist, idx, orig, emit, g, dq, s); \
         else                                                                \
-            batch_traverse_td<T, DD, false>(p, q, dist, idx, orig, emit, s);\
+            batch_traverse_td<T, DD, false>(p, q, dist, idx, orig, emit, g, dq, s);\
         break;
     switch (p.dims) {
         TK_DIMS(1)
