@@ -274,6 +274,28 @@ def test_c_abi_build_query_cpu_and_errors():
     assert b"dimensionality" in lib.tk_last_error()
 
 
+def test_c_abi_gpu_build_validates_before_touching_a_device():
+    """tk_tree_build_gpu / tk_tree_build_many_gpu reject bad arguments with the host build's
+    messages before any device call (so these run without a GPU)."""
+    lib = ctypes.CDLL(os.path.join(ROOT, "torch_kdtree_amd", "libtorch_kdtree_amd.so"))
+    lib.tk_last_error.restype = ctypes.c_char_p
+    vp = ctypes.c_void_p
+    out = vp()
+    cases = [((10, 5, 0, 2), b"dimensionality"), ((10, 3, 7, 2), b"dtype"), ((10, 3, 0, 0), b"levels"),
+             ((10, 3, 0, 25), b"levels"), ((10, 3, 0, 6), b"Ran out of points")]
+    for (n, d, dt, lv), msg in cases:
+        rc = lib.tk_tree_build_gpu(vp(1), ctypes.c_int64(n), d, dt, lv, -1, None, ctypes.byref(out))
+        assert rc != 0 and msg in lib.tk_last_error(), (n, d, dt, lv, lib.tk_last_error())
+        assert not out.value
+    outs = (vp * 2)()
+    ns = (ctypes.c_int64 * 2)(100, 3)
+    lvs = (ctypes.c_int32 * 2)(3, 3)
+    rc = lib.tk_tree_build_many_gpu(vp(1), ns, 2, 3, 0, lvs, -1, None, outs)
+    assert rc != 0 and b"Ran out of points" in lib.tk_last_error()
+    assert not outs[0] and not outs[1]
+    assert lib.tk_tree_build_many_gpu(vp(1), ns, 0, 3, 0, lvs, -1, None, outs) != 0
+
+
 # --- tree serialization --------------------------------------------------------------------------
 
 @pytest.mark.parametrize("dtype,d", [(np.float32, 3), (np.float64, 2), (np.float32, 8), (np.float64, 1)])
