@@ -66,9 +66,9 @@ __global__ __launch_bounds__(kTreeBlock) void knn_batch_kernel(TreeView<T, true>
                                                                 T* __restrict__ q_sorted, int tag_shift,
                                                                 const T* __restrict__ fuse_g,
                                                                 T* __restrict__ fuse_dq) {
-    constexpr int CH = StageBytes<T, K, kLower>::kChunk;
-    constexpr int LD = StageBytes<T, K, kLower>::kLoads;
-    static_assert(LD <= kRecPad, "a load group may read at most kRecPad records past a leaf's end");
+    constexpr int CH = StageBytes<T, D, K, kLower>::kChunk;
+    constexpr int LD = StageBytes<T, D, K, kLower>::kLoads;
+    static_assert(CH <= kRecPad, "a chunk may read at most kRecPad records past a leaf's end");
     __shared__ T stage_d[CH * kTreeBlock];
     __shared__ int32_t stage_o[kLower ? CH * kTreeBlock : 1];
     const int64_t t = (int64_t)blockIdx.x * kTreeBlock + threadIdx.x;

@@ -53,7 +53,7 @@ struct TreeView {
 #define TK_TREE_BLOCK 128
 #endif
 #ifndef TK_STAGE_CHUNK_F32
-#define TK_STAGE_CHUNK_F32 4
+#define TK_STAGE_CHUNK_F32 8
 #endif
 #ifndef TK_SCAN_LOADS_F32
 #define TK_SCAN_LOADS_F32 4
@@ -225,13 +225,20 @@ __device__ __forceinline__ void scan_leaf_staged(const TreeView<T, kB>& tv, int3
         const T w = kb.d[K - 1];
         const int32_t wo = kLower ? kb.x[K - 1] : 0;
         uint32_t mask = 0;
-        // pass 1, LD records in flight at a time
+        // pass 1, LD records in flight at a time; every group is read (records past the leaf's
+        // end are padding or the next leaf's, and are masked out), so the loads of a chunk are
+        // issued without a data-dependent exit
+#pragma unroll
         for (int g = 0; g < CH; g += LD) {
-            if (g >= n) break;
+            Rec<T, D> rg[LD];
+#pragma unroll
+            for (int jj = 0; jj < LD; ++jj) rg[jj] = load_rec<T, D>(tv.recs, c0 + g + jj);
+            // keeps the group's loads issued together ahead of their use
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int jj = 0; jj < LD; ++jj) {
                 const int j = g + jj;
-                const Rec<T, D> r = load_rec<T, D>(tv.recs, c0 + j);
+                const Rec<T, D>& r = rg[jj];
                 const T dd = sq_dist<T, D>(q, r.c);
                 const bool acc = kLower ? (dd < w || (dd == w && r.o < wo)) : (dd < w);
                 mask |= (acc && j < n) ? (1u << j) : 0u;
@@ -474,9 +481,12 @@ __device__ __forceinline__ void load_query(const T* __restrict__ queries, int64_
     for (int i = 0; i < D; ++i) q[i] = queries[qi * D + i];
 }
 
-template <typename T, int K, bool kLower>
+// Points per staged chunk of the leaf scan (measured on MI355X): TK_STAGE_CHUNK_F32 (8) for float
+// searches with d <= 3 and lists of 8 or more, 4 for other float searches (d = 8, or k <= 4,
+// where the larger chunk's extra stale-worst pops cost more than its fewer exits), 16 for double.
+template <typename T, int D, int K, bool kLower>
 struct StageBytes {
-    static constexpr int kChunk = sizeof(T) == 4 ? TK_STAGE_CHUNK_F32 : 16;
+    static constexpr int kChunk = sizeof(T) == 4 ? ((D <= 3 && K >= 8) ? TK_STAGE_CHUNK_F32 : 4) : 16;
     static constexpr int kLoads = sizeof(T) == 4 ? TK_SCAN_LOADS_F32 : 4;
     static constexpr int kBytes = kChunk * kTreeBlock * ((int)sizeof(T) + (kLower ? 4 : 0));
 };
@@ -490,9 +500,9 @@ __global__ __launch_bounds__(kTreeBlock) TK_TREE_OCCUPANCY void knn_tree_kernel(
                                                                int64_t nq, const int32_t* __restrict__ order, int k,
                                                                T* __restrict__ out_d, int32_t* __restrict__ out_i,
                                                                int64_t* __restrict__ out_o) {
-    constexpr int CH = StageBytes<T, K, kLower>::kChunk;
-    constexpr int LD = StageBytes<T, K, kLower>::kLoads;
-    static_assert(LD <= kRecPad, "a load group may read at most kRecPad records past a leaf's end");
+    constexpr int CH = StageBytes<T, D, K, kLower>::kChunk;
+    constexpr int LD = StageBytes<T, D, K, kLower>::kLoads;
+    static_assert(CH <= kRecPad, "a chunk may read at most kRecPad records past a leaf's end");
     __shared__ T stage_d[CH * kTreeBlock];
     __shared__ int32_t stage_o[kLower ? CH * kTreeBlock : 1];
     const int64_t t = (int64_t)blockIdx.x * kTreeBlock + threadIdx.x;
