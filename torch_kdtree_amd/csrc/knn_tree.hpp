@@ -58,6 +58,15 @@ struct TreeView {
 #ifndef TK_SCAN_LOADS_F32
 #define TK_SCAN_LOADS_F32 4
 #endif
+#ifndef TK_STAGE_CHUNK_F64
+#define TK_STAGE_CHUNK_F64 16
+#endif
+#ifndef TK_SCAN_LOADS_F64
+#define TK_SCAN_LOADS_F64 4
+#endif
+#ifndef TK_STAGE_CHUNK_WIDE
+#define TK_STAGE_CHUNK_WIDE 4
+#endif
 // Occupancy target of the register-list traversal kernel (waves per SIMD); 0 leaves it to the
 // compiler.
 #ifndef TK_WAVES_PER_EU
@@ -482,12 +491,15 @@ __device__ __forceinline__ void load_query(const T* __restrict__ queries, int64_
 }
 
 // Points per staged chunk of the leaf scan (measured on MI355X): TK_STAGE_CHUNK_F32 (8) for float
-// searches with d <= 3 and lists of 8 or more, 4 for other float searches (d = 8, or k <= 4,
-// where the larger chunk's extra stale-worst pops cost more than its fewer exits), 16 for double.
+// searches with d <= 3 and lists of 8 or more, TK_STAGE_CHUNK_WIDE (4) for other float searches
+// (d = 8, or k <= 4, where the larger chunk's extra stale-worst pops cost more than its fewer
+// exits), TK_STAGE_CHUNK_F64 for double.
 template <typename T, int D, int K, bool kLower>
 struct StageBytes {
-    static constexpr int kChunk = sizeof(T) == 4 ? ((D <= 3 && K >= 8) ? TK_STAGE_CHUNK_F32 : 4) : 16;
-    static constexpr int kLoads = sizeof(T) == 4 ? TK_SCAN_LOADS_F32 : 4;
+    static constexpr int kChunk =
+        sizeof(T) == 4 ? ((D <= 3 && K >= 8) ? TK_STAGE_CHUNK_F32 : TK_STAGE_CHUNK_WIDE) : TK_STAGE_CHUNK_F64;
+    static constexpr int kLoads = sizeof(T) == 4 ? (TK_SCAN_LOADS_F32 < kChunk ? TK_SCAN_LOADS_F32 : kChunk)
+                                                 : (TK_SCAN_LOADS_F64 < kChunk ? TK_SCAN_LOADS_F64 : kChunk);
     static constexpr int kBytes = kChunk * kTreeBlock * ((int)sizeof(T) + (kLower ? 4 : 0));
 };
 
