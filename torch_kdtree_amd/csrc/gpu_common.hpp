@@ -124,8 +124,14 @@ struct BoxLayout {
 // squared and summed in axis order like a distance.  By the monotonicity of rounding it never
 // exceeds the computed distance of a point in the box.  An empty box (lo = +inf, hi = -inf)
 // gives +inf.
+// The coordinates of one box, loaded (lo then hi).
 template <typename T, int D>
-__device__ __forceinline__ T aabb_bound(const unsigned char* __restrict__ boxes, int32_t node, const Pt<T, D>& q) {
+struct BoxVals {
+    T lo[D], hi[D];
+};
+
+template <typename T, int D>
+__device__ __forceinline__ BoxVals<T, D> load_box(const unsigned char* __restrict__ boxes, int32_t node) {
     using L = BoxLayout<T, D>;
     uint32_t w[L::kWords];
     if constexpr (L::kBytes % 16 == 0) {
@@ -143,16 +149,26 @@ __device__ __forceinline__ T aabb_bound(const unsigned char* __restrict__ boxes,
         w[0] = v.x;
         w[1] = v.y;
     }
-    T lo[D], hi[D];
-    __builtin_memcpy(lo, w, sizeof(T) * D);
-    __builtin_memcpy(hi, reinterpret_cast<const unsigned char*>(w) + sizeof(T) * L::kPad, sizeof(T) * D);
+    BoxVals<T, D> b;
+    __builtin_memcpy(b.lo, w, sizeof(T) * D);
+    __builtin_memcpy(b.hi, reinterpret_cast<const unsigned char*>(w) + sizeof(T) * L::kPad, sizeof(T) * D);
+    return b;
+}
+
+template <typename T, int D>
+__device__ __forceinline__ T box_dist(const BoxVals<T, D>& b, const Pt<T, D>& q) {
     T s = T(0);
 #pragma unroll
     for (int i = 0; i < D; ++i) {
-        const T v = fmax(fmax(lo[i] - q[i], q[i] - hi[i]), T(0));
+        const T v = fmax(fmax(b.lo[i] - q[i], q[i] - b.hi[i]), T(0));
         s = s + v * v;
     }
     return s;
+}
+
+template <typename T, int D>
+__device__ __forceinline__ T aabb_bound(const unsigned char* __restrict__ boxes, int32_t node, const Pt<T, D>& q) {
+    return box_dist<T, D>(load_box<T, D>(boxes, node), q);
 }
 
 template <typename T>

@@ -67,16 +67,19 @@ struct TreeView {
 #ifndef TK_STAGE_CHUNK_WIDE
 #define TK_STAGE_CHUNK_WIDE 4
 #endif
-// Occupancy target of the register-list traversal kernel (waves per SIMD); 0 leaves it to the
-// compiler.
+// Occupancy target (waves per SIMD) of the register-list traversal kernels: 5 for float searches
+// with d <= 3 and lists of at most 16 (measured on MI355X: the compiler then keeps them at 96
+// VGPRs with the climb's box prefetch, without spills), no target otherwise (d = 8 or 32-entry
+// lists would spill).  TK_WAVES_PER_EU overrides it for every kernel (0 = no target anywhere).
 #ifndef TK_WAVES_PER_EU
-#define TK_WAVES_PER_EU 0
+#define TK_WAVES_PER_EU -1
 #endif
-#if TK_WAVES_PER_EU > 0
-#define TK_TREE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(TK_WAVES_PER_EU)))
-#else
-#define TK_TREE_OCCUPANCY
-#endif
+template <typename T, int D, int K, bool kLower>
+struct TreeOccupancy {
+    static constexpr int kWaves = TK_WAVES_PER_EU >= 0 ? TK_WAVES_PER_EU
+                                  : ((sizeof(T) == 4 && D <= 3 && K <= 16 && !kLower) ? 5 : 0);
+};
+#define TK_TREE_OCCUPANCY(T, D, K, L) __attribute__((amdgpu_waves_per_eu(TreeOccupancy<T, D, K, L>::kWaves > 0 ? TreeOccupancy<T, D, K, L>::kWaves : 1)))
 constexpr int kTreeBlock = TK_TREE_BLOCK;
 constexpr int kTreeBlockStride = kTreeBlock;
 
@@ -355,22 +358,51 @@ __device__ __forceinline__ bool lane_step(const TreeView<T, kB>& tv, const Pt<T,
             // The far child of a level entered through its near child is the sibling of the path's
             // child one level down, so these tests need neither the split nor the axis.
             uint32_t near_lv = ((1u << l) - 1u) & ~far_mask;
-            while (near_lv) {
-                const int j = 31 - __builtin_clz(near_lv);
-                const int32_t c = ((node + 1) >> (level - 1 - j)) - 1;
-                const int32_t fc = ((c - 1) ^ 1) + 1;
-                const T b = aabb_bound<T, D>(tv.boxes, tv.box_of(fc), q);
+            if constexpr (kB) {
+                while (near_lv) {
+                    const int j = 31 - __builtin_clz(near_lv);
+                    const int32_t c = ((node + 1) >> (level - 1 - j)) - 1;
+                    const int32_t fc = ((c - 1) ^ 1) + 1;
+                    const T b = aabb_bound<T, D>(tv.boxes, tv.box_of(fc), q);
+                    if (kLower ? b <= kb.worst() : b < kb.worst()) {
+                        ax_set(pp, axis_of_level<D>(j), tv.split_at((c - 1) >> 1));
+                        far_mask = (far_mask & ((1u << j) - 1u)) | (1u << j);
+                        node = fc;
+                        level = j + 1;
+                        lane_descend<T, D>(tv, q, node, level);
+                        return false;
+                    }
+                    near_lv ^= 1u << j;
+                }
+                return true;
+            }
+            // Single tree: the box of the next candidate level is loaded while the current one
+            // is tested (the batched kernels keep the plain loop: the prefetch's registers
+            // would make them spill).
+            if (!near_lv) return true;
+            int j = 31 - __builtin_clz(near_lv);
+            int32_t c = ((node + 1) >> (level - 1 - j)) - 1;
+            BoxVals<T, D> bx = load_box<T, D>(tv.boxes, tv.box_of(((c - 1) ^ 1) + 1));
+            for (;;) {
+                const uint32_t rest = near_lv ^ (1u << j);
+                const int j2 = rest ? 31 - __builtin_clz(rest) : j;
+                const int32_t c2 = ((node + 1) >> (level - 1 - j2)) - 1;
+                const BoxVals<T, D> bx2 = load_box<T, D>(tv.boxes, tv.box_of(((c2 - 1) ^ 1) + 1));
+                const T b = box_dist<T, D>(bx, q);
                 if (kLower ? b <= kb.worst() : b < kb.worst()) {
                     ax_set(pp, axis_of_level<D>(j), tv.split_at((c - 1) >> 1));
                     far_mask = (far_mask & ((1u << j) - 1u)) | (1u << j);
-                    node = fc;
+                    node = ((c - 1) ^ 1) + 1;
                     level = j + 1;
                     lane_descend<T, D>(tv, q, node, level);
                     return false;
                 }
-                near_lv ^= 1u << j;
+                if (!rest) return true;
+                near_lv = rest;
+                j = j2;
+                c = c2;
+                bx = bx2;
             }
-            return true;
         }
         const uint32_t below = (1u << l) - 1u;
         const uint32_t near_lv = below & ~far_mask;
@@ -508,7 +540,7 @@ struct StageBytes {
 // keeps the wavefront's control flow and loads coherent.  (Handing a finished lane the next query
 // was measured to be slower: it breaks that lockstep.)
 template <typename T, int D, int K, bool kLower>
-__global__ __launch_bounds__(kTreeBlock) TK_TREE_OCCUPANCY void knn_tree_kernel(TreeView<T> tv, const T* __restrict__ queries,
+__global__ __launch_bounds__(kTreeBlock) TK_TREE_OCCUPANCY(T, D, K, kLower) void knn_tree_kernel(TreeView<T> tv, const T* __restrict__ queries,
                                                                int64_t nq, const int32_t* __restrict__ order, int k,
                                                                T* __restrict__ out_d, int32_t* __restrict__ out_i,
                                                                int64_t* __restrict__ out_o) {
