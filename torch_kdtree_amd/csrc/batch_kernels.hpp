@@ -54,7 +54,7 @@ __device__ __forceinline__ uint32_t pair_key(int32_t cloud, int32_t local, int32
 
 // One lane per query of the k bucket K; `order` holds the presorted query rows.
 template <typename T, int D, int K, bool kLower>
-__global__ __launch_bounds__(kTreeBlock) void knn_batch_kernel(TreeView<T, true> arena,
+__global__ __launch_bounds__(kTreeBlock) TK_BATCH_OCCUPANCY(T, D, K, kLower) void knn_batch_kernel(TreeView<T, true> arena,
                                                                 const CloudDesc* __restrict__ clouds,
                                                                 const int32_t* __restrict__ qcloud,
                                                                 const T* __restrict__ queries,

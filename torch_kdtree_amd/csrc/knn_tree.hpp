@@ -80,6 +80,20 @@ struct TreeOccupancy {
                                   : ((sizeof(T) == 4 && D <= 3 && K <= 16 && !kLower) ? 5 : 0);
 };
 #define TK_TREE_OCCUPANCY(T, D, K, L) __attribute__((amdgpu_waves_per_eu(TreeOccupancy<T, D, K, L>::kWaves > 0 ? TreeOccupancy<T, D, K, L>::kWaves : 1)))
+// The same for the batched search kernels (more live state per lane: the cloud's offsets).
+#ifndef TK_BATCH_WAVES_K16
+#define TK_BATCH_WAVES_K16 4
+#endif
+#ifndef TK_BATCH_WAVES_K8
+#define TK_BATCH_WAVES_K8 5
+#endif
+template <typename T, int D, int K, bool kLower>
+struct BatchOccupancy {
+    static constexpr int kWaves = (sizeof(T) == 4 && D <= 3 && !kLower)
+                                      ? (K == 16 ? TK_BATCH_WAVES_K16 : K == 8 ? TK_BATCH_WAVES_K8 : 0)
+                                      : 0;
+};
+#define TK_BATCH_OCCUPANCY(T, D, K, L) __attribute__((amdgpu_waves_per_eu(BatchOccupancy<T, D, K, L>::kWaves > 0 ? BatchOccupancy<T, D, K, L>::kWaves : 1)))
 constexpr int kTreeBlock = TK_TREE_BLOCK;
 constexpr int kTreeBlockStride = kTreeBlock;
 
