@@ -64,6 +64,10 @@ struct TreeView {
 #ifndef TK_SCAN_LOADS_F64
 #define TK_SCAN_LOADS_F64 4
 #endif
+// Largest list held in registers (larger k use the LDS-list kernel).
+#ifndef TK_REG_LIST_MAX_K
+#define TK_REG_LIST_MAX_K 32
+#endif
 #ifndef TK_STAGE_CHUNK_WIDE
 #define TK_STAGE_CHUNK_WIDE 4
 #endif

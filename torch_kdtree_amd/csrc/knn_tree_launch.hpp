@@ -20,6 +20,7 @@ void launch_tree_td(const TreeView<T>& tv, const T* q, int64_t nq, const int32_t
     const dim3 block(kTreeBlock);
     int kb = 1;
     while (kb < k) kb <<= 1;
+    if (kb > TK_REG_LIST_MAX_K) kb = 0;  // the LDS-list kernel
 #define TK_LAUNCH_K(KK)                                                                                        \
     case KK:                                                                                                   \
         hipLaunchKernelGGL((knn_tree_kernel<T, D, KK, kLower>), grid, block, 0, s, tv, q, nq, order, k, dist, \
