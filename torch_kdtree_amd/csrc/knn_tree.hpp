@@ -72,16 +72,21 @@ struct TreeView {
 #define TK_STAGE_CHUNK_WIDE 4
 #endif
 // Occupancy target (waves per SIMD) of the register-list traversal kernels: 5 for float searches
-// with d <= 3 and lists of at most 16 (measured on MI355X: the compiler then keeps them at 96
-// VGPRs with the climb's box prefetch, without spills), no target otherwise (d = 8 or 32-entry
-// lists would spill).  TK_WAVES_PER_EU overrides it for every kernel (0 = no target anywhere).
+// with d <= 3 and lists of at most 16 (measured on MI355X: the compiler then keeps the reference
+// policy's kernel at 96 VGPRs with the climb's box prefetch, without spills; the lower-index
+// kernel spills a few registers and is still 7% faster, TK_LOWER_WAVES), no target otherwise
+// (d = 8 or 32-entry lists would spill heavily).  TK_WAVES_PER_EU overrides it for every kernel
+// (0 = no target anywhere).
 #ifndef TK_WAVES_PER_EU
 #define TK_WAVES_PER_EU -1
+#endif
+#ifndef TK_LOWER_WAVES
+#define TK_LOWER_WAVES 1
 #endif
 template <typename T, int D, int K, bool kLower>
 struct TreeOccupancy {
     static constexpr int kWaves = TK_WAVES_PER_EU >= 0 ? TK_WAVES_PER_EU
-                                  : ((sizeof(T) == 4 && D <= 3 && K <= 16 && !kLower) ? 5 : 0);
+                                  : ((sizeof(T) == 4 && D <= 3 && K <= 16 && (!kLower || TK_LOWER_WAVES)) ? 5 : 0);
 };
 #define TK_TREE_OCCUPANCY(T, D, K, L) __attribute__((amdgpu_waves_per_eu(TreeOccupancy<T, D, K, L>::kWaves > 0 ? TreeOccupancy<T, D, K, L>::kWaves : 1)))
 // The same for the batched search kernels (more live state per lane: the cloud's offsets).
