@@ -317,6 +317,7 @@ void batch_free(BatchPlan* p) noexcept {
     dfree(p->run_start);
     dfree(p->run_end);
     dfree(p->q_sorted);
+    dfree(p->g_sorted);
     dfree(p->keys);
     dfree(p->vals);
     dfree(p->keys_s);
@@ -359,6 +360,12 @@ void batch_capture(BatchPlan& p, const void* queries, void* dist, int32_t* idx, 
         ~StreamGuard() { (void)hipStreamDestroy(st); }
     } sg{cs};
     TK_HIP_CHECK(hipStreamSynchronize(s));
+    // a step with both gradients stores the upstream gradient by canonical slot in its forward
+    // (allocated here, outside the capture)
+    const bool both = grad_dist && grad_q && grad_ref && p.total_out > 0;
+    if (both && !p.g_sorted)
+        dmalloc(reinterpret_cast<unsigned char**>(&p.g_sorted), dtype_size(p.dtype) * (size_t)p.total_out);
+    p.g_sorted_valid = both;
     TK_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     try {
         // a captured training step: the upstream gradient is in place before each replay, so the

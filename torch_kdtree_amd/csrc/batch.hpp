@@ -78,6 +78,8 @@ struct BatchPlan {
     uint32_t* run_start = nullptr;  // [total_ref] runs of each record in the sorted pairs
     uint32_t* run_end = nullptr;
     void* q_sorted = nullptr;   // [total_q * dims] queries in sorted order
+    void* g_sorted = nullptr;   // [total_out] upstream gradient by canonical slot (captured training steps)
+    bool g_sorted_valid = false;  // the captured graph's forward writes g_sorted before its backward
     uint32_t* keys = nullptr;   // [total_out] pairs (record, canonical slot)
     uint32_t* vals = nullptr;
     uint32_t* keys_s = nullptr;

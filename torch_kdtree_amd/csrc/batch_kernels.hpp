@@ -65,7 +65,7 @@ __global__ __launch_bounds__(kTreeBlock) TK_BATCH_OCCUPANCY(T, D, K, kLower) voi
                                                                 uint32_t* __restrict__ pair_vals,
                                                                 T* __restrict__ q_sorted, int tag_shift,
                                                                 const T* __restrict__ fuse_g,
-                                                                T* __restrict__ fuse_dq) {
+                                                                T* __restrict__ fuse_dq, T* __restrict__ g_sorted) {
     constexpr int CH = StageBytes<T, D, K, kLower>::kChunk;
     constexpr int LD = StageBytes<T, D, K, kLower>::kLoads;
     static_assert(CH <= kRecPad, "a chunk may read at most kRecPad records past a leaf's end");
@@ -131,6 +131,7 @@ __global__ __launch_bounds__(kTreeBlock) TK_BATCH_OCCUPANCY(T, D, K, kLower) voi
                 const int32_t loc = kLower ? load_i32_at(arena.inv_shuffled, cd.rec_base + x) : x - cd.rec_base;
                 const Rec<T, D> r = load_rec<T, D>(arena.recs, cd.rec_base + loc);
                 const T gij = fuse_g[ob + j - pad];
+                if (g_sorted) g_sorted[sb + j - pad] = gij;
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
                     const T diff = q[a] - r.c[a];
@@ -271,8 +272,9 @@ __global__ __launch_bounds__(kBatchBlock) void batch_run_bounds_kernel(const uin
 }
 
 // dL/dr, one lane per record: its run of sorted pairs (canonical order), written to the record's
-// original row.  A slot's upstream gradient is read from the caller's layout through the order.
-template <typename T, int D>
+// original row.  A slot's upstream gradient is read from the caller's layout through the order, or,
+// with kSlots, from g_sorted, where the captured step's forward stored it by canonical slot.
+template <typename T, int D, bool kSlots>
 __global__ __launch_bounds__(kBatchBlock) void batch_grad_r_kernel(const unsigned char* __restrict__ recs,
                                                                     const int32_t* __restrict__ rec_cloud,
                                                                     const int32_t* __restrict__ shuffled,
@@ -284,6 +286,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_grad_r_kernel(const unsigne
                                                                     const T* __restrict__ q_sorted,
                                                                     const int32_t* __restrict__ order,
                                                                     const T* __restrict__ g, T* __restrict__ dr) {
+    // g: the caller's upstream gradient, or with kSlots the gradient by canonical slot
     const int64_t s = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
     if (s >= total_ref) return;
     const CloudDesc cd = clouds[rec_cloud[s]];
@@ -295,21 +298,28 @@ __global__ __launch_bounds__(kBatchBlock) void batch_grad_r_kernel(const unsigne
     // loads four pairs ahead; the terms are still subtracted in run order
     for (uint32_t e0u = e0; e0u < e1; e0u += kBwUnroll) {
         int64_t t[kBwUnroll], jj[kBwUnroll];
+        uint32_t slot[kBwUnroll];
 #pragma unroll
         for (int u = 0; u < kBwUnroll; ++u) {
             const bool ok = e0u + u < e1;
-            const int64_t off = ok ? (int64_t)vals[e0u + u] - cd.sorted_out_base : 0;
-            t[u] = cd.sorted_q_base + off / cd.k;
-            jj[u] = off % cd.k;
+            slot[u] = ok ? vals[e0u + u] : (uint32_t)cd.sorted_out_base;
+            const uint32_t off = (uint32_t)((int64_t)slot[u] - cd.sorted_out_base);
+            t[u] = cd.sorted_q_base + off / (uint32_t)cd.k;
+            jj[u] = off % (uint32_t)cd.k;
         }
         int64_t i[kBwUnroll];
+        if constexpr (!kSlots) {
 #pragma unroll
-        for (int u = 0; u < kBwUnroll; ++u) i[u] = e0u + u < e1 ? (int64_t)order[t[u]] : cd.q_base;
+            for (int u = 0; u < kBwUnroll; ++u) i[u] = e0u + u < e1 ? (int64_t)order[t[u]] : cd.q_base;
+        }
         T gg[kBwUnroll], qq[kBwUnroll][D];
 #pragma unroll
         for (int u = 0; u < kBwUnroll; ++u) {
             const bool ok = e0u + u < e1;
-            gg[u] = ok ? g[cd.out_base + (i[u] - cd.q_base) * cd.k + jj[u]] : T(0);
+            if constexpr (kSlots)
+                gg[u] = ok ? g[slot[u]] : T(0);
+            else
+                gg[u] = ok ? g[cd.out_base + (i[u] - cd.q_base) * cd.k + jj[u]] : T(0);
 #pragma unroll
             for (int a = 0; a < D; ++a) qq[u][a] = ok ? q_sorted[t[u] * D + a] : T(0);
         }
@@ -362,6 +372,8 @@ void batch_presort_t(const BatchPlan& p, const T* q, hipStream_t s) {
 template <typename T, int D, bool kLower>
 void batch_traverse_td(const BatchPlan& p, const T* q, T* dist, int32_t* idx, int64_t* orig, bool emit,
                        const T* g, T* dq, hipStream_t s) {
+    // the captured training step also stores the upstream gradient by canonical slot
+    T* gs = (emit && g && dq && p.g_sorted) ? static_cast<T*>(p.g_sorted) : nullptr;
     uint32_t* pk = emit ? p.keys : nullptr;
     uint32_t* pv = emit ? p.vals : nullptr;
     T* qs = emit ? static_cast<T*>(p.q_sorted) : nullptr;
@@ -373,7 +385,7 @@ void batch_traverse_td(const BatchPlan& p, const T* q, T* dist, int32_t* idx, in
     case KK:                                                                                                         \
         hipLaunchKernelGGL((knn_batch_kernel<T, D, KK, kLower>), grid, dim3(kTreeBlock), 0, s, arena, p.d_clouds,   \
                            p.qcloud, q, p.order, r.begin, r.count, dist, idx, orig, pk, pv, qs, p.pair_tag_shift,    \
-                           g, dq);                                                                                   \
+                           g, dq, gs);                                                                               \
         break;
         switch (r.K) {
             TK_LAUNCH_K(1)
@@ -447,9 +459,14 @@ void batch_backward_t(const BatchPlan& p, const T* q, const int32_t* idx, const 
                                p.run_start, p.run_end, p.total_ref);                                                 \
             hipLaunchKernelGGL(batch_run_bounds_kernel, batch_grid(p.total_out), dim3(kBatchBlock), 0, s, p.keys_s,  \
                                p.total_out, p.d_clouds, p.pair_tag_shift, p.run_start, p.run_end);                   \
-            hipLaunchKernelGGL((batch_grad_r_kernel<T, DD>), batch_grid(p.total_ref), dim3(kBatchBlock), 0, s,       \
-                               p.recs, p.rec_cloud, p.shuffled, p.d_clouds, p.total_ref, p.run_start, p.run_end,     \
-                               p.vals_s, qs, p.order, g, dr);                                                        \
+            if (from_forward && p.g_sorted && p.g_sorted_valid)                                                      \
+                hipLaunchKernelGGL((batch_grad_r_kernel<T, DD, true>), batch_grid(p.total_ref), dim3(kBatchBlock), 0, \
+                                   s, p.recs, p.rec_cloud, p.shuffled, p.d_clouds, p.total_ref, p.run_start,         \
+                                   p.run_end, p.vals_s, qs, p.order, static_cast<const T*>(p.g_sorted), dr);         \
+            else                                                                                                     \
+                hipLaunchKernelGGL((batch_grad_r_kernel<T, DD, false>), batch_grid(p.total_ref), dim3(kBatchBlock),  \
+                                   0, s, p.recs, p.rec_cloud, p.shuffled, p.d_clouds, p.total_ref, p.run_start,      \
+                                   p.run_end, p.vals_s, qs, p.order, g, dr);                                         \
             TK_HIP_CHECK(hipGetLastError());                                                                         \
         }                                                                                                            \
         break;
