@@ -102,11 +102,20 @@ struct TreeOccupancy {
 #ifndef TK_BATCH_WAVES_K8
 #define TK_BATCH_WAVES_K8 5
 #endif
+// k buckets 4 and 1 (67-75 VGPRs, 6 waves): 7- and 8-wave targets measured within noise.
+#ifndef TK_BATCH_WAVES_K4
+#define TK_BATCH_WAVES_K4 0
+#endif
+#ifndef TK_BATCH_WAVES_K1
+#define TK_BATCH_WAVES_K1 0
+#endif
 template <typename T, int D, int K, bool kLower>
 struct BatchOccupancy {
-    static constexpr int kWaves = (sizeof(T) == 4 && D <= 3 && !kLower)
-                                      ? (K == 16 ? TK_BATCH_WAVES_K16 : K == 8 ? TK_BATCH_WAVES_K8 : 0)
-                                      : 0;
+    static constexpr int kWaves =
+        (sizeof(T) == 4 && D <= 3 && !kLower)
+            ? (K == 16 ? TK_BATCH_WAVES_K16
+                       : K == 8 ? TK_BATCH_WAVES_K8 : K == 4 ? TK_BATCH_WAVES_K4 : K == 1 ? TK_BATCH_WAVES_K1 : 0)
+            : 0;
 };
 #define TK_BATCH_OCCUPANCY(T, D, K, L) __attribute__((amdgpu_waves_per_eu(BatchOccupancy<T, D, K, L>::kWaves > 0 ? BatchOccupancy<T, D, K, L>::kWaves : 1)))
 constexpr int kTreeBlock = TK_TREE_BLOCK;
