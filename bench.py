@@ -132,6 +132,25 @@ def _extra_default_levels(torch, tk, dev, ref, q, k, res_d, res_i, steps):
     return {"levels": tree.levels, "queries_per_s": q.shape[0] * steps / el, "ms_per_step": el * 1e3 / steps}
 
 
+def _extra_lower_index(torch, tree, q, k, res_d, res_i, steps):
+    """The plain_gpu query on the same tree with exact distance ties ordered by the lower original
+    index (tie_break="lower_index")."""
+    policy = tree.tie_break
+    tree.tie_break = "lower_index"
+    try:
+        tree.query(q, nr_nns_searches=k, result_dists=res_d, result_idx=res_i)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            tree.query(q, nr_nns_searches=k, result_dists=res_d, result_idx=res_i)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+    finally:
+        tree.tie_break = policy
+    return {"tie_break": "lower_index", "levels": tree.levels, "queries_per_s": q.shape[0] * steps / el,
+            "ms_per_step": el * 1e3 / steps}
+
+
 def _extra_stress(np, torch, torch_knn, tk, workloads, dev, reps):
     """Higher-dimension and fp64 stress: d=8 fwd + bwd at k=32, and the d=3 fp64 forward at k=16."""
     out = {}
@@ -297,7 +316,8 @@ def _parse():
                     help="run one secondary measurement alone (for profiling) and print its JSON; 'all' = the "
                          "benchmark line")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the secondary measurements (brute force, fwd+bwd, reference default depth)")
+                    help="skip the secondary measurements (brute force, lower-index policy, reference default depth, "
+                         "fwd+bwd, d=8 and fp64 stress, batching, GPU tree build)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before the timed steps, run one device-to-device copy of this many MiB (a kernel of known "
                          "HBM traffic, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -416,6 +436,8 @@ def main():
         extras = {}
         log("secondary: brute force")
         extras["bruteforce"] = _extra_bruteforce(np, torch, torch_knn, dev, w)
+        log("secondary: plain_gpu with the lower-index tie policy")
+        extras["lower_index_policy"] = _extra_lower_index(torch, tree, q, k, res_d, res_i, 3)
         log("secondary: plain_gpu at the reference's default depth")
         extras["reference_default_levels"] = _extra_default_levels(torch, tk, dev, ref, q, k, res_d, res_i, 3)
         del res_d, res_i, q
