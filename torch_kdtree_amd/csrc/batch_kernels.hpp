@@ -107,9 +107,9 @@ __global__ __launch_bounds__(kTreeBlock) TK_BATCH_OCCUPANCY(T, D, K, kLower) voi
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         if (j >= pad) {
-            const int32_t x = kb.x[j];
+            const int32_t x = kb.key(j);
             const int32_t loc = kLower ? load_i32_at(arena.inv_shuffled, cd.rec_base + x) : x - cd.rec_base;
-            out_d[ob + j - pad] = kb.d[j];
+            out_d[ob + j - pad] = kb.dist(j);
             out_i[ob + j - pad] = loc;
             if (out_o) out_o[ob + j - pad] = kLower ? x : load_i32_at(arena.shuffled, x);
             if (pair_keys) {
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kTreeBlock) TK_BATCH_OCCUPANCY(T, D, K, kLower) voi
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             if (j >= pad) {
-                const int32_t x = kb.x[j];
+                const int32_t x = kb.key(j);
                 const int32_t loc = kLower ? load_i32_at(arena.inv_shuffled, cd.rec_base + x) : x - cd.rec_base;
                 const Rec<T, D> r = load_rec<T, D>(arena.recs, cd.rec_base + loc);
                 const T gij = fuse_g[ob + j - pad];

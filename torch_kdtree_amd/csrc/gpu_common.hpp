@@ -203,12 +203,27 @@ struct KBest {
     }
 
     __device__ __forceinline__ T worst() const { return d[K - 1]; }
+    __device__ __forceinline__ T dist(int j) const { return d[j]; }
+    __device__ __forceinline__ int32_t key(int j) const { return x[j]; }
 
     __device__ __forceinline__ bool accepts(T dist, int32_t key) const {
         if constexpr (kLower)
             return dist < d[K - 1] || (dist == d[K - 1] && key < x[K - 1]);
         else
             return dist < d[K - 1];
+    }
+
+    // The worst entry as of now, for acceptance tests against a stale list.
+    struct Stale {
+        T d;
+        int32_t x;
+    };
+    __device__ __forceinline__ Stale stale() const { return Stale{d[K - 1], kLower ? x[K - 1] : 0}; }
+    static __device__ __forceinline__ bool accepts_stale(const Stale& w, T dist, int32_t key) {
+        if constexpr (kLower)
+            return dist < w.d || (dist == w.d && key < w.x);
+        else
+            return dist < w.d;
     }
 
     __device__ __forceinline__ bool after(int j, T dist, int32_t key) const {
@@ -241,6 +256,49 @@ struct KBest {
             return __builtin_amdgcn_fmed3f(a, b, c);
         else
             return fmax(fmin(a, b), fmin(fmax(a, b), c));
+    }
+};
+
+// The lower-index list for float: an entry is one 64-bit key, the distance's bits above the
+// original index.  Distances are never negative, so their bits order like the distances (+inf
+// included; a NaN orders after everything and is never accepted), and one unsigned compare orders
+// (distance, index) pairs lexicographically: an insertion takes one compare and two 64-bit selects
+// per slot.  Padding slots hold 0, which nothing orders before; empty slots hold (+inf, INT_MAX).
+template <int K>
+struct KBest<float, K, true> {
+    uint64_t k[K];
+
+    static __device__ __forceinline__ uint64_t pack(float d, int32_t key) {
+        return ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)key;
+    }
+
+    __device__ __forceinline__ void init(int kk) {
+        const int pad = K - kk;
+#pragma unroll
+        for (int j = 0; j < K; ++j) k[j] = j < pad ? uint64_t(0) : pack(__builtin_huge_valf(), 0x7fffffff);
+    }
+
+    __device__ __forceinline__ float worst() const { return __uint_as_float((uint32_t)(k[K - 1] >> 32)); }
+    __device__ __forceinline__ float dist(int j) const { return __uint_as_float((uint32_t)(k[j] >> 32)); }
+    __device__ __forceinline__ int32_t key(int j) const { return (int32_t)(uint32_t)k[j]; }
+
+    __device__ __forceinline__ bool accepts(float dist, int32_t key) const { return pack(dist, key) < k[K - 1]; }
+
+    using Stale = uint64_t;
+    __device__ __forceinline__ Stale stale() const { return k[K - 1]; }
+    static __device__ __forceinline__ bool accepts_stale(Stale w, float dist, int32_t key) { return pack(dist, key) < w; }
+
+    __device__ __forceinline__ void insert(float dist, int32_t key) {
+        const uint64_t c = pack(dist, key);
+        bool here_next = k[K - 1] > c;
+#pragma unroll
+        for (int j = K - 1; j > 0; --j) {
+            const bool shift = k[j - 1] > c;
+            const bool here = here_next;
+            here_next = shift;
+            k[j] = shift ? k[j - 1] : (here ? c : k[j]);
+        }
+        k[0] = here_next ? c : k[0];
     }
 };
 

@@ -272,8 +272,7 @@ __device__ __forceinline__ void scan_leaf_staged(const TreeView<T, kB>& tv, int3
     const int32_t end = tv.leaf_end(leaf);
     for (int32_t c0 = beg; c0 < end; c0 += CH) {
         const int n = min(CH, end - c0);
-        const T w = kb.d[K - 1];
-        const int32_t wo = kLower ? kb.x[K - 1] : 0;
+        const auto w = kb.stale();
         uint32_t mask = 0;
         // pass 1, LD records in flight at a time; every group is read (records past the leaf's
         // end are padding or the next leaf's, and are masked out), so the loads of a chunk are
@@ -290,7 +289,7 @@ __device__ __forceinline__ void scan_leaf_staged(const TreeView<T, kB>& tv, int3
                 const int j = g + jj;
                 const Rec<T, D>& r = rg[jj];
                 const T dd = sq_dist<T, D>(q, r.c);
-                const bool acc = kLower ? (dd < w || (dd == w && r.o < wo)) : (dd < w);
+                const bool acc = kb.accepts_stale(w, dd, kLower ? r.o : 0);
                 mask |= (acc && j < n) ? (1u << j) : 0u;
                 st_d[j * kTreeBlockStride] = dd;
                 if constexpr (kLower) st_o[j * kTreeBlockStride] = r.o;
@@ -604,15 +603,15 @@ __global__ __launch_bounds__(kTreeBlock) TK_TREE_OCCUPANCY(T, D, K, kLower) void
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         if (j >= pad) {
-            od[j - pad] = kb.d[j];
-            oi[j - pad] = kLower ? load_i32_at(tv.inv_shuffled, kb.x[j]) : kb.x[j];
+            od[j - pad] = kb.dist(j);
+            oi[j - pad] = kLower ? load_i32_at(tv.inv_shuffled, kb.key(j)) : kb.key(j);
         }
     }
     if (out_o) {
         int64_t* oo = out_o + qi * k;
 #pragma unroll
         for (int j = 0; j < K; ++j) {
-            if (j >= pad) oo[j - pad] = kLower ? kb.x[j] : load_i32_at(tv.shuffled, kb.x[j]);
+            if (j >= pad) oo[j - pad] = kLower ? kb.key(j) : load_i32_at(tv.shuffled, kb.key(j));
         }
     }
 }
