@@ -24,11 +24,19 @@ constexpr int kBfBlock = 256;
 constexpr int kBfTile = 512;
 constexpr int kBfTargetBlocks = 4096;
 
+constexpr int kBfChunk = 8;  // candidates tested against one stale worst before their insertions
+
+// Elements per point in the LDS tile: d = 3 is padded to 4 so a point is one 16-byte read.
+template <int D>
+constexpr int bf_stride() { return D == 3 ? 4 : D; }
+
 template <typename T, int D, int K>
 __global__ __launch_bounds__(kBfBlock) void bf_kernel(const T* __restrict__ ref, int32_t n, const T* __restrict__ queries,
                                                       int64_t nq, int k, int32_t chunk, T* __restrict__ out_d,
                                                       int32_t* __restrict__ out_i) {
-    __shared__ T tile[kBfTile * D];
+    constexpr int S = bf_stride<D>();
+    __shared__ T tile[kBfTile * S];
+    __shared__ T stage[kBfChunk * kBfBlock];
     const int64_t qi = (int64_t)blockIdx.x * kBfBlock + threadIdx.x;
     const bool active = qi < nq;
     T q[D];
@@ -36,20 +44,38 @@ __global__ __launch_bounds__(kBfBlock) void bf_kernel(const T* __restrict__ ref,
     for (int i = 0; i < D; ++i) q[i] = active ? queries[qi * D + i] : T(0);
     KBest<T, K, false> kb;
     kb.init(k);
+    T* st = stage + threadIdx.x;
     const int32_t cbeg = (int32_t)blockIdx.y * chunk;
     const int32_t cend = (int32_t)min((int64_t)n, (int64_t)cbeg + chunk);
     for (int32_t t0 = cbeg; t0 < cend; t0 += kBfTile) {
         const int32_t cnt = min(kBfTile, cend - t0);
         __syncthreads();
-        for (int e = threadIdx.x; e < cnt * D; e += kBfBlock) tile[e] = ref[(int64_t)t0 * D + e];
+        for (int e = threadIdx.x; e < cnt * D; e += kBfBlock) tile[(e / D) * S + e % D] = ref[(int64_t)t0 * D + e];
         __syncthreads();
         if (active) {
-            for (int32_t j = 0; j < cnt; ++j) {
-                T r[D];
+            // chunks of kBfChunk candidates: distances against the worst at the chunk's start, then
+            // the accepted ones inserted in index order (a candidate the stale worst rejects is
+            // rejected by every later worst too, so the list ends as a one-by-one scan leaves it)
+            for (int32_t j0 = 0; j0 < cnt; j0 += kBfChunk) {
+                const T w = kb.worst();
+                uint32_t mask = 0;
 #pragma unroll
-                for (int i = 0; i < D; ++i) r[i] = tile[j * D + i];
-                const T dd = sq_dist<T, D>(q, r);
-                if (dd < kb.worst()) kb.insert(dd, t0 + j);
+                for (int c = 0; c < kBfChunk; ++c) {
+                    const int32_t j = min(j0 + c, cnt - 1);
+                    T r[D];
+#pragma unroll
+                    for (int i = 0; i < D; ++i) r[i] = tile[j * S + i];
+                    const T dd = sq_dist<T, D>(q, r);
+                    mask |= (dd < w && j0 + c < cnt) ? (1u << c) : 0u;
+                    st[c * kBfBlock] = dd;
+                }
+                while (mask) {
+                    const int c = __builtin_ctz(mask);
+                    mask &= mask - 1u;
+                    const T dd = st[c * kBfBlock];
+                    const bool ok = dd < kb.worst();
+                    kb.insert(ok ? dd : pos_inf<T>(), ok ? t0 + j0 + c : 0x7fffffff);
+                }
             }
         }
     }
