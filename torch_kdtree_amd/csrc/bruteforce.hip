@@ -20,11 +20,17 @@
 namespace tk {
 namespace {
 
-constexpr int kBfBlock = 256;
+#ifndef TK_BF_BLOCK
+#define TK_BF_BLOCK 256
+#endif
+#ifndef TK_BF_CHUNK
+#define TK_BF_CHUNK 16
+#endif
+constexpr int kBfBlock = TK_BF_BLOCK;
 constexpr int kBfTile = 512;
 constexpr int kBfTargetBlocks = 4096;
 
-constexpr int kBfChunk = 8;  // candidates tested against one stale worst before their insertions
+constexpr int kBfChunk = TK_BF_CHUNK;  // candidates tested against one stale worst before their insertions
 
 // Elements per point in the LDS tile: d = 3 is padded to 4 so a point is one 16-byte read.
 template <int D>
