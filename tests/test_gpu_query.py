@@ -126,6 +126,31 @@ def test_gpu_k_limit(cuda):
         tree.query(torch.from_numpy(ref[:10]).to(cuda), nr_nns_searches=129)
 
 
+def test_more_queries_than_one_presort_slice(cuda):
+    """Over 2^28 queries (the presort sorts at most 2^28 per slice): results on both sides of the
+    slice boundary and at the end equal the oracle's."""
+    rng = np.random.default_rng(28)
+    ref = rng.standard_normal((5000, 3)).astype(np.float32)
+    tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels="auto")
+    n = (1 << 28) + 3000
+    gen = torch.Generator(device=cuda)
+    gen.manual_seed(5)
+    q = torch.randn((n, 3), device=cuda, generator=gen)
+    d, i = tree.query(q, nr_nns_searches=1)
+    torch.cuda.synchronize()
+    picks = np.concatenate([np.arange(0, 2000), np.arange((1 << 28) - 2000, (1 << 28) + 2000),
+                            np.arange(n - 2000, n)])
+    sel = torch.from_numpy(picks).to(cuda)
+    qs = q[sel].cpu().numpy()
+    gd = d[sel].cpu().numpy()
+    gi = i[sel].cpu().numpy()
+    del q, d, i
+    od, oi = oracle.OracleTree(ref, tree.levels).search(qs, 1)
+    assert_same(gd, od, "distances")
+    sh = tree.kdtree.get_shuffled_inds()
+    assert_same(gi, sh[oi].astype(np.int64), "original indices")
+
+
 def test_plain_cpu_workload_exact(cuda):
     """Workload 1 (100k refs, 10k queries, k=5, seed 0) at the reference's default depth."""
     w = workloads.plain_cpu()

@@ -393,7 +393,12 @@ int canonical_morton_bits(int64_t nq, int dims) {
     return b;
 }
 
+// Queries are presorted and searched in slices of at most this many (the presort's radix sort
+// orders at most 2^28 pairs); queries are independent, so slicing changes no result.
+constexpr int64_t kQuerySlice = int64_t(1) << 28;
+
 size_t query_gpu_workspace(const Tree& t, int64_t nq, const QueryOpts& o) {
+    nq = std::min(nq, kQuerySlice);
     const int mode = presort_mode(t, nq, o);
     if (mode == kPresortOff) return 0;
     return presort_layout(t, nq, mode).total;
@@ -410,6 +415,16 @@ void query_gpu(const Tree& t, const void* queries, int64_t nq, int k, void* dist
     if (nq < 0) throw std::runtime_error("negative query count");
     if (nq == 0) return;
     if (nq >= (int64_t(1) << 31)) throw std::runtime_error("Only 32 bit signed query indexing implemented");
+    if (nq > kQuerySlice) {
+        const size_t ts = dtype_size(t.dtype);
+        for (int64_t s0 = 0; s0 < nq; s0 += kQuerySlice) {
+            const int64_t n1 = std::min(kQuerySlice, nq - s0);
+            query_gpu(t, static_cast<const unsigned char*>(queries) + (size_t)s0 * t.dims * ts, n1, k,
+                      static_cast<unsigned char*>(dist) + (size_t)s0 * k * ts, idx + s0 * k,
+                      orig ? orig + s0 * k : nullptr, o, stream, workspace, workspace_bytes);
+        }
+        return;
+    }
     DeviceGuard g(t.dev->device);
     const int32_t* order = nullptr;
     std::unique_ptr<StreamScratch> own;
