@@ -152,7 +152,8 @@ int tk_knn_bruteforce_gpu_workspace(int32_t n_ref, int64_t n_queries, int dims, 
  * original-order queries[nq*dims], ref[n_ref*dims], neighbour indices idx[nq*k] (original,
  * int64) and the upstream gradient grad_dist[nq*k] of the squared distances, writes
  * grad_queries[nq*dims] and grad_ref[n_ref*dims] (either may be NULL).  grad_ref is
- * deterministic: no atomics, a fixed summation order. */
+ * deterministic: no atomics, a fixed summation order.  n_queries * k <= 2^28 (the pairs are
+ * ordered by one radix sort). */
 int tk_knn_backward_gpu(const void* queries, const void* ref, int64_t n_queries, int32_t n_ref, int dims, int dtype,
                         int k, const int64_t* idx, const void* grad_dist, void* grad_queries, void* grad_ref,
                         void* workspace, size_t workspace_bytes, void* stream);
@@ -173,7 +174,8 @@ int tk_knn_backward_gpu_workspace(int64_t n_queries, int32_t n_ref, int dims, in
  *     grad_ref is [sum n_points, dims], cloud c owning rows ref_off[c] .. ref_off[c+1] in its
  *     original point order.  Either gradient may be NULL.  Deterministic, and equal to
  *     tk_knn_backward_gpu on each cloud.
- * All scratch memory is allocated with the plan; a plan serves one stream at a time.
+ * All scratch memory is allocated with the plan; a plan serves one stream at a time.  At most
+ * 2^27 queries and 2^28 output elements per plan.
  * tk_batch_capture records forward (+ backward when grad_dist and a gradient are given) with
  * the given buffers into a HIP graph; tk_batch_launch replays it on a stream. */
 typedef struct tk_batch tk_batch;
