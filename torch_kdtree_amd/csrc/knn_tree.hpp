@@ -89,10 +89,22 @@ struct TreeView {
 #ifndef TK_LOWER_WAVES
 #define TK_LOWER_WAVES 1
 #endif
+// Measured neutral (double, d = 3, k = 16: 126 VGPRs at 4 waves, 1.555 vs 1.543 ms) or worse
+// (d = 8, k = 32 at 4 waves spills 72 VGPRs); off.
+#ifndef TK_F64_WAVES
+#define TK_F64_WAVES 0
+#endif
+#ifndef TK_WIDE_WAVES
+#define TK_WIDE_WAVES 0
+#endif
 template <typename T, int D, int K, bool kLower>
 struct TreeOccupancy {
-    static constexpr int kWaves = TK_WAVES_PER_EU >= 0 ? TK_WAVES_PER_EU
-                                  : ((sizeof(T) == 4 && D <= 3 && K <= 16 && (!kLower || TK_LOWER_WAVES)) ? 5 : 0);
+    static constexpr int kWaves =
+        TK_WAVES_PER_EU >= 0 ? TK_WAVES_PER_EU
+        : (sizeof(T) == 4 && D <= 3 && K <= 16 && (!kLower || TK_LOWER_WAVES)) ? 5
+        : (sizeof(T) == 8 && D <= 3 && K == 16 && !kLower) ? TK_F64_WAVES
+        : (sizeof(T) == 4 && D == 8 && K == 32 && !kLower) ? TK_WIDE_WAVES
+                                                            : 0;
 };
 #define TK_TREE_OCCUPANCY(T, D, K, L) __attribute__((amdgpu_waves_per_eu(TreeOccupancy<T, D, K, L>::kWaves > 0 ? TreeOccupancy<T, D, K, L>::kWaves : 1)))
 // The same for the batched search kernels (more live state per lane: the cloud's offsets).
