@@ -58,7 +58,9 @@ __global__ __launch_bounds__(kRadixBlock) void radix_scatter_kernel(const uint32
     __shared__ int32_t placed[kRadixBins];            // pairs of each digit placed so far
     __shared__ int32_t wave_cnt[kWaves][kRadixBins];  // pairs of each digit per wave in the current 256
     __shared__ int32_t scan_tmp[kWaves];
+    __shared__ int32_t blk_off[kRadixBins];           // this block's output start of each digit
     const int t = threadIdx.x;
+    blk_off[t] = offsets[(int64_t)t * n_blocks + blockIdx.x];
     const int lane = t & 63;
     const int w = t >> 6;
     // tile_start: exclusive scan of this tile's digit counts
@@ -122,7 +124,7 @@ __global__ __launch_bounds__(kRadixBlock) void radix_scatter_kernel(const uint32
     for (int x = t; x < n_tile; x += kRadixBlock) {
         const uint32_t key = tile_k[x];
         const int dg = (int)((key >> shift) & (kRadixBins - 1));
-        const int64_t pos = (int64_t)offsets[(int64_t)dg * n_blocks + blockIdx.x] + (x - tile_start[dg]);
+        const int64_t pos = (int64_t)blk_off[dg] + (x - tile_start[dg]);
         keys_out[pos] = key;
         vals_out[pos] = tile_v[x];
     }
