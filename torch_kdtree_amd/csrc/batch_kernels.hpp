@@ -71,7 +71,7 @@ __global__ __launch_bounds__(kTreeBlock) TK_BATCH_OCCUPANCY(T, D, K, kLower) voi
     static_assert(CH <= kRecPad, "a chunk may read at most kRecPad records past a leaf's end");
     __shared__ T stage_d[CH * kTreeBlock];
     __shared__ int32_t stage_o[kLower ? CH * kTreeBlock : 1];
-    const int64_t t = (int64_t)blockIdx.x * kTreeBlock + threadIdx.x;
+    const int64_t t = xcd_tile(blockIdx.x, gridDim.x) * kTreeBlock + threadIdx.x;
     if (t >= count) return;
     const int64_t qi = order[begin + t];
     const int32_t cloud = qcloud[qi];

@@ -18,6 +18,36 @@ namespace tk {
     } while (0)
 
 // ---------------------------------------------------------------------------------------------
+// XCD-aware tile order.  The dispatcher hands workgroups to the 8 XCDs round-robin (workgroup b
+// runs on XCD b % 8), and each XCD has its own L2.  xcd_tile maps workgroup b to a tile so that
+// each XCD takes runs of TK_XCD_GROUP consecutive tiles, the runs dealt to the XCDs in turn:
+// presorted neighbouring queries, which read the same leaves, then share one L2 instead of
+// fetching those leaves into eight, while every XCD still sees all regions of the query set (one
+// contiguous range per XCD, TK_XCD_GROUP 0, leaves the XCDs unevenly loaded when regions differ in
+// cost).  The map is a bijection of [0, n_tiles) for any n_tiles.
+// ---------------------------------------------------------------------------------------------
+#ifndef TK_XCDS
+#define TK_XCDS 8
+#endif
+#ifndef TK_XCD_GROUP
+#define TK_XCD_GROUP 256
+#endif
+
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t n_tiles) {
+    if (TK_XCDS <= 1) return b;
+    const int64_t x = b % TK_XCDS, j = b / TK_XCDS;
+    if (TK_XCD_GROUP > 0) {
+        // runs of TK_XCD_GROUP tiles, dealt to the XCDs in turn; the last partial round in order
+        constexpr int64_t G = TK_XCD_GROUP;
+        const int64_t full = n_tiles / (G * TK_XCDS) * (G * TK_XCDS);
+        if (b >= full) return b;
+        return ((j / G) * TK_XCDS + x) * G + j % G;
+    }
+    const int64_t per = n_tiles / TK_XCDS, extra = n_tiles % TK_XCDS;
+    return x * per + (x < extra ? x : extra) + j;
+}
+
+// ---------------------------------------------------------------------------------------------
 // Point records in HBM.  A leaf is a contiguous run of records in structured order; each record
 // holds the point's coordinates followed by its original index, padded so that one record is
 // one (or a few) 16-byte loads: fp32 d=3 is exactly one float4 (x, y, z, index bits).

@@ -596,7 +596,7 @@ __global__ __launch_bounds__(kTreeBlock) TK_TREE_OCCUPANCY(T, D, K, kLower) void
     static_assert(CH <= kRecPad, "a chunk may read at most kRecPad records past a leaf's end");
     __shared__ T stage_d[CH * kTreeBlock];
     __shared__ int32_t stage_o[kLower ? CH * kTreeBlock : 1];
-    const int64_t t = (int64_t)blockIdx.x * kTreeBlock + threadIdx.x;
+    const int64_t t = xcd_tile(blockIdx.x, gridDim.x) * kTreeBlock + threadIdx.x;
     if (t >= nq) return;
     const int64_t qi = order ? (int64_t)order[t] : t;
     Pt<T, D> q, pp;
@@ -637,7 +637,7 @@ __global__ __launch_bounds__(kLdsBlock) void knn_tree_lds_kernel(TreeView<T> tv,
                                                                   int32_t* __restrict__ out_i,
                                                                   int64_t* __restrict__ out_o) {
     extern __shared__ unsigned char smem[];
-    const int64_t t = (int64_t)blockIdx.x * kLdsBlock + threadIdx.x;
+    const int64_t t = xcd_tile(blockIdx.x, gridDim.x) * kLdsBlock + threadIdx.x;
     if (t >= nq) return;
     const int64_t qi = order ? (int64_t)order[t] : t;
     T* sd = reinterpret_cast<T*>(smem);
