@@ -19,8 +19,27 @@ def oracle_for(tree):
     return oracle.tree_from_product(tree)
 
 
+def with_nan_rows(qry, rng, frac=0.1):
+    """A copy of qry with NaN in one or more coordinates of about frac of its rows; returns (copy, rows)."""
+    q = qry.copy()
+    rows = np.sort(rng.choice(len(q), size=max(1, int(len(q) * frac)), replace=False))
+    for r in rows:
+        m = rng.random(q.shape[1]) < 0.5
+        if not m.any():
+            m[rng.integers(q.shape[1])] = True
+        q[r, m] = np.nan
+    return q, rows
+
+
 def assert_same(a, b, what):
-    if not np.array_equal(a, b):
-        bad = np.argwhere(a != b)
+    """Exact equality; NaN equals NaN (at the same positions) in floating-point arrays."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.shape != b.shape:
+        raise AssertionError("%s: shapes differ, %s vs %s" % (what, a.shape, b.shape))
+    nan_ok = a.dtype.kind == "f" and b.dtype.kind == "f"
+    if not np.array_equal(a, b, equal_nan=nan_ok):
+        same = (a == b) | (np.isnan(a) & np.isnan(b)) if nan_ok else (a == b)
+        bad = np.argwhere(~same)
         raise AssertionError("%s differs at %d positions, first %s: %r vs %r" % (
             what, len(bad), bad[0].tolist(), a[tuple(bad[0])], b[tuple(bad[0])]))

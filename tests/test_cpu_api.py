@@ -11,7 +11,7 @@ import torch
 import oracle
 import torch_kdtree_amd as tk
 from torch_kdtree_amd import torch_knn
-from tests._util import assert_same
+from tests._util import assert_same, with_nan_rows
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -84,6 +84,30 @@ def test_cpu_query_plain_gpu_distribution(levels):
     od, oi = oracle.tree_from_product(t).search(w["queries"], 16)
     assert_same(dist.numpy(), od, "distances")
     assert_same(res_i.numpy(), oi, "structured indices")
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [1, 3, 8])
+def test_cpu_query_nan_rows_match_oracle(dtype, d):
+    """Queries with a NaN coordinate: NaN distances on the last records of the one leaf the
+    reference scans, then (+inf, 0); the other rows as without them."""
+    rng = np.random.default_rng(60 + d)
+    ref = (rng.standard_normal((5000, d)) * 1e3).astype(dtype)
+    qry = (rng.standard_normal((900, d)) * 1e3).astype(dtype)
+    qn, rows = with_nan_rows(qry, rng)
+    t = tk.build_kd_tree(torch.from_numpy(ref), device="cpu", levels=7)
+    ot = oracle.tree_from_product(t)
+    keep = np.setdiff1d(np.arange(len(qry)), rows)
+    for k in (1, 5, 16, 60):
+        res_i = torch.empty((qn.shape[0], k), dtype=torch.int32)
+        dist, inds = t.query(torch.from_numpy(qn), nr_nns_searches=k, result_idx=res_i)
+        od, oi = ot.search(qn, k)
+        assert_same(dist.numpy(), od, "distances")
+        assert_same(res_i.numpy(), oi, "structured indices")
+        assert_same(inds.numpy(), t.kdtree.get_shuffled_inds()[oi].astype(np.int64), "original indices")
+        assert np.isnan(dist.numpy()[rows, 0]).all()
+        d0, _ = t.query(torch.from_numpy(qry), nr_nns_searches=k)
+        assert_same(dist.numpy()[keep], d0.numpy()[keep], "rows without NaN")
 
 
 def test_cpu_query_ties_match_oracle():

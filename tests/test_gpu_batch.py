@@ -11,7 +11,7 @@ import torch
 import oracle
 import torch_kdtree_amd as tk
 from torch_kdtree_amd import torch_knn
-from tests._util import assert_same
+from tests._util import assert_same, with_nan_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -122,6 +122,62 @@ def test_batch_backward_matches_single_tree(cuda, dtype, n_clouds, sizes):
         torch.cuda.synchronize()
         assert_same(dq[q_off[c]:q_off[c + 1]].cpu().numpy(), sq.cpu().numpy(), "cloud %d dL/dq" % c)
         assert_same(dr[r_off[c]:r_off[c + 1]].cpu().numpy(), sr.cpu().numpy(), "cloud %d dL/dr" % c)
+
+
+@pytest.mark.parametrize("tie", ["reference", "lower_index"])
+def test_batch_nan_query_rows(cuda, tie):
+    """Clouds whose query sets hold rows with NaN coordinates: forward equal to the single-tree
+    path (whose NaN rows follow the oracle) and to the oracle; backward (reference policy) equal to
+    the single-tree backward, NaN in the same places."""
+    rng = np.random.default_rng(611 + (tie == "lower_index"))
+    clouds, queries, ks = _clouds(rng, 8, 3, np.float32, sizes=(40, 4000))
+    queries = [with_nan_rows(q, rng)[0] if q.shape[0] else q for q in queries]
+    batch = tk.build_kd_tree_batch(clouds, device=cuda, tie_break=tie)
+    plan = batch.plan([q.shape[0] for q in queries], ks)
+    total_q, total_out, total_ref, _ = plan.sizes()
+    q_cat = torch.from_numpy(np.concatenate(queries)).to(cuda)
+    dd, ii, oo = _plan_forward(plan, q_cat, total_out, q_cat.dtype, cuda)
+    q_off, out_off, r_off = plan.offsets()
+    stream = torch.cuda.current_stream(cuda).cuda_stream
+    if tie == "reference":
+        g = torch.from_numpy(rng.random(total_out).astype(np.float32)).to(cuda)
+        dq = torch.empty_like(q_cat)
+        dr = torch.empty((total_ref, 3), dtype=q_cat.dtype, device=cuda)
+        plan.backward(q_cat.data_ptr(), ii.data_ptr(), g.data_ptr(), dq.data_ptr(), dr.data_ptr(), stream)
+        torch.cuda.synchronize()
+    for c, (pts, qry, k) in enumerate(zip(clouds, queries, ks)):
+        if qry.shape[0] == 0:
+            continue
+        bd = dd[out_off[c]:out_off[c + 1]].view(-1, k).cpu().numpy()
+        bi = ii[out_off[c]:out_off[c + 1]].view(-1, k).cpu().numpy()
+        bo = oo[out_off[c]:out_off[c + 1]].view(-1, k).cpu().numpy()
+        single = tk.build_kd_tree(torch.from_numpy(pts), device=cuda, levels=batch.levels[c], tie_break=tie)
+        q = torch.from_numpy(qry).to(cuda)
+        sd = torch.empty((q.shape[0], k), dtype=q.dtype, device=cuda)
+        si = torch.empty((q.shape[0], k), dtype=torch.int32, device=cuda)
+        _, so = single.query(q, nr_nns_searches=k, result_dists=sd, result_idx=si)
+        torch.cuda.synchronize()
+        assert_same(bd, sd.cpu().numpy(), "cloud %d distances vs single tree" % c)
+        assert_same(bi, si.cpu().numpy(), "cloud %d structured indices vs single tree" % c)
+        assert_same(bo, so.cpu().numpy(), "cloud %d original indices vs single tree" % c)
+        if tie == "reference":
+            od, oi = oracle.OracleTree(pts, batch.levels[c]).search(qry, k)
+            assert_same(bd, od, "cloud %d distances vs oracle" % c)
+            assert_same(bi, oi, "cloud %d structured indices vs oracle" % c)
+            r = torch.from_numpy(pts).to(cuda)
+            idx = oo[out_off[c]:out_off[c + 1]].contiguous()
+            gc = g[out_off[c]:out_off[c + 1]].contiguous()
+            sq = torch.empty_like(q)
+            sr = torch.empty_like(r)
+            torch_knn.knn_backward(q.data_ptr(), r.data_ptr(), q.shape[0], r.shape[0], 3, "float32", k,
+                                   idx.data_ptr(), gc.data_ptr(), sq.data_ptr(), sr.data_ptr(), stream)
+            torch.cuda.synchronize()
+            assert_same(dq[q_off[c]:q_off[c + 1]].cpu().numpy(), sq.cpu().numpy(), "cloud %d dL/dq" % c)
+            assert_same(dr[r_off[c]:r_off[c + 1]].cpu().numpy(), sr.cpu().numpy(), "cloud %d dL/dr" % c)
+        else:
+            od, oi = oracle.bruteforce(pts, qry, k)
+            assert_same(bd, od, "cloud %d distances vs brute force" % c)
+            assert_same(bo, oi.astype(np.int64), "cloud %d indices vs brute force" % c)
 
 
 def test_batch_graph_replays_equal_eager(cuda):

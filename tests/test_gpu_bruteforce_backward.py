@@ -7,7 +7,7 @@ import oracle
 import workloads
 import torch_kdtree_amd as tk
 from torch_kdtree_amd import torch_knn
-from tests._util import assert_same
+from tests._util import assert_same, with_nan_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -35,6 +35,49 @@ def test_bruteforce_matches_oracle(cuda, dtype, d, k):
     od, oi = oracle.bruteforce(ref, qry, k)
     assert_same(gd, od, "distances")
     assert_same(gi, oi, "indices")
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("k", [1, 7, 32])
+def test_bruteforce_nan_query_rows(cuda, dtype, k):
+    """A query with a NaN coordinate: every point in index order, each NaN distance to the front,
+    so (NaN, n-1), (NaN, n-2), ... as in the oracle; the other rows as without them."""
+    rng = np.random.default_rng(320 + k)
+    ref = (rng.standard_normal((3000, 3)) * 1e3).astype(dtype)
+    qry = (rng.standard_normal((2500, 3)) * 1e3).astype(dtype)
+    qn, rows = with_nan_rows(qry, rng)
+    gd, gi = _bruteforce_gpu(ref, qn, k, cuda)
+    od, oi = oracle.bruteforce(ref, qn, k)
+    assert_same(gd, od, "distances")
+    assert_same(gi, oi, "indices")
+    assert np.isnan(gd[rows]).all() and (gi[rows] == np.arange(2999, 2999 - k, -1)).all()
+
+
+def test_backward_with_nan_query_rows(cuda):
+    """fwd+bwd where some queries hold NaN: dL/dq of those queries and dL/dr of the points they
+    returned are NaN on the NaN axes, as in autograd of the reference's recomputation
+    (nn_distance.py:112) in fp64; every other gradient entry within the fp32 tolerance of that
+    recomputation."""
+    w = workloads.diff_sphere(n=20000)
+    rng = np.random.default_rng(8)
+    qn, rows = with_nan_rows(w["queries"], rng, frac=0.02)
+    wt = torch.from_numpy(w["weights"]).to(cuda)
+    r = torch.from_numpy(w["ref"]).to(cuda).requires_grad_(True)
+    q = torch.from_numpy(qn).to(cuda).requires_grad_(True)
+    tree = tk.build_kd_tree(r, levels="auto")
+    dist, idx = tree.query(q, nr_nns_searches=8)
+    (wt * dist).sum().backward()
+    r64 = r.detach().double().requires_grad_(True)
+    q64 = q.detach().double().requires_grad_(True)
+    (wt.double() * torch.sum((q64[:, None] - r64[idx]) ** 2, dim=-1)).sum().backward()
+    for got, want, what in ((q.grad, q64.grad, "dL/dq"), (r.grad, r64.grad, "dL/dr")):
+        got = got.double().cpu().numpy()
+        want = want.cpu().numpy()
+        assert np.array_equal(np.isnan(got), np.isnan(want)), what + " NaN positions"
+        fin = ~np.isnan(want)
+        scale = np.abs(want[fin]).max()
+        assert np.abs(got[fin] - want[fin]).max() <= 1e-5 * scale, what
+    assert np.isnan(q.grad.cpu().numpy()[rows]).any(axis=1).all()
 
 
 def test_bruteforce_ties_go_to_lower_index(cuda):

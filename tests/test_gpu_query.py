@@ -11,7 +11,7 @@ import torch
 import oracle
 import workloads
 import torch_kdtree_amd as tk
-from tests._util import assert_same, gpu_query, oracle_for
+from tests._util import assert_same, gpu_query, oracle_for, with_nan_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -47,6 +47,58 @@ def test_lower_index_policy_matches_bruteforce(cuda, dtype, d, k):
     assert_same(dist, bd, "distances")
     assert_same(oidx, bi.astype(np.int64), "original indices")
     assert_same(tree.kdtree.get_shuffled_inds()[sidx], bi, "structured -> original")
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [1, 3, 8])
+@pytest.mark.parametrize("k", [1, 5, 16, 40])
+def test_nan_query_rows(cuda, dtype, d, k):
+    """Queries with a NaN coordinate.  Reference policy: the oracle's result (NaN distances on the
+    last records of the one leaf the reference scans, then (+inf, 0) when that leaf holds fewer
+    than k points; leaves here hold about 31 points).  Lower-index policy: brute force's (NaN,
+    n-1), (NaN, n-2), ...  The other rows are as without them.  k = 40 uses the LDS list."""
+    rng = np.random.default_rng(700 + d * 3 + k + (dtype == np.float64))
+    ref, qry = _data(rng, 4000, 800, d, dtype)
+    qn, rows = with_nan_rows(qry, rng)
+    keep = np.setdiff1d(np.arange(len(qry)), rows)
+    for policy in ("reference", "lower_index"):
+        tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels=7, tie_break=policy)
+        sh = tree.kdtree.get_shuffled_inds()
+        dist, sidx, oidx = gpu_query(tree, qn, k, cuda)
+        d0, s0, o0 = gpu_query(tree, qry, k, cuda)
+        assert_same(dist[keep], d0[keep], policy + " distances of rows without NaN")
+        assert_same(sidx[keep], s0[keep], policy + " indices of rows without NaN")
+        assert np.isnan(dist[rows, 0]).all()
+        if policy == "reference":
+            od, oi = oracle_for(tree).search(qn, k)
+            assert_same(dist, od, "distances")
+            assert_same(sidx, oi, "structured indices")
+            assert_same(oidx, sh[oi].astype(np.int64), "original indices")
+        else:
+            bd, bi = oracle.bruteforce(ref, qn, k)
+            assert_same(dist, bd, "distances")
+            assert_same(oidx, bi.astype(np.int64), "original indices")
+            assert_same(sh[sidx], bi, "structured -> original")
+
+
+def test_unaligned_output_rows(cuda):
+    """Output arrays that start 4 bytes past a 16-byte boundary (written with 4-byte stores) hold
+    what aligned ones (16-byte stores when k is the list width) hold."""
+    rng = np.random.default_rng(17)
+    ref, qry = _data(rng, 20000, 3000, 3, np.float32)
+    q = torch.from_numpy(qry).to(cuda)
+    for policy in ("reference", "lower_index"):
+        tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels=9, tie_break=policy)
+        for k in (4, 5, 16, 32):
+            d0, s0, o0 = gpu_query(tree, qry, k, cuda)
+            bd = torch.empty(q.shape[0] * k + 1, dtype=torch.float32, device=cuda)[1:].view(-1, k)
+            bi = torch.empty(q.shape[0] * k + 1, dtype=torch.int32, device=cuda)[1:].view(-1, k)
+            assert bd.data_ptr() % 16 == 4 and bi.data_ptr() % 16 == 4
+            d, o = tree.query(q, nr_nns_searches=k, result_dists=bd, result_idx=bi)
+            torch.cuda.synchronize()
+            assert_same(bd.cpu().numpy(), d0, "%s k=%d distances" % (policy, k))
+            assert_same(bi.cpu().numpy(), s0, "%s k=%d structured indices" % (policy, k))
+            assert_same(o.cpu().numpy(), o0, "%s k=%d original indices" % (policy, k))
 
 
 @pytest.mark.parametrize("policy", ["reference", "lower_index"])

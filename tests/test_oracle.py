@@ -101,6 +101,52 @@ def test_search_against_bruteforce_and_recursive(dtype, d):
             assert np.array_equal(dist[:, j], _numpy_sq_dist(qry, e["structured"][idx[:, j]]))
 
 
+def _nan_query_closed_form(e, q, k):
+    """The reference's list for a query with a NaN coordinate, from its comparisons alone: the
+    insertion (src/nndistance.hpp:77-85) rejects only when d >= worst and places a candidate at the
+    upper bound found with `value >= *it`, so a NaN distance is always taken and put in front; the
+    bound tests `< worst` all fail, so only the leaf the descent reaches (a NaN coordinate never
+    compares below a split) is scanned."""
+    axis, split, leaf_off, leaf_n = e["axis"], e["median"], e["leaf_off"], e["leaf_n"]
+    n_nodes = len(split)
+    node = 0
+    while node < n_nodes:
+        node = 2 * node + (1 if q[axis[node]] < split[node] else 2)
+    leaf = node - n_nodes
+    dist = [np.inf] * k
+    idx = [0] * k
+    for s in range(leaf_off[leaf], leaf_off[leaf] + leaf_n[leaf]):
+        dist = [np.nan] + dist[:-1]
+        idx = [s] + idx[:-1]
+    return np.array(dist), np.array(idx)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [1, 3, 8])
+def test_nan_queries_follow_the_insertion_rule(dtype, d):
+    rng = np.random.default_rng(40 + d)
+    for n, levels in ((40, 3), (3000, 7), (9, 1)):
+        ref = (rng.standard_normal((n, d)) * 10).astype(dtype)
+        t = oracle.OracleTree(ref, levels)
+        e = t.export()
+        qry = (rng.standard_normal((150, d)) * 10).astype(dtype)
+        for i in range(len(qry)):
+            m = rng.random(d) < 0.5
+            m[i % d] = True
+            qry[i, m] = np.nan
+        for k in sorted({1, 3, min(7, n), min(60, n)}):
+            dist, idx = t.search(qry, k)
+            rd, ri = t.search(qry, k, recursive=True)
+            assert np.array_equal(dist, rd, equal_nan=True) and np.array_equal(idx, ri)
+            for i in range(len(qry)):
+                cd, ci = _nan_query_closed_form(e, qry[i], k)
+                assert np.array_equal(dist[i], cd, equal_nan=True) and np.array_equal(idx[i], ci), (n, k, i)
+            # brute force, every point in index order and each to the front: n-1, n-2, ...
+            bd, bi = oracle.bruteforce(ref, qry, k)
+            assert np.isnan(bd).all()
+            assert np.array_equal(bi, np.broadcast_to(np.arange(n - 1, n - 1 - k, -1), bi.shape))
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("d", [1, 2, 3, 8])
 @pytest.mark.parametrize("n,levels", [(1000, 6), (777, 5), (64, 7), (5, 1), (3001, 10)])

@@ -91,10 +91,18 @@ __global__ __launch_bounds__(kTreeBlock) TK_BATCH_OCCUPANCY(T, D, K, kLower) voi
     int level;
     load_query<T, D>(queries, qi, q);
     lane_start<T, D, K, kLower>(tv, k, q, pp, kb, far_mask, node, level);
-    while (!lane_step<T, D, K, kLower, CH, LD>(tv, q, pp, kb, far_mask, node, level, stage_d + threadIdx.x,
-                                                stage_o + (kLower ? threadIdx.x : 0))) {
-    }
-    // keys: reference policy = arena record position, lower-index policy = the cloud's original index
+    if (has_nan<T, D>(q))
+        nan_fill<T, D, K, kLower>(tv, q, k, kb);
+    else
+        while (!lane_step<T, D, K, kLower, CH, LD>(tv, q, pp, kb, far_mask, node, level, stage_d + threadIdx.x,
+                                                    stage_o + (kLower ? threadIdx.x : 0))) {
+        }
+    // keys: reference policy = arena record position, lower-index policy = the cloud's original index.
+    // Every gather (index maps, and the neighbour records of the fused dL/dq) is issued for all K
+    // slots before the first store or sum, so the K latencies overlap; padding slots read a valid
+    // entry and are never stored.  A slot left empty (a query with a NaN coordinate) holds key 0:
+    // under the reference policy that is taken as the cloud's first record (structured index 0,
+    // as for a single tree), so that every index and pair key stays inside the cloud.
     const int pad = K - k;
     const int64_t ob = cd.out_base + (qi - cd.q_base) * k;
     // training outputs: this query at its sorted position, its pairs at their canonical slots
@@ -104,16 +112,26 @@ __global__ __launch_bounds__(kTreeBlock) TK_BATCH_OCCUPANCY(T, D, K, kLower) voi
 #pragma unroll
         for (int a = 0; a < D; ++a) q_sorted[tpos * D + a] = q[a];
     }
+    int32_t x[K], loc[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        x[j] = j >= pad ? kb.key(j) : (kLower ? 0 : cd.rec_base);
+        if (!kLower) x[j] = max(x[j], cd.rec_base);
+        loc[j] = kLower ? load_i32_at(arena.inv_shuffled, cd.rec_base + x[j]) : x[j] - cd.rec_base;
+    }
+    int32_t orig[K] = {};
+    if (!kLower && out_o) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) orig[j] = load_i32_at(arena.shuffled, x[j]);
+    }
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         if (j >= pad) {
-            const int32_t x = kb.key(j);
-            const int32_t loc = kLower ? load_i32_at(arena.inv_shuffled, cd.rec_base + x) : x - cd.rec_base;
             out_d[ob + j - pad] = kb.dist(j);
-            out_i[ob + j - pad] = loc;
-            if (out_o) out_o[ob + j - pad] = kLower ? x : load_i32_at(arena.shuffled, x);
+            out_i[ob + j - pad] = loc[j];
+            if (out_o) out_o[ob + j - pad] = kLower ? x[j] : orig[j];
             if (pair_keys) {
-                pair_keys[sb + j - pad] = pair_key(cloud, loc, cd.rec_base, tag_shift);
+                pair_keys[sb + j - pad] = pair_key(cloud, loc[j], cd.rec_base, tag_shift);
                 pair_vals[sb + j - pad] = (uint32_t)(sb + j - pad);
             }
         }
@@ -121,21 +139,24 @@ __global__ __launch_bounds__(kTreeBlock) TK_BATCH_OCCUPANCY(T, D, K, kLower) voi
     // dL/dq of a training step whose upstream gradient is already in place (a captured step):
     // the terms of batch_grad_q_sorted_kernel, in neighbour order, without a second pass
     if (fuse_dq) {
+        Rec<T, D> r[K];
+        T gij[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            r[j] = load_rec<T, D>(arena.recs, cd.rec_base + loc[j]);
+            gij[j] = j >= pad ? fuse_g[ob + j - pad] : T(0);
+        }
         T acc[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) acc[a] = T(0);
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             if (j >= pad) {
-                const int32_t x = kb.key(j);
-                const int32_t loc = kLower ? load_i32_at(arena.inv_shuffled, cd.rec_base + x) : x - cd.rec_base;
-                const Rec<T, D> r = load_rec<T, D>(arena.recs, cd.rec_base + loc);
-                const T gij = fuse_g[ob + j - pad];
-                if (g_sorted) g_sorted[sb + j - pad] = gij;
+                if (g_sorted) g_sorted[sb + j - pad] = gij[j];
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
-                    const T diff = q[a] - r.c[a];
-                    acc[a] = acc[a] + gij * (diff + diff);
+                    const T diff = q[a] - r[j].c[a];
+                    acc[a] = acc[a] + gij[j] * (diff + diff);
                 }
             }
         }

@@ -123,6 +123,32 @@ __global__ __launch_bounds__(256) void bf_merge_kernel(const T* __restrict__ par
     }
 }
 
+// Rows of queries with a NaN coordinate.  Every distance such a query forms is NaN, and the
+// reference's insertion (src/nndistance.hpp:77-85: rejected only when d >= worst, placed at the
+// upper bound found with `value >= *it`) takes each NaN and puts it in front, so scanning the
+// points in index order leaves (NaN, n-1), (NaN, n-2), ..., (NaN, n-k).  The kernels above never
+// insert a NaN; this pass rewrites those rows.
+template <typename T, int D>
+__global__ __launch_bounds__(256) void bf_nan_rows_kernel(const T* __restrict__ ref, int32_t n,
+                                                          const T* __restrict__ queries, int64_t nq, int k,
+                                                          T* __restrict__ out_d, int32_t* __restrict__ out_i) {
+    const int64_t qi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (qi >= nq) return;
+    T q[D];
+    bool nan = false;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        q[i] = queries[qi * D + i];
+        nan |= __builtin_isnan(q[i]);
+    }
+    if (!nan) return;
+    for (int j = 0; j < k; ++j) {
+        const int32_t r = n - 1 - j;
+        out_d[qi * k + j] = sq_dist<T, D>(q, ref + (int64_t)r * D);
+        out_i[qi * k + j] = r;
+    }
+}
+
 struct BfPlan {
     int nchunks;
     int32_t chunk;
@@ -156,14 +182,17 @@ void bf_launch(const T* ref, int32_t n, const T* q, int64_t nq, int k, T* dist, 
     if (p.nchunks == 1) {
         hipLaunchKernelGGL((bf_kernel<T, D, K>), grid, dim3(kBfBlock), 0, s, ref, n, q, nq, k, p.chunk, dist, idx);
         TK_HIP_CHECK(hipGetLastError());
-        return;
+    } else {
+        T* pd = reinterpret_cast<T*>(ws);
+        int32_t* pi = reinterpret_cast<int32_t*>(ws + p.part_bytes_d);
+        hipLaunchKernelGGL((bf_kernel<T, D, K>), grid, dim3(kBfBlock), 0, s, ref, n, q, nq, k, p.chunk, pd, pi);
+        TK_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL((bf_merge_kernel<T, K>), dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, pd, pi,
+                           p.nchunks, nq, k, dist, idx);
+        TK_HIP_CHECK(hipGetLastError());
     }
-    T* pd = reinterpret_cast<T*>(ws);
-    int32_t* pi = reinterpret_cast<int32_t*>(ws + p.part_bytes_d);
-    hipLaunchKernelGGL((bf_kernel<T, D, K>), grid, dim3(kBfBlock), 0, s, ref, n, q, nq, k, p.chunk, pd, pi);
-    TK_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL((bf_merge_kernel<T, K>), dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, pd, pi, p.nchunks,
-                       nq, k, dist, idx);
+    hipLaunchKernelGGL((bf_nan_rows_kernel<T, D>), dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, ref, n, q, nq,
+                       k, dist, idx);
     TK_HIP_CHECK(hipGetLastError());
 }
 

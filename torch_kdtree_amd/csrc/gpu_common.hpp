@@ -217,6 +217,9 @@ __device__ __forceinline__ double pos_inf<double>() { return __builtin_huge_val(
 // kLower == false (reference policy, src/nndistance.hpp:41-85): key = structured index; insert
 //   iff d < worst, at the upper-bound position (after every equal distance already present).
 // kLower == true (lower-index policy): key = original index; the list is ordered by (d, key).
+//   Empty slots start as (+inf, 0), which no (+inf, key) or NaN entry orders before: like the
+//   reference policy, an infinite or NaN distance is never inserted, and a list left unfilled
+//   (a query with a NaN coordinate) reports (+inf, index 0).
 // ---------------------------------------------------------------------------------------------
 template <typename T, int K, bool kLower>
 struct KBest {
@@ -228,13 +231,18 @@ struct KBest {
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             d[j] = j < pad ? -pos_inf<T>() : pos_inf<T>();
-            x[j] = kLower ? (j < pad ? -1 : 0x7fffffff) : 0;
+            x[j] = kLower ? (j < pad ? -1 : 0) : 0;
         }
     }
 
     __device__ __forceinline__ T worst() const { return d[K - 1]; }
     __device__ __forceinline__ T dist(int j) const { return d[j]; }
     __device__ __forceinline__ int32_t key(int j) const { return x[j]; }
+    // slot j set directly (a result known without a search)
+    __device__ __forceinline__ void set(int j, T dist, int32_t key) {
+        d[j] = dist;
+        x[j] = key;
+    }
 
     __device__ __forceinline__ bool accepts(T dist, int32_t key) const {
         if constexpr (kLower)
@@ -293,7 +301,7 @@ struct KBest {
 // original index.  Distances are never negative, so their bits order like the distances (+inf
 // included; a NaN orders after everything and is never accepted), and one unsigned compare orders
 // (distance, index) pairs lexicographically: an insertion takes one compare and two 64-bit selects
-// per slot.  Padding slots hold 0, which nothing orders before; empty slots hold (+inf, INT_MAX).
+// per slot.  Padding slots hold 0, which nothing orders before; empty slots hold (+inf, 0).
 template <int K>
 struct KBest<float, K, true> {
     uint64_t k[K];
@@ -305,12 +313,13 @@ struct KBest<float, K, true> {
     __device__ __forceinline__ void init(int kk) {
         const int pad = K - kk;
 #pragma unroll
-        for (int j = 0; j < K; ++j) k[j] = j < pad ? uint64_t(0) : pack(__builtin_huge_valf(), 0x7fffffff);
+        for (int j = 0; j < K; ++j) k[j] = j < pad ? uint64_t(0) : pack(__builtin_huge_valf(), 0);
     }
 
     __device__ __forceinline__ float worst() const { return __uint_as_float((uint32_t)(k[K - 1] >> 32)); }
     __device__ __forceinline__ float dist(int j) const { return __uint_as_float((uint32_t)(k[j] >> 32)); }
     __device__ __forceinline__ int32_t key(int j) const { return (int32_t)(uint32_t)k[j]; }
+    __device__ __forceinline__ void set(int j, float dist, int32_t key) { k[j] = pack(dist, key); }
 
     __device__ __forceinline__ bool accepts(float dist, int32_t key) const { return pack(dist, key) < k[K - 1]; }
 

@@ -14,6 +14,13 @@
 // The reference keeps a per-node tag array; here the state is the node index and the
 // child we came back from, which is all the tag array encodes on the active path.
 //
+// A query with a NaN coordinate forms only NaN distances and bounds.  The insertion above never
+// rejects a NaN (d >= worst is false) and puts it in front (d >= bd[pos] is false), and every
+// bound test against worst fails, so the reference scans exactly the leaf its descent reaches (a
+// NaN coordinate never compares below a split) and nothing else: its list is that leaf's last k
+// records in reverse order, with their NaN distances, then (+inf, 0) entries when the leaf holds
+// fewer than k points.  nan_query() writes that result directly.
+//
 // Extra pruning that keeps the reference's output: besides the reference's bound, a leaf or a
 // far side is tested against the distance from q to the bounding box of its points.  When
 // that box distance is >= worst, every point there has d >= worst and the reference would
@@ -110,7 +117,28 @@ struct Searcher {
 
     const T* box(int32_t heap_node) const { return aabb + (size_t)heap_node * 2 * D; }
 
+    static bool has_nan(const T* q) {
+        for (int i = 0; i < D; ++i)
+            if (q[i] != q[i]) return true;
+        return false;
+    }
+
+    void nan_query(const T* q, T* bd, int32_t* bi) const {
+        int32_t node = 0;
+        for (int level = 0; level < t.levels; ++level) node = 2 * node + (q[level % D] < split[node] ? 1 : 2);
+        const int32_t leaf = node - t.n_nodes;
+        const int32_t beg = t.leaf_off[leaf], end = t.leaf_off[leaf + 1];
+        for (int j = 0; j < k && end - 1 - j >= beg; ++j) {
+            bd[j] = sq_dist<T, D>(q, pts + (int64_t)(end - 1 - j) * D);
+            bi[j] = end - 1 - j;
+        }
+    }
+
     void one(const T* q, T* bd, int32_t* bi) const {
+        if (has_nan(q)) {
+            nan_query(q, bd, bi);
+            return;
+        }
         T pp[D];
         for (int i = 0; i < D; ++i) pp[i] = q[i];
         const int L = t.levels;

@@ -48,6 +48,12 @@ struct TreeView {
     }
 };
 
+// base[i] for 0 <= i < 2^31, addressed with a 32-bit byte offset (keeps i a 32-bit value; a
+// sign-extended 64-bit index would tie each k-list key to an aligned register pair).
+__device__ __forceinline__ int32_t load_i32_at(const int32_t* __restrict__ base, int32_t i) {
+    return *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(base) + ((uint32_t)i << 2));
+}
+
 // Threads per block of the traversal kernel; per-lane LDS staging slots are strided by it.
 #ifndef TK_TREE_BLOCK
 #define TK_TREE_BLOCK 128
@@ -145,14 +151,15 @@ struct KBestLds {
     T w;        // cached worst distance
     int32_t wo; // cached worst original index
 
-    __device__ __forceinline__ void init() {
+    // empty slots: (+inf, structured index s0, original index 0); see KBest for why 0
+    __device__ __forceinline__ void init(int32_t s0) {
         for (int j = 0; j < k; ++j) {
             d[j * stride] = pos_inf<T>();
-            s[j * stride] = 0;
-            if constexpr (kLower) o[j * stride] = 0x7fffffff;
+            s[j * stride] = s0;
+            if constexpr (kLower) o[j * stride] = 0;
         }
         w = pos_inf<T>();
-        wo = 0x7fffffff;
+        wo = 0;
     }
     __device__ __forceinline__ T worst() const { return w; }
     __device__ __forceinline__ bool accepts(T dist, int32_t orig) const {
@@ -358,6 +365,68 @@ __device__ __forceinline__ void lane_descend(const TreeView<T, kB>& tv, const Pt
     }
 }
 
+// A query with a NaN coordinate forms only NaN distances and bounds.  The reference's insertion
+// (src/nndistance.hpp:77-85) rejects a candidate only when d >= worst and places it at the upper
+// bound found with `value >= *it`, so a NaN distance is always taken and goes to the front; every
+// bound test against worst fails, so its search (src/kdtree.cpp:16-110) scans exactly the leaf
+// its descent reaches (a NaN coordinate never compares below a split) and prunes the rest.  Under
+// the reference policy the list is that leaf's last k records in reverse order, with their
+// distances, then the untouched (+inf, 0) entries when the leaf holds fewer than k points.  The
+// lower-index policy gives what the brute force gives (every point in index order, each to the
+// front): original indices n-1, n-2, ..., n-k.
+template <typename T, int D>
+__device__ __forceinline__ bool has_nan(const Pt<T, D>& q) {
+    bool r = false;
+#pragma unroll
+    for (int i = 0; i < D; ++i) r |= __builtin_isnan(q[i]);
+    return r;
+}
+
+// Record range of the leaf the descent of q reaches.
+template <typename T, int D, bool kB>
+__device__ __forceinline__ void descent_leaf(const TreeView<T, kB>& tv, const Pt<T, D>& q, int32_t& beg,
+                                             int32_t& end) {
+    int32_t node = 0;
+    int a = 0;
+    for (int level = 0; level < tv.levels; ++level) {
+        node = 2 * node + (ax_get(q, a) < tv.split_at(node) ? 1 : 2);
+        a = ax_next<D>(a);
+    }
+    const int32_t leaf = node - tv.n_nodes;
+    beg = tv.leaf_begin(leaf);
+    end = tv.leaf_end(leaf);
+}
+
+// Output slot c (0 = nearest) of a NaN query: record position (arena-wide), key, whether filled.
+template <typename T, int D, bool kLower, bool kB>
+__device__ __forceinline__ bool nan_slot(const TreeView<T, kB>& tv, int32_t beg, int32_t end, int c,
+                                         int32_t& rec, int32_t& key) {
+    if constexpr (kLower) {
+        const int32_t first = tv.leaf_begin(0);
+        const int32_t n = tv.leaf_end(tv.n_nodes) - first;
+        key = n - 1 - c;  // original index within the tree (k <= n)
+        rec = first + load_i32_at(tv.inv_shuffled, first + key);
+        return true;
+    } else {
+        rec = end - 1 - c;
+        key = rec;
+        return rec >= beg;
+    }
+}
+
+template <typename T, int D, int K, bool kLower, bool kB>
+__device__ __forceinline__ void nan_fill(const TreeView<T, kB>& tv, const Pt<T, D>& q, int k, KBest<T, K, kLower>& kb) {
+    int32_t beg = 0, end = 0;
+    if (!kLower) descent_leaf<T, D, kB>(tv, q, beg, end);
+    const int pad = K - k;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        int32_t rec, key;
+        if (j >= pad && nan_slot<T, D, kLower, kB>(tv, beg, end, j - pad, rec, key))
+            kb.set(j, sq_dist<T, D>(q, load_rec<T, D>(tv.recs, rec).c), key);
+    }
+}
+
 // Starts the search of the query in q.
 template <typename T, int D, int K, bool kLower, bool kB>
 __device__ __forceinline__ void lane_start(const TreeView<T, kB>& tv, int k, const Pt<T, D>& q, Pt<T, D>& pp,
@@ -557,10 +626,52 @@ __device__ __forceinline__ void knn_search_one(const TreeView<T, kB>& tv, const 
     }
 }
 
-// base[i] for 0 <= i < 2^31, addressed with a 32-bit byte offset (keeps i a 32-bit value; a
-// sign-extended 64-bit index would tie each k-list key to an aligned register pair).
-__device__ __forceinline__ int32_t load_i32_at(const int32_t* __restrict__ base, int32_t i) {
-    return *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(base) + ((uint32_t)i << 2));
+// Stores val(K-k) .. val(K-1) to p[0 .. k); with vec (k == K and p 16-byte aligned) in 16-byte
+// stores.
+template <typename V, int K, typename F>
+__device__ __forceinline__ void store_row(V* __restrict__ p, bool vec, int pad, F&& val) {
+    constexpr int kPer = 16 / (int)sizeof(V);
+    if constexpr (K % kPer == 0) {
+        if (vec) {
+#pragma unroll
+            for (int j = 0; j < K; j += kPer) {
+                V tmp[kPer];
+#pragma unroll
+                for (int u = 0; u < kPer; ++u) tmp[u] = val(j + u);
+                uint4 w;
+                __builtin_memcpy(&w, tmp, 16);
+                *reinterpret_cast<uint4*>(p + j) = w;
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        if (j >= pad) p[j - pad] = val(j);
+}
+
+// Writes one query's list to row qi of the outputs: distances, structured indices and (with
+// out_o) original indices.  The reference policy keeps structured indices and gathers the
+// original ones through `shuffled`; the lower-index policy keeps original indices and gathers the
+// structured ones through `inv_shuffled`.  Every gather is issued before the first store, so the
+// K latencies overlap instead of following one another; when k == K and the three arrays are
+// 16-byte aligned, each row goes out in 16-byte stores.
+template <typename T, int K, bool kLower>
+__device__ __forceinline__ void store_list(const KBest<T, K, kLower>& kb, int k, const TreeView<T>& tv, int64_t qi,
+                                           T* __restrict__ out_d, int32_t* __restrict__ out_i,
+                                           int64_t* __restrict__ out_o) {
+    const int pad = K - k;
+    const bool vec = pad == 0 && ((((uintptr_t)out_d | (uintptr_t)out_i | (uintptr_t)out_o) & 15u) == 0);
+    int32_t m[K] = {};
+    if (kLower || out_o) {
+        const int32_t* map = kLower ? tv.inv_shuffled : tv.shuffled;
+#pragma unroll
+        for (int j = 0; j < K; ++j) m[j] = load_i32_at(map, j >= pad ? kb.key(j) : 0);
+    }
+    store_row<T, K>(out_d + qi * k, vec, pad, [&](int j) { return kb.dist(j); });
+    store_row<int32_t, K>(out_i + qi * k, vec, pad, [&](int j) { return kLower ? m[j] : kb.key(j); });
+    if (out_o)
+        store_row<int64_t, K>(out_o + qi * k, vec, pad, [&](int j) { return (int64_t)(kLower ? kb.key(j) : m[j]); });
 }
 
 template <typename T, int D>
@@ -606,26 +717,13 @@ __global__ __launch_bounds__(kTreeBlock) TK_TREE_OCCUPANCY(T, D, K, kLower) void
     int level;
     load_query<T, D>(queries, qi, q);
     lane_start<T, D, K, kLower>(tv, k, q, pp, kb, far_mask, node, level);
-    while (!lane_step<T, D, K, kLower, CH, LD>(tv, q, pp, kb, far_mask, node, level, stage_d + threadIdx.x,
-                                            stage_o + (kLower ? threadIdx.x : 0))) {
-    }
-    const int pad = K - k;
-    T* od = out_d + qi * k;
-    int32_t* oi = out_i + qi * k;
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-        if (j >= pad) {
-            od[j - pad] = kb.dist(j);
-            oi[j - pad] = kLower ? load_i32_at(tv.inv_shuffled, kb.key(j)) : kb.key(j);
+    if (has_nan<T, D>(q))
+        nan_fill<T, D, K, kLower>(tv, q, k, kb);
+    else
+        while (!lane_step<T, D, K, kLower, CH, LD>(tv, q, pp, kb, far_mask, node, level, stage_d + threadIdx.x,
+                                                stage_o + (kLower ? threadIdx.x : 0))) {
         }
-    }
-    if (out_o) {
-        int64_t* oo = out_o + qi * k;
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-            if (j >= pad) oo[j - pad] = kLower ? kb.key(j) : load_i32_at(tv.shuffled, kb.key(j));
-        }
-    }
+    store_list<T, K, kLower>(kb, k, tv, qi, out_d, out_i, out_o);
 }
 
 constexpr int kLdsBlock = 64;
@@ -644,10 +742,22 @@ __global__ __launch_bounds__(kLdsBlock) void knn_tree_lds_kernel(TreeView<T> tv,
     int32_t* ss = reinterpret_cast<int32_t*>(sd + (size_t)k * kLdsBlock);
     int32_t* so = ss + (size_t)k * kLdsBlock;
     KBestLds<T, kLower> kb{sd + threadIdx.x, ss + threadIdx.x, so + threadIdx.x, k, kLdsBlock, T(0), 0};
-    kb.init();
+    kb.init(kLower ? tv.inv_shuffled[0] : 0);
     Pt<T, D> q;
     load_query<T, D>(queries, qi, q);
-    knn_search_one<T, D, kLower>(tv, q, kb);
+    if (has_nan<T, D>(q)) {
+        int32_t beg = 0, end = 0;
+        if (!kLower) descent_leaf<T, D>(tv, q, beg, end);
+        for (int c = 0; c < k; ++c) {
+            int32_t rec, key;
+            if (!nan_slot<T, D, kLower>(tv, beg, end, c, rec, key)) break;
+            kb.d[c * kLdsBlock] = sq_dist<T, D>(q, load_rec<T, D>(tv.recs, rec).c);
+            kb.s[c * kLdsBlock] = rec;
+            if constexpr (kLower) kb.o[c * kLdsBlock] = key;
+        }
+    } else {
+        knn_search_one<T, D, kLower>(tv, q, kb);
+    }
     for (int j = 0; j < k; ++j) {
         out_d[qi * k + j] = kb.d[j * kLdsBlock];
         out_i[qi * k + j] = kb.s[j * kLdsBlock];
