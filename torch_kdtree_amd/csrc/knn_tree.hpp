@@ -83,17 +83,22 @@ __device__ __forceinline__ int32_t load_i32_at(const int32_t* __restrict__ base,
 #ifndef TK_STAGE_CHUNK_WIDE
 #define TK_STAGE_CHUNK_WIDE 4
 #endif
-// Occupancy target (waves per SIMD) of the register-list traversal kernels: 5 for float searches
-// with d <= 3 and lists of at most 16 (measured on MI355X: the compiler then keeps the reference
-// policy's kernel at 96 VGPRs with the climb's box prefetch, without spills; the lower-index
-// kernel spills a few registers and is still 7% faster, TK_LOWER_WAVES), no target otherwise
-// (d = 8 or 32-entry lists would spill heavily).  TK_WAVES_PER_EU overrides it for every kernel
-// (0 = no target anywhere).
+// Occupancy target (waves per SIMD) of the register-list traversal kernels, for float searches
+// with d <= 3 and lists of at most 16 (measured on MI355X, plain workload, k = 16):
+//   reference policy, TK_F32_WAVES = 7: 72 VGPRs, no spills, 8.48 ms (5 waves: the same 72 VGPRs
+//     scheduled differently, 8.77 ms; no target: 91 VGPRs, 9.14 ms; 8 waves: spills, 9.29 ms);
+//   lower-index policy, TK_F32_LOWER_WAVES = 5: 92 VGPRs, 10.62 ms (6 waves: 28 bytes of spills,
+//     10.65 ms; 7 waves: 204 bytes of spills, 26.4 ms; no target: 105 VGPRs, 10.98 ms).
+// No target otherwise (d = 8 or 32-entry lists would spill heavily).  TK_WAVES_PER_EU overrides
+// it for every kernel (0 = no target anywhere).
 #ifndef TK_WAVES_PER_EU
 #define TK_WAVES_PER_EU -1
 #endif
-#ifndef TK_LOWER_WAVES
-#define TK_LOWER_WAVES 1
+#ifndef TK_F32_WAVES
+#define TK_F32_WAVES 7
+#endif
+#ifndef TK_F32_LOWER_WAVES
+#define TK_F32_LOWER_WAVES 5
 #endif
 // Measured neutral (double, d = 3, k = 16: 126 VGPRs at 4 waves, 1.555 vs 1.543 ms) or worse
 // (d = 8, k = 32 at 4 waves spills 72 VGPRs); off.
@@ -107,7 +112,7 @@ template <typename T, int D, int K, bool kLower>
 struct TreeOccupancy {
     static constexpr int kWaves =
         TK_WAVES_PER_EU >= 0 ? TK_WAVES_PER_EU
-        : (sizeof(T) == 4 && D <= 3 && K <= 16 && (!kLower || TK_LOWER_WAVES)) ? 5
+        : (sizeof(T) == 4 && D <= 3 && K <= 16) ? (kLower ? TK_F32_LOWER_WAVES : TK_F32_WAVES)
         : (sizeof(T) == 8 && D <= 3 && K == 16 && !kLower) ? TK_F64_WAVES
         : (sizeof(T) == 4 && D == 8 && K == 32 && !kLower) ? TK_WIDE_WAVES
                                                             : 0;
