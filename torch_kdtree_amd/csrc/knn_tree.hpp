@@ -76,6 +76,11 @@ __device__ __forceinline__ int32_t load_i32_at(const int32_t* __restrict__ base,
 #ifndef TK_BATCH_CLIMB_PREFETCH
 #define TK_BATCH_CLIMB_PREFETCH 1
 #endif
+// The same for d > 3, where a box is 2d values (on: d = 8, k = 32, 500K queries 95.9 ms; off:
+// 99.0 ms, with the same registers).
+#ifndef TK_WIDE_CLIMB_PREFETCH
+#define TK_WIDE_CLIMB_PREFETCH 1
+#endif
 // Largest list held in registers (larger k use the LDS-list kernel).
 #ifndef TK_REG_LIST_MAX_K
 #define TK_REG_LIST_MAX_K 32
@@ -89,8 +94,8 @@ __device__ __forceinline__ int32_t load_i32_at(const int32_t* __restrict__ base,
 //     scheduled differently, 8.77 ms; no target: 91 VGPRs, 9.14 ms; 8 waves: spills, 9.29 ms);
 //   lower-index policy, TK_F32_LOWER_WAVES = 5: 92 VGPRs, 10.62 ms (6 waves: 28 bytes of spills,
 //     10.65 ms; 7 waves: 204 bytes of spills, 26.4 ms; no target: 105 VGPRs, 10.98 ms).
-// No target otherwise (d = 8 or 32-entry lists would spill heavily).  TK_WAVES_PER_EU overrides
-// it for every kernel (0 = no target anywhere).
+// Other kernels: see TK_F64_WAVES and TK_WIDE_WAVES below, no target for the rest.
+// TK_WAVES_PER_EU overrides every target (0 = no target anywhere).
 #ifndef TK_WAVES_PER_EU
 #define TK_WAVES_PER_EU -1
 #endif
@@ -100,13 +105,14 @@ __device__ __forceinline__ int32_t load_i32_at(const int32_t* __restrict__ base,
 #ifndef TK_F32_LOWER_WAVES
 #define TK_F32_LOWER_WAVES 5
 #endif
-// Measured neutral (double, d = 3, k = 16: 126 VGPRs at 4 waves, 1.555 vs 1.543 ms) or worse
-// (d = 8, k = 32 at 4 waves spills 72 VGPRs); off.
+// Double, d = 3, k = 16: no target (119 VGPRs, 4 waves; a 5-wave target spills 48 bytes, and a
+// 4-wave one measured neutral).  Float, d = 8, k = 32: 4 waves (128 VGPRs, 12 bytes of spills,
+// 95.9 ms per 500K queries against 99.1 ms at 3 waves and 133 VGPRs).
 #ifndef TK_F64_WAVES
 #define TK_F64_WAVES 0
 #endif
 #ifndef TK_WIDE_WAVES
-#define TK_WIDE_WAVES 0
+#define TK_WIDE_WAVES 4
 #endif
 template <typename T, int D, int K, bool kLower>
 struct TreeOccupancy {
@@ -481,7 +487,7 @@ __device__ __forceinline__ bool lane_step(const TreeView<T, kB>& tv, const Pt<T,
             // The far child of a level entered through its near child is the sibling of the path's
             // child one level down, so these tests need neither the split nor the axis.
             uint32_t near_lv = ((1u << l) - 1u) & ~far_mask;
-            if constexpr (kB && !TK_BATCH_CLIMB_PREFETCH) {
+            if constexpr ((kB && !TK_BATCH_CLIMB_PREFETCH) || (D > 3 && !TK_WIDE_CLIMB_PREFETCH)) {
                 while (near_lv) {
                     const int j = 31 - __builtin_clz(near_lv);
                     const int32_t c = ((node + 1) >> (level - 1 - j)) - 1;
@@ -631,15 +637,15 @@ __device__ __forceinline__ void knn_search_one(const TreeView<T, kB>& tv, const 
     }
 }
 
-// Stores val(K-k) .. val(K-1) to p[0 .. k); with vec (k == K and p 16-byte aligned) in 16-byte
-// stores.
-template <typename V, int K, typename F>
-__device__ __forceinline__ void store_row(V* __restrict__ p, bool vec, int pad, F&& val) {
+// Stores val(j) for the slots j in [J0, J0 + N) that are live (j >= pad) to p[j - pad]; with vec
+// (k == K and p 16-byte aligned) in 16-byte stores.
+template <typename V, int J0, int N, typename F>
+__device__ __forceinline__ void store_slots(V* __restrict__ p, bool vec, int pad, F&& val) {
     constexpr int kPer = 16 / (int)sizeof(V);
-    if constexpr (K % kPer == 0) {
+    if constexpr (N % kPer == 0) {
         if (vec) {
 #pragma unroll
-            for (int j = 0; j < K; j += kPer) {
+            for (int j = J0; j < J0 + N; j += kPer) {
                 V tmp[kPer];
 #pragma unroll
                 for (int u = 0; u < kPer; ++u) tmp[u] = val(j + u);
@@ -651,32 +657,38 @@ __device__ __forceinline__ void store_row(V* __restrict__ p, bool vec, int pad, 
         }
     }
 #pragma unroll
-    for (int j = 0; j < K; ++j)
+    for (int j = J0; j < J0 + N; ++j)
         if (j >= pad) p[j - pad] = val(j);
 }
 
 // Writes one query's list to row qi of the outputs: distances, structured indices and (with
 // out_o) original indices.  The reference policy keeps structured indices and gathers the
 // original ones through `shuffled`; the lower-index policy keeps original indices and gathers the
-// structured ones through `inv_shuffled`.  Every gather is issued before the first store, so the
-// K latencies overlap instead of following one another; when k == K and the three arrays are
-// 16-byte aligned, each row goes out in 16-byte stores.
-template <typename T, int K, bool kLower>
+// structured ones through `inv_shuffled`.  The gathers of up to 16 slots are issued before their
+// stores, so their latencies overlap instead of following one another (a 32-entry list goes out
+// in two halves, which keeps the live registers below those of its search); when k == K and the
+// three arrays are 16-byte aligned, the rows go out in 16-byte stores.
+template <typename T, int K, bool kLower, int J0 = 0>
 __device__ __forceinline__ void store_list(const KBest<T, K, kLower>& kb, int k, const TreeView<T>& tv, int64_t qi,
                                            T* __restrict__ out_d, int32_t* __restrict__ out_i,
                                            int64_t* __restrict__ out_o) {
+    constexpr int N = K - J0 < 16 ? K - J0 : 16;
     const int pad = K - k;
     const bool vec = pad == 0 && ((((uintptr_t)out_d | (uintptr_t)out_i | (uintptr_t)out_o) & 15u) == 0);
-    int32_t m[K] = {};
+    int32_t m[N] = {};
     if (kLower || out_o) {
         const int32_t* map = kLower ? tv.inv_shuffled : tv.shuffled;
 #pragma unroll
-        for (int j = 0; j < K; ++j) m[j] = load_i32_at(map, j >= pad ? kb.key(j) : 0);
+        for (int u = 0; u < N; ++u) m[u] = load_i32_at(map, J0 + u >= pad ? kb.key(J0 + u) : 0);
     }
-    store_row<T, K>(out_d + qi * k, vec, pad, [&](int j) { return kb.dist(j); });
-    store_row<int32_t, K>(out_i + qi * k, vec, pad, [&](int j) { return kLower ? m[j] : kb.key(j); });
+    T* od = out_d + qi * k;
+    int32_t* oi = out_i + qi * k;
+    store_slots<T, J0, N>(od, vec, pad, [&](int j) { return kb.dist(j); });
+    store_slots<int32_t, J0, N>(oi, vec, pad, [&](int j) { return kLower ? m[j - J0] : kb.key(j); });
     if (out_o)
-        store_row<int64_t, K>(out_o + qi * k, vec, pad, [&](int j) { return (int64_t)(kLower ? kb.key(j) : m[j]); });
+        store_slots<int64_t, J0, N>(out_o + qi * k, vec, pad,
+                                    [&](int j) { return (int64_t)(kLower ? kb.key(j) : m[j - J0]); });
+    if constexpr (J0 + N < K) store_list<T, K, kLower, J0 + N>(kb, k, tv, qi, out_d, out_i, out_o);
 }
 
 template <typename T, int D>
