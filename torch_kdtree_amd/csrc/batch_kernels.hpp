@@ -66,8 +66,8 @@ __global__ __launch_bounds__(kTreeBlock) TK_BATCH_OCCUPANCY(T, D, K, kLower) voi
                                                                 T* __restrict__ q_sorted, int tag_shift,
                                                                 const T* __restrict__ fuse_g,
                                                                 T* __restrict__ fuse_dq, T* __restrict__ g_sorted) {
-    constexpr int CH = StageBytes<T, D, K, kLower>::kChunk;
-    constexpr int LD = StageBytes<T, D, K, kLower>::kLoads;
+    constexpr int CH = StageBytes<T, D, K, kLower, true>::kChunk;
+    constexpr int LD = StageBytes<T, D, K, kLower, true>::kLoads;
     static_assert(CH <= kRecPad, "a chunk may read at most kRecPad records past a leaf's end");
     __shared__ T stage_d[CH * kTreeBlock];
     __shared__ int32_t stage_o[kLower ? CH * kTreeBlock : 1];

@@ -61,6 +61,12 @@ __device__ __forceinline__ int32_t load_i32_at(const int32_t* __restrict__ base,
 #ifndef TK_STAGE_CHUNK_F32
 #define TK_STAGE_CHUNK_F32 8
 #endif
+#ifndef TK_STAGE_CHUNK_F32_LOWER
+#define TK_STAGE_CHUNK_F32_LOWER 4
+#endif
+#ifndef TK_STAGE_CHUNK_F32_K16
+#define TK_STAGE_CHUNK_F32_K16 16
+#endif
 #ifndef TK_SCAN_LOADS_F32
 #define TK_SCAN_LOADS_F32 4
 #endif
@@ -697,14 +703,22 @@ __device__ __forceinline__ void load_query(const T* __restrict__ queries, int64_
     for (int i = 0; i < D; ++i) q[i] = queries[qi * D + i];
 }
 
-// Points per staged chunk of the leaf scan (measured on MI355X): TK_STAGE_CHUNK_F32 (8) for float
-// searches with d <= 3 and lists of 8 or more, TK_STAGE_CHUNK_WIDE (4) for other float searches
-// (d = 8, or k <= 4, where the larger chunk's extra stale-worst pops cost more than its fewer
-// exits), TK_STAGE_CHUNK_F64 for double.
-template <typename T, int D, int K, bool kLower>
+// Points per staged chunk of the leaf scan (measured on MI355X), for float searches with d <= 3
+// and lists of 8 or more: TK_STAGE_CHUNK_F32_K16 (16) for the single-tree reference-policy
+// search with a 16-entry list (plain workload 8.32 ms against 8.44 at 8), TK_STAGE_CHUNK_F32 (8)
+// for the other reference-policy ones (k = 8: 6.14 ms against 6.20 at 16; batched replay 21.76 ms
+// against 21.93), TK_STAGE_CHUNK_F32_LOWER (4) for the lower-index policy (10.14 ms against 10.56
+// at 8).  TK_STAGE_CHUNK_WIDE (4) for other float searches (d = 8, or k <= 4, where a larger
+// chunk's extra stale-worst pops cost more than its fewer exits); TK_STAGE_CHUNK_F64 for double.
+// kB: the batched search.
+template <typename T, int D, int K, bool kLower, bool kB = false>
 struct StageBytes {
     static constexpr int kChunk =
-        sizeof(T) == 4 ? ((D <= 3 && K >= 8) ? TK_STAGE_CHUNK_F32 : TK_STAGE_CHUNK_WIDE) : TK_STAGE_CHUNK_F64;
+        sizeof(T) == 4
+            ? ((D <= 3 && K >= 8) ? (kLower ? TK_STAGE_CHUNK_F32_LOWER
+                                            : (K == 16 && !kB ? TK_STAGE_CHUNK_F32_K16 : TK_STAGE_CHUNK_F32))
+                                  : TK_STAGE_CHUNK_WIDE)
+            : TK_STAGE_CHUNK_F64;
     static constexpr int kLoads = sizeof(T) == 4 ? (TK_SCAN_LOADS_F32 < kChunk ? TK_SCAN_LOADS_F32 : kChunk)
                                                  : (TK_SCAN_LOADS_F64 < kChunk ? TK_SCAN_LOADS_F64 : kChunk);
     static constexpr int kBytes = kChunk * kTreeBlock * ((int)sizeof(T) + (kLower ? 4 : 0));
