@@ -5,14 +5,14 @@
 //      thread, 256 threads) in LDS and writes count[digit * n_blocks + block];
 //   2. an exclusive scan of those counts gives, for each (digit, block), the output position of
 //      the block's first pair with that digit (all smaller digits, then earlier blocks);
-//   3. radix_scatter_kernel: every block walks its tile in index order, 256 pairs at a time.
-//      Within a wavefront, lanes holding the same digit are found with one ballot per digit bit;
-//      a pair's slot in the tile is its digit's start in the tile, plus the pairs of its digit
-//      already placed, plus those in lower waves and lower lanes of the current 256.  The tile,
-//      rearranged so in LDS, is then written out slot by slot: each digit's run goes to
-//      consecutive output positions from the block's offset for that digit.  The order of
-//      equal digits is the input order: the sort is stable, and the result does not depend on
-//      scheduling.
+//   3. radix_scatter_kernel: every wavefront of a block ranks its quarter of the tile, in index
+//      order, 64 pairs per round (lanes holding the same digit are found with one ballot per
+//      digit bit; per-digit running counts live in the wavefront's own LDS row).  A pair's slot
+//      in the tile is its digit's start in the tile, plus the pairs of its digit in lower
+//      wavefronts, plus its rank within its wavefront.  The tile, rearranged so in LDS, is then
+//      written out slot by slot: each digit's run goes to consecutive output positions from the
+//      block's offset for that digit.  The order of equal digits is the input order: the sort is
+//      stable, and the result does not depend on scheduling.
 // Nothing is cleared with a memset and nothing waits on the host, so the sort can be captured in
 // a HIP graph and replayed.
 #include <algorithm>
@@ -52,17 +52,17 @@ __global__ __launch_bounds__(kRadixBlock) void radix_scatter_kernel(const uint32
                                                                      uint32_t* __restrict__ keys_out,
                                                                      uint32_t* __restrict__ vals_out) {
     constexpr int kWaves = kRadixBlock / 64;
+    constexpr int kRounds = kRadixTile / kRadixBlock;  // rounds of 64 pairs per wavefront
     __shared__ uint32_t tile_k[kRadixTile];           // the tile in digit order (stable)
     __shared__ uint32_t tile_v[kRadixTile];
     __shared__ int32_t tile_start[kRadixBins + 1];    // first tile slot of each digit
-    __shared__ int32_t placed[kRadixBins];            // pairs of each digit placed so far
-    __shared__ int32_t wave_cnt[kWaves][kRadixBins];  // pairs of each digit per wave in the current 256
+    __shared__ int32_t wave_cnt[kWaves][kRadixBins];  // per wavefront: pairs of each digit, then their first slot
     __shared__ int32_t scan_tmp[kWaves];
     __shared__ int32_t blk_off[kRadixBins];           // this block's output start of each digit
     const int t = threadIdx.x;
-    blk_off[t] = offsets[(int64_t)t * n_blocks + blockIdx.x];
     const int lane = t & 63;
     const int w = t >> 6;
+    blk_off[t] = offsets[(int64_t)t * n_blocks + blockIdx.x];
     // tile_start: exclusive scan of this tile's digit counts
     {
         const int c = hist[(int64_t)t * n_blocks + blockIdx.x];
@@ -79,19 +79,30 @@ __global__ __launch_bounds__(kRadixBlock) void radix_scatter_kernel(const uint32
         tile_start[t] = base + inc - c;
         if (t == kRadixBlock - 1) tile_start[kRadixBins] = base + inc;
     }
-    placed[t] = 0;
 #pragma unroll
     for (int v = 0; v < kWaves; ++v) wave_cnt[v][t] = 0;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kRadixTile;
+    // Wavefront w ranks the pairs [w * kRounds * 64, (w + 1) * kRounds * 64) of the tile, 64 per
+    // round: a pair's rank is the pairs of its digit in the wavefront's earlier rounds (its
+    // running count in LDS) plus those in lower lanes of this round (lanes holding the same digit
+    // are found with one ballot per digit bit).  One wavefront's LDS accesses take effect in
+    // program order, so the running counts need no barrier.
+    const int64_t base = (int64_t)blockIdx.x * kRadixTile + (int64_t)w * kRounds * 64;
     const uint64_t lower_lanes = (1ull << lane) - 1ull;
-    for (int s = 0; s < kRadixItems; ++s) {
-        const int64_t i = base + (int64_t)s * kRadixBlock + t;
+    uint32_t key[kRounds], val[kRounds];
+    int32_t rank[kRounds];
+#pragma unroll
+    for (int s = 0; s < kRounds; ++s) {
+        const int64_t i = base + (int64_t)s * 64 + lane;
         const bool valid = i < n;
-        const uint32_t key = valid ? keys_in[i] : 0u;
-        const uint32_t val = valid ? vals_in[i] : 0u;
+        key[s] = valid ? keys_in[i] : 0u;
+        val[s] = valid ? vals_in[i] : 0u;
+    }
+#pragma unroll
+    for (int s = 0; s < kRounds; ++s) {
+        const bool valid = base + (int64_t)s * 64 + lane < n;
         // invalid slots get digit 256, a group of their own
-        const int dg = valid ? (int)((key >> shift) & (kRadixBins - 1)) : kRadixBins;
+        const int dg = valid ? (int)((key[s] >> shift) & (kRadixBins - 1)) : kRadixBins;
         uint64_t same = ~0ull;
 #pragma unroll
         for (int b = 0; b < 9; ++b) {
@@ -99,33 +110,40 @@ __global__ __launch_bounds__(kRadixBlock) void radix_scatter_kernel(const uint32
             const uint64_t bal = __ballot(bit);
             same &= bit ? bal : ~bal;
         }
-        const int rank = __popcll(same & lower_lanes);
-        const bool last_of_group = (same >> lane) == 1ull;
-        if (valid && last_of_group) wave_cnt[w][dg] = rank + 1;
-        __syncthreads();
-        if (valid) {
-            int pos = tile_start[dg] + placed[dg] + rank;
-            for (int v = 0; v < w; ++v) pos += wave_cnt[v][dg];
-            tile_k[pos] = key;
-            tile_v[pos] = val;
-        }
-        __syncthreads();
-        int add = 0;
+        const int r = __popcll(same & lower_lanes);
+        const int prev = valid ? wave_cnt[w][dg] : 0;
+        rank[s] = prev + r;
+        if (valid && (same >> lane) == 1ull) wave_cnt[w][dg] = prev + r + 1;
+    }
+    __syncthreads();
+    // first slot of each digit's pairs from each wavefront: the wavefronts in order after tile_start
+    {
+        int run = tile_start[t];
 #pragma unroll
         for (int v = 0; v < kWaves; ++v) {
-            add += wave_cnt[v][t];
-            wave_cnt[v][t] = 0;
+            const int c = wave_cnt[v][t];
+            wave_cnt[v][t] = run;
+            run += c;
         }
-        placed[t] += add;
-        __syncthreads();
     }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kRounds; ++s) {
+        if (base + (int64_t)s * 64 + lane < n) {
+            const int dg = (int)((key[s] >> shift) & (kRadixBins - 1));
+            const int pos = wave_cnt[w][dg] + rank[s];
+            tile_k[pos] = key[s];
+            tile_v[pos] = val[s];
+        }
+    }
+    __syncthreads();
     // write out in tile order: each digit's slots go to consecutive output positions
     const int n_tile = tile_start[kRadixBins];
     for (int x = t; x < n_tile; x += kRadixBlock) {
-        const uint32_t key = tile_k[x];
-        const int dg = (int)((key >> shift) & (kRadixBins - 1));
+        const uint32_t k = tile_k[x];
+        const int dg = (int)((k >> shift) & (kRadixBins - 1));
         const int64_t pos = (int64_t)blk_off[dg] + (x - tile_start[dg]);
-        keys_out[pos] = key;
+        keys_out[pos] = k;
         vals_out[pos] = tile_v[x];
     }
 }
