@@ -98,8 +98,9 @@ __device__ __forceinline__ int32_t load_i32_at(const int32_t* __restrict__ base,
 // with d <= 3 and lists of at most 16 (measured on MI355X, plain workload, k = 16):
 //   reference policy, TK_F32_WAVES = 7: 72 VGPRs, no spills, 8.48 ms (5 waves: the same 72 VGPRs
 //     scheduled differently, 8.77 ms; no target: 91 VGPRs, 9.14 ms; 8 waves: spills, 9.29 ms);
-//   lower-index policy, TK_F32_LOWER_WAVES = 5: 92 VGPRs, 10.62 ms (6 waves: 28 bytes of spills,
-//     10.65 ms; 7 waves: 204 bytes of spills, 26.4 ms; no target: 105 VGPRs, 10.98 ms).
+//   lower-index policy (leaf chunks of 4), TK_F32_LOWER_WAVES = 7: 72 VGPRs and 16 bytes of
+//     spills, 9.75 ms (5 waves: 77 VGPRs, 10.08 ms).  With chunks of 8 the kernel needed 92 VGPRs
+//     at 5 waves (10.62 ms) and spilled 204 bytes at 7 (26.4 ms).
 // Other kernels: see TK_F64_WAVES and TK_WIDE_WAVES below, no target for the rest.
 // TK_WAVES_PER_EU overrides every target (0 = no target anywhere).
 #ifndef TK_WAVES_PER_EU
@@ -109,7 +110,7 @@ __device__ __forceinline__ int32_t load_i32_at(const int32_t* __restrict__ base,
 #define TK_F32_WAVES 7
 #endif
 #ifndef TK_F32_LOWER_WAVES
-#define TK_F32_LOWER_WAVES 5
+#define TK_F32_LOWER_WAVES 7
 #endif
 // Double, d = 3, k = 16: no target (119 VGPRs, 4 waves; a 5-wave target spills 48 bytes, and a
 // 4-wave one measured neutral).  Float, d = 8, k = 32: 4 waves (128 VGPRs, 12 bytes of spills,
