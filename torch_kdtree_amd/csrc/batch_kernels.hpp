@@ -12,7 +12,13 @@ namespace tk {
 namespace {
 
 constexpr int kBatchBlock = 256;
-constexpr int kBwUnroll = 4;  // backward loads issued ahead per lane
+// Gathers in flight per lane in the backward kernels (measured: 8 against 4 took the fwd+bwd
+// workload's backward from 1.007 to 0.969 ms and the 256-cloud replay from 20.82 to 20.60 ms;
+// 2 was slower than 4).
+#ifndef TK_BW_UNROLL
+#define TK_BW_UNROLL 8
+#endif
+constexpr int kBwUnroll = TK_BW_UNROLL;  // backward loads issued ahead per lane
 
 template <typename T>
 __global__ __launch_bounds__(kBatchBlock) void batch_zero_kernel(T* __restrict__ p, int64_t n) {
@@ -236,7 +242,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_grad_q_sorted_kernel(
         qi[a] = q_sorted[t * D + a];
         acc[a] = T(0);
     }
-    // loads four neighbours ahead; the terms are still added in neighbour order
+    // loads kBwUnroll neighbours ahead; the terms are still added in neighbour order
     for (int j0 = 0; j0 < cd.k; j0 += kBwUnroll) {
         int32_t id[kBwUnroll];
         T gg[kBwUnroll];
@@ -316,7 +322,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_grad_r_kernel(const unsigne
     T acc[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) acc[a] = T(0);
-    // loads four pairs ahead; the terms are still subtracted in run order
+    // loads kBwUnroll pairs ahead; the terms are still subtracted in run order
     for (uint32_t e0u = e0; e0u < e1; e0u += kBwUnroll) {
         int64_t t[kBwUnroll], jj[kBwUnroll];
         uint32_t slot[kBwUnroll];
