@@ -327,11 +327,12 @@ __device__ __forceinline__ void scan_leaf_staged(const TreeView<T, kB>& tv, int3
                 const Rec<T, D>& r = rg[jj];
                 const T dd = sq_dist<T, D>(q, r.c);
                 const bool acc = kb.accepts_stale(w, dd, kLower ? r.o : 0);
-                mask |= (acc && j < n) ? (1u << j) : 0u;
+                mask |= acc ? (1u << j) : 0u;
                 st_d[j * kTreeBlockStride] = dd;
                 if constexpr (kLower) st_o[j * kTreeBlockStride] = r.o;
             }
         }
+        mask &= n >= 32 ? ~0u : ((1u << n) - 1u);  // only the leaf's points
         // pass 2
         while (mask) {
             const int j = __builtin_ctz(mask);
