@@ -245,7 +245,7 @@ void backward_typed(const T* q, const T* ref, int64_t nq, int32_t n, int k, cons
     const unsigned box_blocks = (unsigned)std::min<int64_t>(1024, ((int64_t)n + kBoxBlock - 1) / kBoxBlock);
     hipLaunchKernelGGL((bbox_kernel<T, D>), dim3(box_blocks), dim3(kBoxBlock), 0, s, ref, (int64_t)n, box);
     const int bits = canonical_morton_bits(nq, D);
-    hipLaunchKernelGGL((morton_bucket_kernel<T, D>), qgrid, dim3(256), 0, s, q, nq, box, bits, qkey, qrow);
+    hipLaunchKernelGGL((morton_bucket_kernel<T, D>), qgrid, dim3(256), 0, s, q, nq, box, bits, qkey, qrow, false);
     TK_HIP_CHECK(hipGetLastError());
     radix_sort_pairs(qkey, qrow, qkey_s, order, nq, bits * D, ws + l.temp, l.temp_bytes, s);
     hipLaunchKernelGGL(rank_kernel, qgrid, dim3(256), 0, s, order, nq, rank);

@@ -832,15 +832,24 @@ __global__ __launch_bounds__(kPresortBlock) void leaf_bucket_kernel(TreeView<T> 
 }
 
 // ---------------------------------------------------------------------------------------------
-// Morton keys: the Morton code of a query's cell in a 2^B-per-axis grid over a box (B*D <= 21).
+// Morton keys: the Morton code of a query's cell in a 2^B-per-axis grid over a box (B*D <= 24).
 // The forward presort uses the queries' own bounding box and the finest grid, so queries
 // outside the tree's cloud are grouped by where they really are.  The backward's canonical
 // order uses the reference points' bounding box and B from canonical_morton_bits(Q, D)
 // (knn_gpu.hpp), the same rule in the single-tree and the batched backward.
 // ---------------------------------------------------------------------------------------------
+// Bits per axis of the forward presort's grid (measured on MI355X): d = 3, 8 bits (a 24-bit key,
+// still three radix passes; plain workload traversal 7.92 ms against 8.23 with 7 bits, depth 13
+// 20.29 against 21.33); d = 8, 3 bits (24-bit key; 90.4 ms per 500K queries against 94.0 with 2).
+#ifndef TK_MORTON_BITS_3D
+#define TK_MORTON_BITS_3D 8
+#endif
+#ifndef TK_MORTON_BITS_8D
+#define TK_MORTON_BITS_8D 3
+#endif
 template <int D>
 struct MortonBits {
-    static constexpr int kBits = D == 1 ? 20 : D == 2 ? 10 : D == 3 ? 7 : 2;
+    static constexpr int kBits = D == 1 ? 20 : D == 2 ? 10 : D == 3 ? TK_MORTON_BITS_3D : TK_MORTON_BITS_8D;
     static constexpr int kBuckets = 1 << (kBits * D);
 };
 
@@ -856,6 +865,52 @@ __device__ __forceinline__ uint32_t morton_cell(const float* lo, const float* hi
         const float x = ((float)q[a] - lo[a]) * scale;
         const uint32_t g = (uint32_t)fminf(fmaxf(x, 0.0f), cells - 1.0f);
         for (int b = 0; b < bits; ++b) code |= ((g >> b) & 1u) << (b * D + a);
+    }
+    return code;
+}
+
+// Hilbert index of q's cell in the same grid as morton_cell: the cell's integer coordinates go
+// through Skilling's transpose transform (J. Skilling, "Programming the Hilbert curve", AIP Conf.
+// Proc. 707, 2004: undo the excess work, then Gray-encode), and the transposed coordinates are
+// interleaved most significant bit first.  Consecutive indices are adjacent cells, which the
+// Z-order's jumps between blocks of cells are not.
+template <typename T, int D>
+__device__ __forceinline__ uint32_t hilbert_cell(const float* lo, const float* hi, int bits, const T* __restrict__ q) {
+    const float cells = (float)(1 << bits);
+    uint32_t x[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        const float scale = hi[a] > lo[a] ? cells / (hi[a] - lo[a]) : 0.0f;
+        const float v = ((float)q[a] - lo[a]) * scale;
+        x[a] = (uint32_t)fminf(fmaxf(v, 0.0f), cells - 1.0f);
+    }
+    if (bits > 0) {
+        const uint32_t m = 1u << (bits - 1);
+        for (uint32_t qb = m; qb > 1; qb >>= 1) {
+            const uint32_t p = qb - 1;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                if (x[a] & qb) {
+                    x[0] ^= p;
+                } else {
+                    const uint32_t t = (x[0] ^ x[a]) & p;
+                    x[0] ^= t;
+                    x[a] ^= t;
+                }
+            }
+        }
+#pragma unroll
+        for (int a = 1; a < D; ++a) x[a] ^= x[a - 1];
+        uint32_t t = 0;
+        for (uint32_t qb = m; qb > 1; qb >>= 1)
+            if (x[D - 1] & qb) t ^= qb - 1;
+#pragma unroll
+        for (int a = 0; a < D; ++a) x[a] ^= t;
+    }
+    uint32_t code = 0;
+    for (int b = bits - 1; b >= 0; --b) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) code = (code << 1) | ((x[a] >> b) & 1u);
     }
     return code;
 }
@@ -932,12 +987,16 @@ __global__ __launch_bounds__(kBoxBlock) void bbox_kernel(const T* __restrict__ p
     }
 }
 
-// Morton key over a box held as ordered integers (from bbox_kernel).
+// Cell key over a box held as ordered integers (from bbox_kernel): the Hilbert index when hilbert
+// (the forward presort at d <= 3: plain workload traversal 7.78 ms against 7.88 with the Morton
+// code, fp64 workload 1.41 against 1.49; at d = 8 the Morton code is faster, 90.6 against 92.2 ms),
+// else the Morton code (always for the backward's canonical order).
 template <typename T, int D>
 __global__ __launch_bounds__(kPresortBlock) void morton_bucket_kernel(const T* __restrict__ queries, int64_t nq,
                                                                        const OrderedT<T>* __restrict__ box, int bits,
                                                                        uint32_t* __restrict__ key,
-                                                                       uint32_t* __restrict__ row) {
+                                                                       uint32_t* __restrict__ row,
+                                                                       bool hilbert) {
     const int64_t i = (int64_t)blockIdx.x * kPresortBlock + threadIdx.x;
     if (i >= nq) return;
     float lo[D], hi[D];
@@ -946,7 +1005,7 @@ __global__ __launch_bounds__(kPresortBlock) void morton_bucket_kernel(const T* _
         lo[a] = (float)from_ordered(box[a]);
         hi[a] = (float)from_ordered(box[D + a]);
     }
-    key[i] = morton_cell<T, D>(lo, hi, bits, queries + i * D);
+    key[i] = hilbert ? hilbert_cell<T, D>(lo, hi, bits, queries + i * D) : morton_cell<T, D>(lo, hi, bits, queries + i * D);
     row[i] = (uint32_t)i;
 }
 

@@ -67,6 +67,9 @@ void launch_tree_search_t(const Tree& t, const T* q, int64_t nq, const int32_t* 
 #undef TK_DIMS
 }
 
+#ifndef TK_HILBERT_MAX_D
+#define TK_HILBERT_MAX_D 3
+#endif
 // bbox: 2*dims ordered integers of sizeof(T) bytes each, initialised by the caller (init_bbox).
 template <typename T>
 void launch_morton_bucket_t(const Tree& t, const T* q, int64_t nq, void* bbox, uint32_t* key, uint32_t* row,
@@ -78,7 +81,7 @@ void launch_morton_bucket_t(const Tree& t, const T* q, int64_t nq, void* bbox, u
     case DD:                                                                                                     \
         hipLaunchKernelGGL((bbox_kernel<T, DD>), dim3(box_blocks), dim3(kBoxBlock), 0, s, q, nq, box);          \
         hipLaunchKernelGGL((morton_bucket_kernel<T, DD>), grid, dim3(kPresortBlock), 0, s, q, nq, box,           \
-                           MortonBits<DD>::kBits, key, row);                                                     \
+                           MortonBits<DD>::kBits, key, row, DD <= TK_HILBERT_MAX_D);                                                     \
         break;
     switch (t.dims) {
         TK_DIMS(1)
