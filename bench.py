@@ -3,7 +3,7 @@
 
 Default workload (``plain_gpu``): d=3 fp32, 4,000,000 reference points U[0,1]^3, 16,000,000
 queries (half U[-0.1,1.1]^3, half from a 32-component Gaussian mixture, sigma 0.02), k=16.
-A step is one ``TorchKDTree.query`` over all 16M queries (Morton presort + traversal + outputs:
+A step is one ``TorchKDTree.query`` over all 16M queries (Hilbert-order presort + traversal + outputs:
 squared distances, structured int32 indices and original int64 indices), with the tree and the
 queries already resident in HBM.  The tree build and upload are outside the timed region, as
 in the reference's benchmark (README.md:75).

@@ -46,9 +46,10 @@ enum { TK_F32 = 0, TK_F64 = 1 };
 enum { TK_TIE_REFERENCE = 0, TK_TIE_LOWER_INDEX = 1 };
 
 /* Query presort before the traversal, so that the lanes of a wavefront get nearby queries:
- * TK_PRESORT_MORTON buckets queries by the Morton code of their position in the queries'
- * bounding box, TK_PRESORT_LEAF by the tree leaf they fall into; AUTO = MORTON from 4096
- * queries up.  The presort changes the order of work only, never the results. */
+ * TK_PRESORT_MORTON orders queries along a space-filling curve over a grid on the queries'
+ * bounding box (the Hilbert curve for d <= 3, the Morton code for d = 8; the name is kept),
+ * TK_PRESORT_LEAF by the tree leaf they fall into; AUTO = MORTON from 4096 queries up.  The
+ * presort changes the order of work only, never the results. */
 enum { TK_PRESORT_AUTO = -1, TK_PRESORT_OFF = 0, TK_PRESORT_MORTON = 1, TK_PRESORT_LEAF = 2 };
 
 /* Thread-local message of the last failure on this thread. */

@@ -256,8 +256,9 @@ def build_kd_tree(points_ref: Union[torch.Tensor, np.ndarray], device: torch.dev
         "reference" (default): exact distance ties in the reference CPU search's order.
         "lower_index": ties ordered by the lower original index.
     presort : "auto", "on"/"morton", "leaf" or "off"
-        How GPU queries are ordered before the traversal (Morton buckets of their positions, or
-        buckets by tree leaf); the order changes speed only, never results.
+        How GPU queries are ordered before the traversal (along a space-filling curve over their
+        positions, Hilbert for d <= 3 and Morton for d = 8, or by tree leaf); the order changes
+        speed only, never results.
     builder : "cpu" (default) or "gpu"
         "gpu" builds a GPU tree on the device.  It gives the same tree as the CPU build whenever no
         node has two points with equal coordinates on its split axis; points with such ties may be
