@@ -36,32 +36,28 @@ namespace {
 #endif
 constexpr int kBwUnroll = TK_BW_UNROLL;  // loads issued ahead per lane
 
-// Inverse of the canonical order: rank[order[t]] = t.
-__global__ __launch_bounds__(256) void rank_kernel(const uint32_t* __restrict__ order, int64_t n,
-                                                   uint32_t* __restrict__ rank) {
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (t < n) rank[order[t]] = (uint32_t)t;
-}
-
-// dL/dq, one lane per query row, terms added in neighbour order; with q_sorted (for dL/dr) it also
-// copies the query to its canonical position t = rank[i] and, per neighbour j, the upstream
-// gradient and the pair (reference index, slot) to the canonical slot t * k + j.
+// dL/dq, terms added in neighbour order.  Without q_sorted one lane per query row.  With q_sorted
+// (for dL/dr) lane t takes the query i = order[t] of canonical position t, so that neighbouring
+// lanes gather neighbouring reference rows, and also copies the query to q_sorted[t] and, per
+// neighbour j, the upstream gradient and the pair (reference index, slot) to the canonical slot
+// t * k + j, written contiguously across lanes.
 template <typename T, int D>
 __global__ __launch_bounds__(256) void grad_q_kernel(const T* __restrict__ q, const T* __restrict__ ref, int64_t nq,
                                                      int k, const int64_t* __restrict__ idx,
-                                                     const T* __restrict__ g, const uint32_t* __restrict__ rank,
+                                                     const T* __restrict__ g, const uint32_t* __restrict__ order,
                                                      T* __restrict__ dq, T* __restrict__ q_sorted,
                                                      T* __restrict__ g_sorted, uint32_t* __restrict__ keys,
                                                      uint32_t* __restrict__ vals) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= nq) return;
+    const int64_t lane = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (lane >= nq) return;
+    const int64_t t = q_sorted ? lane : 0;
+    const int64_t i = q_sorted ? (int64_t)order[lane] : lane;
     T qi[D], acc[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         qi[a] = q[i * D + a];
         acc[a] = T(0);
     }
-    const int64_t t = q_sorted ? (int64_t)rank[i] : 0;
     if (q_sorted) {
 #pragma unroll
         for (int a = 0; a < D; ++a) q_sorted[t * D + a] = qi[a];
@@ -186,7 +182,7 @@ int key_bits(int64_t n) {
 }
 
 struct BwLayout {
-    size_t qkey, qrow, qkey_s, order, rank, run_start, run_end, bbox, q_sorted, g_sorted, keys, vals, keys_s, vals_s,
+    size_t qkey, qrow, qkey_s, order, run_start, run_end, bbox, q_sorted, g_sorted, keys, vals, keys_s, vals_s,
         temp, temp_bytes, total;
 };
 
@@ -199,7 +195,6 @@ BwLayout bw_layout(int64_t nq, int32_t n, int k, int dims, int dtype) {
     l.qrow = off; off += a256(4 * (size_t)nq);
     l.qkey_s = off; off += a256(4 * (size_t)nq);
     l.order = off; off += a256(4 * (size_t)nq);
-    l.rank = off; off += a256(4 * (size_t)nq);
     l.run_start = off; off += a256(4 * (size_t)n);
     l.run_end = off; off += a256(4 * (size_t)n);
     l.bbox = off; off += a256(16 * (size_t)dims);
@@ -234,12 +229,11 @@ void backward_typed(const T* q, const T* ref, int64_t nq, int32_t n, int k, cons
         TK_HIP_CHECK(hipGetLastError());
         return;
     }
-    // canonical query order and its inverse
+    // canonical query order
     uint32_t* qkey = reinterpret_cast<uint32_t*>(ws + l.qkey);
     uint32_t* qrow = reinterpret_cast<uint32_t*>(ws + l.qrow);
     uint32_t* qkey_s = reinterpret_cast<uint32_t*>(ws + l.qkey_s);
     uint32_t* order = reinterpret_cast<uint32_t*>(ws + l.order);
-    uint32_t* rank = reinterpret_cast<uint32_t*>(ws + l.rank);
     auto* box = reinterpret_cast<OrderedT<T>*>(ws + l.bbox);
     init_bbox(box, sizeof(T) == 4 ? kF32 : kF64, D, s);
     const unsigned box_blocks = (unsigned)std::min<int64_t>(1024, ((int64_t)n + kBoxBlock - 1) / kBoxBlock);
@@ -248,8 +242,6 @@ void backward_typed(const T* q, const T* ref, int64_t nq, int32_t n, int k, cons
     hipLaunchKernelGGL((morton_bucket_kernel<T, D>), qgrid, dim3(256), 0, s, q, nq, box, bits, qkey, qrow, false);
     TK_HIP_CHECK(hipGetLastError());
     radix_sort_pairs(qkey, qrow, qkey_s, order, nq, bits * D, ws + l.temp, l.temp_bytes, s);
-    hipLaunchKernelGGL(rank_kernel, qgrid, dim3(256), 0, s, order, nq, rank);
-    TK_HIP_CHECK(hipGetLastError());
     // dL/dq, canonical copies and pairs; pairs sorted by reference index; dL/dr per reference point
     T* q_sorted = reinterpret_cast<T*>(ws + l.q_sorted);
     T* g_sorted = reinterpret_cast<T*>(ws + l.g_sorted);
@@ -259,7 +251,7 @@ void backward_typed(const T* q, const T* ref, int64_t nq, int32_t n, int k, cons
     uint32_t* vals_s = reinterpret_cast<uint32_t*>(ws + l.vals_s);
     uint32_t* run_start = reinterpret_cast<uint32_t*>(ws + l.run_start);
     uint32_t* run_end = reinterpret_cast<uint32_t*>(ws + l.run_end);
-    hipLaunchKernelGGL((grad_q_kernel<T, D>), qgrid, dim3(256), 0, s, q, ref, nq, k, idx, g, rank, dq, q_sorted,
+    hipLaunchKernelGGL((grad_q_kernel<T, D>), qgrid, dim3(256), 0, s, q, ref, nq, k, idx, g, order, dq, q_sorted,
                        g_sorted, keys, vals);
     TK_HIP_CHECK(hipGetLastError());
     radix_sort_pairs(keys, vals, keys_s, vals_s, m, key_bits(n), ws + l.temp, l.temp_bytes, s);
