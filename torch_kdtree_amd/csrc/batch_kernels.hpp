@@ -281,21 +281,34 @@ __global__ __launch_bounds__(kBatchBlock) void batch_zero_runs_kernel(uint32_t* 
 
 // Bounds of each record's run in the sorted pairs (zero for records without pairs).  A run is a
 // maximal range of equal keys (whole keys, tag included).
+constexpr int kRunItems = 4;
 __global__ __launch_bounds__(kBatchBlock) void batch_run_bounds_kernel(const uint32_t* __restrict__ keys, int64_t m,
                                                                         const CloudDesc* __restrict__ clouds,
                                                                         int tag_shift,
                                                                         uint32_t* __restrict__ run_start,
                                                                         uint32_t* __restrict__ run_end) {
-    const int64_t p = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x;
-    if (p >= m) return;
-    const uint32_t key = keys[p];
-    const bool first = p == 0 || keys[p - 1] != key;
-    const bool last = p == m - 1 || keys[p + 1] != key;
-    if (!first && !last) return;
-    const uint32_t rec = tag_shift ? (uint32_t)clouds[key >> tag_shift].rec_base + (key & ((1u << tag_shift) - 1u))
-                                   : key;
-    if (first) run_start[rec] = (uint32_t)p;
-    if (last) run_end[rec] = (uint32_t)(p + 1);
+    // each lane looks at kRunItems consecutive pairs, with one key on either side
+    const int64_t p0 = ((int64_t)blockIdx.x * kBatchBlock + threadIdx.x) * kRunItems;
+    if (p0 >= m) return;
+    uint32_t k[kRunItems + 2];
+#pragma unroll
+    for (int u = 0; u < kRunItems + 2; ++u) {
+        const int64_t p = p0 - 1 + u;
+        k[u] = (p >= 0 && p < m) ? keys[p] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kRunItems; ++u) {
+        const int64_t p = p0 + u;
+        if (p >= m) break;
+        const uint32_t key = k[u + 1];
+        const bool first = p == 0 || k[u] != key;
+        const bool last = p == m - 1 || k[u + 2] != key;
+        if (!first && !last) continue;
+        const uint32_t rec = tag_shift ? (uint32_t)clouds[key >> tag_shift].rec_base + (key & ((1u << tag_shift) - 1u))
+                                       : key;
+        if (first) run_start[rec] = (uint32_t)p;
+        if (last) run_end[rec] = (uint32_t)(p + 1);
+    }
 }
 
 // dL/dr, one lane per record: its run of sorted pairs (canonical order), written to the record's
@@ -484,7 +497,8 @@ void batch_backward_t(const BatchPlan& p, const T* q, const int32_t* idx, const 
                              p.radix_ws, p.radix_ws_bytes, s);                                                       \
             hipLaunchKernelGGL(batch_zero_runs_kernel, batch_grid(p.total_ref), dim3(kBatchBlock), 0, s,             \
                                p.run_start, p.run_end, p.total_ref);                                                 \
-            hipLaunchKernelGGL(batch_run_bounds_kernel, batch_grid(p.total_out), dim3(kBatchBlock), 0, s, p.keys_s,  \
+            hipLaunchKernelGGL(batch_run_bounds_kernel, batch_grid((p.total_out + kRunItems - 1) / kRunItems),      \
+                               dim3(kBatchBlock), 0, s, p.keys_s,                                                    \
                                p.total_out, p.d_clouds, p.pair_tag_shift, p.run_start, p.run_end);                   \
             if (from_forward && p.g_sorted && p.g_sorted_valid)                                                      \
                 hipLaunchKernelGGL((batch_grad_r_kernel<T, DD, true>), batch_grid(p.total_ref), dim3(kBatchBlock), 0, \
