@@ -64,6 +64,9 @@ __device__ __forceinline__ int32_t load_i32_at(const int32_t* __restrict__ base,
 #ifndef TK_STAGE_CHUNK_F32_LOWER
 #define TK_STAGE_CHUNK_F32_LOWER 4
 #endif
+#ifndef TK_STAGE_CHUNK_SMALLK
+#define TK_STAGE_CHUNK_SMALLK 8
+#endif
 #ifndef TK_STAGE_CHUNK_F32_K16
 #define TK_STAGE_CHUNK_F32_K16 16
 #endif
@@ -710,8 +713,10 @@ __device__ __forceinline__ void load_query(const T* __restrict__ queries, int64_
 // search with a 16-entry list (plain workload 8.32 ms against 8.44 at 8), TK_STAGE_CHUNK_F32 (8)
 // for the other reference-policy ones (k = 8: 6.14 ms against 6.20 at 16; batched replay 21.76 ms
 // against 21.93), TK_STAGE_CHUNK_F32_LOWER (4) for the lower-index policy (10.14 ms against 10.56
-// at 8).  TK_STAGE_CHUNK_WIDE (4) for other float searches (d = 8, or k <= 4, where a larger
-// chunk's extra stale-worst pops cost more than its fewer exits); TK_STAGE_CHUNK_F64 for double.
+// at 8).  For lists of at most 4 at d <= 3, TK_STAGE_CHUNK_SMALLK (8) in the single-tree search
+// (k = 4: 4.78 ms against 4.91 at 4; k = 1: 3.75 against 3.80) and TK_STAGE_CHUNK_WIDE (4) in the
+// batched one (replay 20.43 ms against 20.74 at 8).  TK_STAGE_CHUNK_WIDE (4) also for d = 8, where a
+// larger chunk's extra stale-worst pops cost more than its fewer exits; TK_STAGE_CHUNK_F64 for double.
 // kB: the batched search.
 template <typename T, int D, int K, bool kLower, bool kB = false>
 struct StageBytes {
@@ -719,7 +724,7 @@ struct StageBytes {
         sizeof(T) == 4
             ? ((D <= 3 && K >= 8) ? (kLower ? TK_STAGE_CHUNK_F32_LOWER
                                             : (K == 16 && !kB ? TK_STAGE_CHUNK_F32_K16 : TK_STAGE_CHUNK_F32))
-                                  : TK_STAGE_CHUNK_WIDE)
+                                  : (D <= 3 && !kB ? TK_STAGE_CHUNK_SMALLK : TK_STAGE_CHUNK_WIDE))
             : TK_STAGE_CHUNK_F64;
     static constexpr int kLoads = sizeof(T) == 4 ? (TK_SCAN_LOADS_F32 < kChunk ? TK_SCAN_LOADS_F32 : kChunk)
                                                  : (TK_SCAN_LOADS_F64 < kChunk ? TK_SCAN_LOADS_F64 : kChunk);
