@@ -113,8 +113,11 @@ int tk_tree_upload(tk_tree* tree, int device);
 void tk_tree_free(tk_tree* tree);
 
 /* Tree facts: out[0]=dims, out[1]=dtype, out[2]=levels, out[3]=n_points, out[4]=device
- * (-1 when not uploaded). */
-int tk_tree_info(const tk_tree* tree, int32_t out[5]);
+ * (-1 when not uploaded), out[5]=1 when every coordinate and split value is finite.  A tree with a
+ * non-finite value can produce NaN distances, which the reference's insertion takes into the list
+ * (src/nndistance.hpp:41-85); its reference-policy queries then follow the reference's walk step
+ * for step, without point-box pruning. */
+int tk_tree_info(const tk_tree* tree, int32_t out[6]);
 
 /* Replaces KDTree::get_shuffled_inds (src/interface.cpp:45-50): out[n_points] receives the
  * original index of each structured position. */

@@ -18,8 +18,9 @@
 //   * pruning bound ..... src/kdtree.hpp:606-651 (projectionDist, partitionNecessary)
 //   * leaf scan ......... src/kdtree.hpp:663-692, insertion src/nndistance.hpp:41-127
 //   * brute force ....... src/nndistance_g.cu:6-72 semantics (scan in index order, insert
-//                          when d < worst at the upper-bound position), i.e. ties go to the
-//                          lower index.
+//                          when d < worst, :57, at the upper-bound position), i.e. ties go to
+//                          the lower index and a NaN distance is never inserted; slots left
+//                          empty hold (+inf, 0) (the reference leaves their index unset).
 //
 // Pins (see tests/test_oracle.py): brute force on every query, the reference's own
 // iterative-vs-recursive equality, hand-traced known-answer vectors in tests/golden/,
@@ -307,7 +308,8 @@ void search_all(const OTree<T, D>& t, const T* qflat, int64_t nq, int k, T* dist
     }
 }
 
-// nndistance_g.cu:6-72 semantics on the original (unpermuted) points.
+// nndistance_g.cu:6-72 semantics on the original (unpermuted) points: a candidate is inserted
+// iff d < worst (nndistance_g.cu:57, 126), so NaN distances never are.
 template <typename T, int D>
 void bruteforce(const T* rflat, int32_t n, const T* qflat, int64_t nq, int k, T* dist, int32_t* idx,
                 int nthreads) {
@@ -320,7 +322,7 @@ void bruteforce(const T* rflat, int32_t n, const T* qflat, int64_t nq, int k, T*
         std::vector<int32_t> bi(k, 0);
         for (int32_t j = 0; j < n; ++j) {
             const T d = quadr_dist<T, D>(qs[i], rs[j]);
-            if (d >= bd[k - 1]) continue;
+            if (!(d < bd[k - 1])) continue;
             const uint32_t pos = upper_bound_pos<T>(bd.data(), static_cast<uint32_t>(k), d);
             for (int s = k - 1; s > static_cast<int>(pos); --s) {
                 bd[s] = bd[s - 1];
@@ -493,7 +495,7 @@ int oracle_search(const void* hp, const void* q, int64_t nq, int k, void* dist, 
     return 0;
 }
 
-// Exact KNN by (distance, original index) on the original point order.
+// Exact KNN by (distance, original index) on the original point order; NaN distances excluded.
 int oracle_bruteforce(const void* ref, int32_t n, const void* q, int64_t nq, int dims, int k, void* dist,
                       int32_t* idx, int dtype, int nthreads) {
     try {

@@ -11,7 +11,7 @@ import torch
 import oracle
 import torch_kdtree_amd as tk
 from torch_kdtree_amd import torch_knn
-from tests._util import assert_same, with_nan_rows
+from tests._util import assert_same, nonfinite_cloud, nonfinite_queries, with_nan_rows
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -62,6 +62,7 @@ def test_cpu_query_matches_oracle(dtype, d):
     qry = (rng.standard_normal((4000, d)) * 1e3).astype(dtype)
     for levels in (None, 13):
         t = tk.build_kd_tree(torch.from_numpy(ref), device="cpu", levels=levels)
+        assert t.kdtree.is_finite()
         ot = oracle.tree_from_product(t)
         for k in (1, 7, 16, 150):
             res_i = torch.empty((qry.shape[0], k), dtype=torch.int32)
@@ -108,6 +109,48 @@ def test_cpu_query_nan_rows_match_oracle(dtype, d):
         assert np.isnan(dist.numpy()[rows, 0]).all()
         d0, _ = t.query(torch.from_numpy(qry), nr_nns_searches=k)
         assert_same(dist.numpy()[keep], d0.numpy()[keep], "rows without NaN")
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [1, 3, 8])
+@pytest.mark.parametrize("kind", ["one_nan", "nan_rows", "inf_rows", "mixed", "overflow"])
+def test_cpu_query_nonfinite_cloud_matches_oracle(dtype, d, kind):
+    """Reference clouds with NaN or infinite coordinates (or medians that overflow): the build
+    equals the oracle's two-sort build, and the search equals the reference's step for step, NaN
+    distances included (taken by `d >= worst` failing, placed by upperBound's probes,
+    src/nndistance.hpp:41-85), on queries with NaN and infinite coordinates as well."""
+    rng = np.random.default_rng(70 + d + 7 * (dtype == np.float64) + 31 * len(kind))
+    ref = nonfinite_cloud(rng, 2000, d, dtype, kind)
+    qry = nonfinite_queries(rng, 500, d, dtype)
+    for levels in (1, 5, 8):
+        t = tk.build_kd_tree(torch.from_numpy(ref), device="cpu", levels=levels)
+        ob = oracle.OracleTree(ref, levels).export()
+        assert t.kdtree.is_finite() == (np.isfinite(ref).all() and np.isfinite(ob["median"]).all())
+        assert_same(t.kdtree.get_structured_points(), ob["structured"], "structured points")
+        assert_same(t.kdtree.get_shuffled_inds(), ob["shuffled"], "shuffled indices")
+        ot = oracle.tree_from_product(t)
+        for k in (1, 4, 5, 16, 40):
+            res_i = torch.empty((qry.shape[0], k), dtype=torch.int32)
+            dist, inds = t.query(torch.from_numpy(qry), nr_nns_searches=k, result_idx=res_i)
+            od, oi = ot.search(qry, k)
+            assert_same(dist.numpy(), od, "distances")
+            assert_same(res_i.numpy(), oi, "structured indices")
+            assert_same(inds.numpy(), t.kdtree.get_shuffled_inds()[oi].astype(np.int64), "original indices")
+
+
+def test_cpu_query_one_nan_coordinate_2000_points():
+    """One NaN coordinate in a 2,000-point cloud, depth 5, k = 4: every row equals the oracle."""
+    rng = np.random.default_rng(11)
+    ref = rng.random((2000, 3)).astype(np.float32)
+    ref[1234, 1] = np.nan
+    qry = rng.random((500, 3)).astype(np.float32)
+    t = tk.build_kd_tree(torch.from_numpy(ref), device="cpu", levels=5)
+    res_i = torch.empty((500, 4), dtype=torch.int32)
+    dist, _ = t.query(torch.from_numpy(qry), nr_nns_searches=4, result_idx=res_i)
+    od, oi = oracle.tree_from_product(t).search(qry, 4)
+    assert_same(dist.numpy(), od, "distances")
+    assert_same(res_i.numpy(), oi, "structured indices")
+    assert np.isnan(od).any(), "the NaN point must reach some lists"
 
 
 def test_cpu_query_ties_match_oracle():

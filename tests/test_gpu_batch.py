@@ -243,3 +243,83 @@ def test_batch_errors(cuda):
         batch.plan([3, 3], [1, 33])
     with pytest.raises(RuntimeError):
         batch.query([torch.zeros((2, 3)), torch.zeros((2, 3))], [1, 6])
+
+
+def test_two_steps_of_equal_shapes_keep_their_own_graphs(cuda):
+    """Two BatchSteps with the same query counts and k but different queries and weights: each
+    replay writes its own step's buffers with its own inputs, equal to an eager run of those."""
+    rng = np.random.default_rng(321)
+    clouds, queries, ks = _clouds(rng, 6, 3, np.float32, sizes=(100, 8000))
+    batch = tk.build_kd_tree_batch(clouds, device=cuda)
+    qa = [torch.from_numpy(q) for q in queries]
+    qb = [torch.from_numpy((q + rng.normal(0, 0.01, q.shape)).astype(np.float32)) for q in queries]
+    wa = [torch.from_numpy(rng.random((q.shape[0], k)).astype(np.float32)) for q, k in zip(queries, ks)]
+    wb = [torch.from_numpy(rng.random((q.shape[0], k)).astype(np.float32)) for q, k in zip(queries, ks)]
+    step_a = tk.BatchStep(batch, qa, ks, wa)
+    step_b = tk.BatchStep(batch, qb, ks, wb)
+    assert step_a.plan is not step_b.plan
+    stream = torch.cuda.current_stream(cuda).cuda_stream
+    want = {}
+    for name, step in (("a", step_a), ("b", step_b)):
+        plan = batch.new_plan(step.nq, step.ks)
+        total_q, total_out, total_ref, _ = plan.sizes()
+        ed, ei, eo = _plan_forward(plan, step.queries, total_out, torch.float32, cuda)
+        edq = torch.empty_like(step.queries)
+        edr = torch.empty((total_ref, 3), dtype=torch.float32, device=cuda)
+        plan.backward(step.queries.data_ptr(), ei.data_ptr(), step.weights.data_ptr(), edq.data_ptr(),
+                      edr.data_ptr(), stream)
+        torch.cuda.synchronize()
+        want[name] = (ed, ei, eo, edq, edr)
+    assert not torch.equal(want["a"][0], want["b"][0])
+    for rep in range(2):
+        for name, step in (("a", step_a), ("b", step_b), ("a", step_a)):
+            for buf in (step.dist, step.grad_queries, step.grad_refs):
+                buf.fill_(-1.0)
+            step.replay()
+            torch.cuda.synchronize()
+            ed, ei, eo, edq, edr = want[name]
+            got = (step.dist, step.idx, step.orig, step.grad_queries, step.grad_refs)
+            for what, g, w in zip(("dist", "idx", "orig", "dL/dq", "dL/dr"), got, (ed, ei, eo, edq, edr)):
+                assert torch.equal(g, w), "step %s replay %d %s" % (name, rep, what)
+
+
+def test_batch_autograd_with_mixed_array_and_tensor_clouds(cuda):
+    """Clouds given partly as numpy arrays, partly as tensors requiring grad: the tensor clouds get
+    their per-cloud gradients, the array clouds none, and the queries theirs."""
+    rng = np.random.default_rng(77)
+    clouds, queries, ks = _clouds(rng, 4, 3, np.float32, sizes=(200, 3000))
+    given = [torch.from_numpy(c).to(cuda).requires_grad_(True) if i % 2 == 0 else c for i, c in enumerate(clouds)]
+    qs = [torch.from_numpy(q).to(cuda).requires_grad_(True) for q in queries]
+    ws = [torch.from_numpy(rng.random((q.shape[0], k)).astype(np.float32)).to(cuda) for q, k in zip(queries, ks)]
+    batch = tk.build_kd_tree_batch(given, device=cuda)
+    dists, _ = batch.query(qs, ks)
+    sum((w * d).sum() for w, d in zip(ws, dists)).backward()
+    for c in range(len(clouds)):
+        r2 = torch.from_numpy(clouds[c]).to(cuda).requires_grad_(True)
+        q2 = torch.from_numpy(queries[c]).to(cuda).requires_grad_(True)
+        d2, _ = tk.build_kd_tree(r2, device=cuda, levels=batch.levels[c]).query(q2, nr_nns_searches=ks[c])
+        if q2.shape[0]:
+            (ws[c] * d2).sum().backward()
+            assert torch.equal(q2.grad, qs[c].grad), "cloud %d dL/dq" % c
+            if c % 2 == 0:
+                assert torch.equal(r2.grad, given[c].grad), "cloud %d dL/dr" % c
+
+
+def test_batch_refuses_nonfinite_clouds_under_reference_policy(cuda):
+    """A cloud with a NaN coordinate: the reference policy's batched plan refuses it with a message
+    (its single-tree query follows the reference's own steps); the lower-index plan accepts it and
+    equals brute force, which never inserts a NaN distance."""
+    rng = np.random.default_rng(5)
+    a = rng.random((3000, 3)).astype(np.float32)
+    b = rng.random((2000, 3)).astype(np.float32)
+    b[17, 2] = np.nan
+    qa, qb = rng.random((500, 3)).astype(np.float32), rng.random((400, 3)).astype(np.float32)
+    batch = tk.build_kd_tree_batch([a, b], device=cuda)
+    with pytest.raises(RuntimeError, match="non-finite"):
+        batch.plan([500, 400], [4, 4])
+    low = tk.build_kd_tree_batch([a, b], device=cuda, tie_break="lower_index")
+    dists, inds = low.query([torch.from_numpy(qa), torch.from_numpy(qb)], [4, 4])
+    for pts, qry, d, i in ((a, qa, dists[0], inds[0]), (b, qb, dists[1], inds[1])):
+        bd, bi = oracle.bruteforce(pts, qry, 4)
+        assert_same(d.cpu().numpy(), bd, "distances")
+        assert_same(i.cpu().numpy(), bi.astype(np.int64), "indices")

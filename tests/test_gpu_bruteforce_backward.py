@@ -40,8 +40,9 @@ def test_bruteforce_matches_oracle(cuda, dtype, d, k):
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("k", [1, 7, 32])
 def test_bruteforce_nan_query_rows(cuda, dtype, k):
-    """A query with a NaN coordinate: every point in index order, each NaN distance to the front,
-    so (NaN, n-1), (NaN, n-2), ... as in the oracle; the other rows as without them."""
+    """A query with a NaN coordinate: the reference's brute force inserts only when d < worst
+    (src/nndistance_g.cu:57), which a NaN never is, so the row stays (+inf, 0), as in the oracle;
+    reference points with a NaN coordinate are never returned.  The other rows as without them."""
     rng = np.random.default_rng(320 + k)
     ref = (rng.standard_normal((3000, 3)) * 1e3).astype(dtype)
     qry = (rng.standard_normal((2500, 3)) * 1e3).astype(dtype)
@@ -50,7 +51,13 @@ def test_bruteforce_nan_query_rows(cuda, dtype, k):
     od, oi = oracle.bruteforce(ref, qn, k)
     assert_same(gd, od, "distances")
     assert_same(gi, oi, "indices")
-    assert np.isnan(gd[rows]).all() and (gi[rows] == np.arange(2999, 2999 - k, -1)).all()
+    assert np.isposinf(gd[rows]).all() and (gi[rows] == 0).all()
+    rn, rrows = with_nan_rows(ref, rng, frac=0.05)
+    gd, gi = _bruteforce_gpu(rn, qn, k, cuda)
+    od, oi = oracle.bruteforce(rn, qn, k)
+    assert_same(gd, od, "distances, NaN reference rows")
+    assert_same(gi, oi, "indices, NaN reference rows")
+    assert not np.isin(gi, rrows).any()
 
 
 def test_backward_with_nan_query_rows(cuda):

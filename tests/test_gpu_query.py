@@ -11,7 +11,7 @@ import torch
 import oracle
 import workloads
 import torch_kdtree_amd as tk
-from tests._util import assert_same, gpu_query, oracle_for, with_nan_rows
+from tests._util import assert_same, gpu_query, nonfinite_cloud, nonfinite_queries, oracle_for, with_nan_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -22,7 +22,7 @@ def _data(rng, n, q, d, dtype, scale=1e3):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("d", [1, 2, 3, 8])
-@pytest.mark.parametrize("k", [1, 5, 16, 33])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 16, 33])
 def test_reference_policy_matches_oracle(cuda, dtype, d, k):
     rng = np.random.default_rng(100 + d * 7 + k)
     ref, qry = _data(rng, 20000, 3000, d, dtype)
@@ -37,7 +37,7 @@ def test_reference_policy_matches_oracle(cuda, dtype, d, k):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("d", [1, 3, 8])
-@pytest.mark.parametrize("k", [1, 8, 32, 100])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 8, 32, 100])
 def test_lower_index_policy_matches_bruteforce(cuda, dtype, d, k):
     rng = np.random.default_rng(200 + d * 11 + k)
     ref, qry = _data(rng, 6000, 1500, d, dtype)
@@ -55,8 +55,9 @@ def test_lower_index_policy_matches_bruteforce(cuda, dtype, d, k):
 def test_nan_query_rows(cuda, dtype, d, k):
     """Queries with a NaN coordinate.  Reference policy: the oracle's result (NaN distances on the
     last records of the one leaf the reference scans, then (+inf, 0) when that leaf holds fewer
-    than k points; leaves here hold about 31 points).  Lower-index policy: brute force's (NaN,
-    n-1), (NaN, n-2), ...  The other rows are as without them.  k = 40 uses the LDS list."""
+    than k points; leaves here hold about 31 points).  Lower-index policy: brute force's, which
+    never inserts a NaN distance (d < worst, src/nndistance_g.cu:57): (+inf, 0) in every slot.
+    The other rows are as without them.  k = 40 uses the LDS list."""
     rng = np.random.default_rng(700 + d * 3 + k + (dtype == np.float64))
     ref, qry = _data(rng, 4000, 800, d, dtype)
     qn, rows = with_nan_rows(qry, rng)
@@ -68,8 +69,8 @@ def test_nan_query_rows(cuda, dtype, d, k):
         d0, s0, o0 = gpu_query(tree, qry, k, cuda)
         assert_same(dist[keep], d0[keep], policy + " distances of rows without NaN")
         assert_same(sidx[keep], s0[keep], policy + " indices of rows without NaN")
-        assert np.isnan(dist[rows, 0]).all()
         if policy == "reference":
+            assert np.isnan(dist[rows, 0]).all()
             od, oi = oracle_for(tree).search(qn, k)
             assert_same(dist, od, "distances")
             assert_same(sidx, oi, "structured indices")
@@ -79,6 +80,37 @@ def test_nan_query_rows(cuda, dtype, d, k):
             assert_same(dist, bd, "distances")
             assert_same(oidx, bi.astype(np.int64), "original indices")
             assert_same(sh[sidx], bi, "structured -> original")
+            assert np.isposinf(dist[rows]).all() and (oidx[rows] == 0).all()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [1, 3, 8])
+@pytest.mark.parametrize("kind", ["one_nan", "nan_rows", "mixed", "overflow"])
+def test_nonfinite_cloud(cuda, dtype, d, kind):
+    """Reference clouds with NaN or infinite coordinates, or medians that overflow.  Reference
+    policy: the oracle's result bit for bit, NaN distances included (the reference's own walk and
+    list, knn_tree_exact_kernel); lower-index policy: brute force's (NaN distances never
+    inserted), with pruning that a NaN split never stops.  Queries hold NaN, +-inf and huge rows."""
+    rng = np.random.default_rng(900 + d + 5 * (dtype == np.float64) + 17 * len(kind))
+    ref = nonfinite_cloud(rng, 3000, d, dtype, kind)
+    qry = nonfinite_queries(rng, 600, d, dtype)
+    for levels in (1, 5, 7):
+        tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels=levels)
+        ot = oracle_for(tree)
+        assert tree.kdtree.is_finite() == (np.isfinite(ref).all() and np.isfinite(ot.export()["median"]).all())
+        sh = tree.kdtree.get_shuffled_inds()
+        for k in (1, 5, 16, 40):
+            dist, sidx, oidx = gpu_query(tree, qry, k, cuda)
+            od, oi = ot.search(qry, k)
+            assert_same(dist, od, "levels %d k %d distances" % (levels, k))
+            assert_same(sidx, oi, "levels %d k %d structured indices" % (levels, k))
+            assert_same(oidx, sh[oi].astype(np.int64), "levels %d k %d original indices" % (levels, k))
+        low = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels=levels, tie_break="lower_index")
+        for k in (1, 5, 16, 40):
+            dist, sidx, oidx = gpu_query(low, qry, k, cuda)
+            bd, bi = oracle.bruteforce(ref, qry, k)
+            assert_same(dist, bd, "lower-index levels %d k %d distances" % (levels, k))
+            assert_same(oidx, bi.astype(np.int64), "lower-index levels %d k %d original indices" % (levels, k))
 
 
 def test_unaligned_output_rows(cuda):

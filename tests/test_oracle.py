@@ -141,10 +141,74 @@ def test_nan_queries_follow_the_insertion_rule(dtype, d):
             for i in range(len(qry)):
                 cd, ci = _nan_query_closed_form(e, qry[i], k)
                 assert np.array_equal(dist[i], cd, equal_nan=True) and np.array_equal(idx[i], ci), (n, k, i)
-            # brute force, every point in index order and each to the front: n-1, n-2, ...
+            # brute force inserts only when d < worst (src/nndistance_g.cu:57): nothing for NaN
             bd, bi = oracle.bruteforce(ref, qry, k)
-            assert np.isnan(bd).all()
-            assert np.array_equal(bi, np.broadcast_to(np.arange(n - 1, n - 1 - k, -1), bi.shape))
+            assert np.isposinf(bd).all() and (bi == 0).all()
+
+
+def _tree_1level(structured, median, n_left, shuffled=None):
+    """A one-level d=1 tree given as arrays: node 0 splits at `median`, leaf 0 holds the first
+    n_left structured points, leaf 1 the rest."""
+    pts = np.asarray(structured, np.float32).reshape(-1, 1)
+    n = pts.shape[0]
+    sh = np.arange(n, dtype=np.int32) if shuffled is None else np.asarray(shuffled, np.int32)
+    return oracle.OracleTree.from_arrays(pts, sh, np.array([0], np.int32), np.array([median], np.float32),
+                                         np.array([n_left, n - n_left], np.int32),
+                                         np.array([0, n_left], np.int32), 1)
+
+
+def test_nan_distance_insertion_known_answer():
+    """The reference's insertion with NaN distances, traced by hand (src/nndistance.hpp:41-85:
+    reject iff d >= worst, else insert at upperBound, whose probes compare `value >= *it`).
+    Query 0, k = 3; the near leaf's points 0.9, 1.0, NaN, 0.7, 0.8, 1.7 give the distances
+    a=0.81, b=1.0, NaN, c=0.49, e=0.64, f=2.89 in scan order:
+      a -> [a, inf, inf]; b -> [a, b, inf]; NaN fails every probe -> [NaN, a, b];
+      c: probe 1 (a) fails, probe 0 (NaN) fails -> [c, NaN, a];
+      e: e < a passes the pre-check; probe 1 (NaN) fails, probe 0 (c) passes -> [c, e, NaN];
+      f: f >= NaN is false, so it is taken; probe 1 (e) passes, probe 2 (NaN) fails -> [c, e, f].
+    The far leaf's bound (100 - 0)^2 is not below worst, so it is never scanned.  A brute force
+    (d < worst, src/nndistance_g.cu:57) drops the NaN and returns the three smallest."""
+    f32 = np.float32
+    x = [0.9, 1.0, np.nan, 0.7, 0.8, 1.7]
+    t = _tree_1level(x + [200.0, 300.0], 100.0, 6)
+    d, i = t.search(np.zeros((1, 1), f32), 3)
+    sq = [f32(v) * f32(v) for v in (0.7, 0.8, 1.7)]
+    assert d[0].tolist() == [float(v) for v in sq] and i[0].tolist() == [3, 4, 5]
+    # with k = 4 the NaN never reaches the last slot: [c, e, NaN, a] after e, then f >= a is rejected
+    d4, i4 = t.search(np.zeros((1, 1), f32), 4)
+    assert i4[0].tolist() == [3, 4, 2, 0] and np.isnan(d4[0, 2])
+    assert d4[0, [0, 1, 3]].tolist() == [float(sq[0]), float(sq[1]), float(f32(0.9) * f32(0.9))]
+    bd, bi = oracle.bruteforce(np.array(x + [200.0, 300.0], f32).reshape(-1, 1), np.zeros((1, 1), f32), 3)
+    assert bi[0].tolist() == [3, 4, 0]
+
+
+def test_nan_split_known_answer():
+    """A NaN split value, traced by hand.  Points (d=1) 1, NaN, 3, 2 at one level: libstdc++'s
+    std::sort on 4 elements is an insertion sort (`val < prev` moves left): NaN compares false
+    both ways and stays, 3 stays behind NaN, 2 moves left past 3 and stops at NaN, so the
+    structured order is 1, NaN, 2, 3 (shuffled 0, 1, 3, 2) and the median 0.5*(2 + NaN) is NaN.
+    Query 0: 0 < NaN is false, so the near leaf is the right one (2, 3): [4, 9, inf] with
+    structured indices [2, 3, 0]; the far leaf's bound ((0-0) + (NaN - 0))^2 is NaN, not below
+    worst, so the point at distance 1 is never seen."""
+    pts = np.array([[1.0], [np.nan], [3.0], [2.0]], np.float32)
+    t = oracle.OracleTree(pts, 1)
+    e = t.export()
+    assert e["shuffled"].tolist() == [0, 1, 3, 2] and np.isnan(e["median"][0])
+    d, i = t.search(np.zeros((1, 1), np.float32), 3)
+    assert d[0].tolist() == [4.0, 9.0, float("inf")] and i[0].tolist() == [2, 3, 0]
+
+
+def test_infinite_query_known_answer():
+    """A query at +inf against finite points: every distance is +inf (rejected, inf >= inf) and
+    every bound NaN (inf - inf); the list stays (+inf, 0).  The same from brute force."""
+    rng = np.random.default_rng(5)
+    pts = rng.standard_normal((300, 3)).astype(np.float32)
+    q = np.array([[np.inf, 0.0, 0.0], [0.0, -np.inf, 1.0]], np.float32)
+    t = oracle.OracleTree(pts, 4)
+    d, i = t.search(q, 5)
+    assert np.isposinf(d).all() and (i == 0).all()
+    bd, bi = oracle.bruteforce(pts, q, 5)
+    assert np.isposinf(bd).all() and (bi == 0).all()
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])

@@ -118,27 +118,41 @@ class KDTreeBatch:
         self._clouds = [c if isinstance(c, torch.Tensor) else None for c in clouds]
         self._plans = {}
 
+    def new_plan(self, n_queries: Sequence[int], ks: Sequence[int]):
+        """A plan of its own (arena, scratch and, once captured, graph) for these query counts and k."""
+        return torch_knn.BatchPlan(self.trees, [int(x) for x in n_queries], [int(x) for x in ks],
+                                   _TIE_POLICIES[self.tie_break],
+                                   self.device.index if self.device.index is not None else -1)
+
     def plan(self, n_queries: Sequence[int], ks: Sequence[int]):
+        """The plan ``query`` uses for these query counts and k, created once and then reused (eager
+        forward and backward only; a captured graph needs a plan of its own, ``new_plan``)."""
         key = (tuple(int(x) for x in n_queries), tuple(int(x) for x in ks))
         p = self._plans.get(key)
         if p is None:
-            p = torch_knn.BatchPlan(self.trees, list(key[0]), list(key[1]), _TIE_POLICIES[self.tie_break],
-                                    self.device.index if self.device.index is not None else -1)
+            p = self.new_plan(key[0], key[1])
             self._plans[key] = p
         return p
 
     def _refs_for_grad(self):
+        """The clouds concatenated for autograd when any tensor cloud requires grad: tensor clouds as
+        given (so gradients flow back to each of them), clouds given as arrays as zeros without grad."""
         refs = self._clouds
-        if any(r is None for r in refs) or not any(r.requires_grad for r in refs):
+        if not any(r is not None and r.requires_grad for r in refs):
             return None
-        return torch.cat([r.to(self.device) for r in refs])
+        parts = [r.to(self.device) if r is not None else
+                 torch.zeros((n, self.dims), dtype=self.dtype, device=self.device)
+                 for r, n in zip(refs, self.nr_ref_points)]
+        return torch.cat(parts)
 
     def query(self, points_query: Sequence[torch.Tensor], nr_nns_searches: Union[int, Sequence[int]] = 1):
         """KNN of every cloud's queries in its own tree.
 
         Returns (dists, inds): lists with one [Q_c, k_c] tensor per cloud, squared distances and
         int64 original indices, as ``TorchKDTree.query`` returns them.  Gradients flow to the
-        queries and to the clouds given as tensors that require grad.
+        queries and to the clouds given as tensors that require grad (clouds given as arrays get
+        none).  The gradient of the reference points is computed at the coordinates the trees were
+        built from (the single-tree path uses the tensor's current values, ``points_ref_bak``).
         """
         if len(points_query) != len(self.nr_ref_points):
             raise ValueError("one query tensor per cloud is required")
@@ -176,7 +190,8 @@ class BatchStep:
     ``weights`` (one [Q_c, k_c] tensor per cloud) is the upstream gradient of the squared
     distances, i.e. the loss is sum_c sum(weights_c * dists_c).  After ``replay()`` the buffers
     ``dist``, ``idx``, ``orig``, ``grad_queries`` and ``grad_refs`` hold the results (flat, in the
-    plan's layout; ``split`` cuts a flat buffer per cloud).
+    plan's layout; ``split`` cuts a flat buffer per cloud).  Each step owns its plan and graph, so
+    several steps of equal shapes (double-buffered inputs, a train and an eval step) coexist.
     """
 
     def __init__(self, batch: KDTreeBatch, queries: Sequence[torch.Tensor], nr_nns_searches, weights=None):
@@ -186,7 +201,7 @@ class BatchStep:
         self.batch = batch
         self.nq = [q.shape[0] for q in queries]
         self.ks = ks
-        self.plan = batch.plan(self.nq, ks)
+        self.plan = batch.new_plan(self.nq, ks)
         total_q, total_out, total_ref, _ = self.plan.sizes()
         self.queries = torch.cat([q.to(dev) for q in queries]).detach().contiguous()
         self.dist = torch.empty(total_out, dtype=batch.dtype, device=dev)

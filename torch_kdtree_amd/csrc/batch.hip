@@ -145,6 +145,11 @@ BatchPlan* batch_create(const Tree* const* trees, int n_clouds, const int64_t* n
         if (t.dims != t0.dims || t.dtype != t0.dtype) throw std::runtime_error("batch: trees differ in dims or dtype");
         if (n_queries[c] < 0) throw std::runtime_error("batch: negative query count");
         if (ks[c] < 1 || ks[c] > kBatchMaxK) throw std::runtime_error("batch: k must be in [1, 32]");
+        if (tie == kTieReference && !t.finite)
+            throw std::runtime_error("batch: cloud " + std::to_string(c) +
+                                     " holds a non-finite coordinate or split value; the reference tie policy "
+                                     "follows the reference's own steps for it, on its own tree (query it with "
+                                     "build_kd_tree(...).query) or use the lower-index tie policy");
         if (ks[c] > t.n_points)
             throw std::runtime_error("You requested more nearest neighbors than there are in the KD-Tree");
     }
@@ -153,6 +158,7 @@ BatchPlan* batch_create(const Tree* const* trees, int n_clouds, const int64_t* n
     std::unique_ptr<BatchPlan, void (*)(BatchPlan*)> p(new BatchPlan(), [](BatchPlan* x) { batch_free(x); });
     p->device = device;
     p->dtype = t0.dtype;
+    for (int c = 0; c < n_clouds; ++c) p->all_finite = p->all_finite && trees[c]->finite;
     p->dims = t0.dims;
     p->tie = tie;
     p->n_clouds = n_clouds;

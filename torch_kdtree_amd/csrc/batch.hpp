@@ -50,6 +50,7 @@ struct BatchPlan {
     int device = -1;
     int dtype = 0, dims = 0, tie = 0;
     int n_clouds = 0;
+    bool all_finite = true;  // every cloud's coordinates and split values are finite
     std::vector<CloudDesc> clouds;
     int64_t total_q = 0, total_out = 0, total_ref = 0;
     int32_t total_buckets = 0;

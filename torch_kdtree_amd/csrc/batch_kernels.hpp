@@ -382,6 +382,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_grad_r_kernel(const unsigne
 template <typename T>
 TreeView<T, true> batch_arena_view(const BatchPlan& p) {
     TreeView<T, true> v{static_cast<const T*>(p.split), p.recs, p.leaf_off, p.shuffled, p.inv_shuffled, p.boxes, 0, 0};
+    v.split_bounds = p.all_finite;
     return v;
 }
 

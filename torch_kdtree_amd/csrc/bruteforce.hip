@@ -2,8 +2,10 @@
 // generalised to any supported d and run over the whole GPU instead of one block).
 //
 // Semantics: reference points are scanned in index order; a candidate enters the k-list iff
-// d < worst, at the upper-bound position; so ties go to the lower reference index and the
-// result is the k smallest (distance, index) pairs.  Distances use the CPU path's arithmetic
+// d < worst (src/nndistance_g.cu:57,126), at the upper-bound position; so ties go to the lower
+// reference index, a NaN distance never enters, and the result is the k smallest (distance,
+// index) pairs among the non-NaN distances.  Slots left empty (a query with a NaN coordinate, or
+// fewer than k non-NaN distances) hold (+inf, 0); the reference leaves their index unset.  Distances use the CPU path's arithmetic
 // (q - r per axis, squares summed left to right, no fused multiply-add).
 //
 // Layout: blockIdx.x picks a tile of kBfBlock queries (one per lane), blockIdx.y a chunk of
@@ -123,32 +125,6 @@ __global__ __launch_bounds__(256) void bf_merge_kernel(const T* __restrict__ par
     }
 }
 
-// Rows of queries with a NaN coordinate.  Every distance such a query forms is NaN, and the
-// reference's insertion (src/nndistance.hpp:77-85: rejected only when d >= worst, placed at the
-// upper bound found with `value >= *it`) takes each NaN and puts it in front, so scanning the
-// points in index order leaves (NaN, n-1), (NaN, n-2), ..., (NaN, n-k).  The kernels above never
-// insert a NaN; this pass rewrites those rows.
-template <typename T, int D>
-__global__ __launch_bounds__(256) void bf_nan_rows_kernel(const T* __restrict__ ref, int32_t n,
-                                                          const T* __restrict__ queries, int64_t nq, int k,
-                                                          T* __restrict__ out_d, int32_t* __restrict__ out_i) {
-    const int64_t qi = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (qi >= nq) return;
-    T q[D];
-    bool nan = false;
-#pragma unroll
-    for (int i = 0; i < D; ++i) {
-        q[i] = queries[qi * D + i];
-        nan |= __builtin_isnan(q[i]);
-    }
-    if (!nan) return;
-    for (int j = 0; j < k; ++j) {
-        const int32_t r = n - 1 - j;
-        out_d[qi * k + j] = sq_dist<T, D>(q, ref + (int64_t)r * D);
-        out_i[qi * k + j] = r;
-    }
-}
-
 struct BfPlan {
     int nchunks;
     int32_t chunk;
@@ -191,9 +167,6 @@ void bf_launch(const T* ref, int32_t n, const T* q, int64_t nq, int k, T* dist, 
                            p.nchunks, nq, k, dist, idx);
         TK_HIP_CHECK(hipGetLastError());
     }
-    hipLaunchKernelGGL((bf_nan_rows_kernel<T, D>), dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, ref, n, q, nq,
-                       k, dist, idx);
-    TK_HIP_CHECK(hipGetLastError());
 }
 
 template <typename T, int D>

@@ -193,7 +193,7 @@ void tk_tree_free(tk_tree* tree) {
     delete tree;
 }
 
-int tk_tree_info(const tk_tree* tree, int32_t out[5]) {
+int tk_tree_info(const tk_tree* tree, int32_t out[6]) {
     return guarded([&] {
         const tk::Tree& t = tree_of(tree);
         out[0] = t.dims;
@@ -201,6 +201,7 @@ int tk_tree_info(const tk_tree* tree, int32_t out[5]) {
         out[2] = t.levels;
         out[3] = t.n_points;
         out[4] = t.dev ? t.dev->device : -1;
+        out[5] = t.finite ? 1 : 0;
     });
 }
 

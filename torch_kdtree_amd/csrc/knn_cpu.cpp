@@ -10,7 +10,8 @@
 //     then the far leaf iff bound(q, pp, a, m) < worst with pp as it stands;
 //   * bound = sum_i v_i^2 with v = pp - q and v[a] += m - pp[a] (src/kdtree.hpp:606-614);
 //   * candidates are scanned in structured order, rejected when d >= worst, otherwise
-//     inserted at the upper-bound position (src/nndistance.hpp:41-85).
+//     inserted at the upper-bound position (src/nndistance.hpp:41-85).  A NaN distance fails
+//     both comparisons: it is taken, and placed by the binary search's path through the list.
 // The reference keeps a per-node tag array; here the state is the node index and the
 // child we came back from, which is all the tag array encodes on the active path.
 //
@@ -21,7 +22,9 @@
 // records in reverse order, with their NaN distances, then (+inf, 0) entries when the leaf holds
 // fewer than k points.  nan_query() writes that result directly.
 //
-// Extra pruning that keeps the reference's output: besides the reference's bound, a leaf or a
+// Extra pruning that keeps the reference's output (trees whose coordinates and split values are
+// all finite; on other trees a NaN distance can enter the list, a box says nothing about it, and
+// the search runs exactly the reference's steps): besides the reference's bound, a leaf or a
 // far side is tested against the distance from q to the bounding box of its points.  When
 // that box distance is >= worst, every point there has d >= worst and the reference would
 // reject all of them; its visit would then change nothing but pp, where it leaves pp[a] = q[a]
@@ -92,12 +95,13 @@ struct Searcher {
     const T* pts;
     const T* aabb;
     int k;
+    bool boxes;  // point-box pruning (t.finite)
 
     void scan(int32_t leaf, const T* q, T* bd, int32_t* bi) const {
         const int32_t beg = t.leaf_off[leaf], end = t.leaf_off[leaf + 1];
         for (int32_t s = beg; s < end; ++s) {
             const T d = sq_dist<T, D>(q, pts + (int64_t)s * D);
-            if (!(d < bd[k - 1])) continue;
+            if (d >= bd[k - 1]) continue;
             int lo = 0, cnt = k;  // upper bound: first position with bd[pos] > d
             while (cnt > 0) {
                 const int step = cnt / 2;
@@ -154,7 +158,7 @@ struct Searcher {
             }
         };
         auto scan_if_useful = [&](int32_t leaf) {
-            if (aabb_bound<T, D>(q, box(nn + leaf)) < bd[k - 1]) scan(leaf, q, bd, bi);
+            if (!boxes || aabb_bound<T, D>(q, box(nn + leaf)) < bd[k - 1]) scan(leaf, q, bd, bi);
         };
         descend();
         for (;;) {
@@ -192,7 +196,7 @@ struct Searcher {
                 pp[a] = m;
                 if (proj_bound<T, D>(q, pp, a, m) < bd[k - 1]) {
                     const int32_t far_child = 2 * nj + (lower ? 2 : 1);
-                    if (aabb_bound<T, D>(q, box(far_child)) < bd[k - 1]) {
+                    if (!boxes || aabb_bound<T, D>(q, box(far_child)) < bd[k - 1]) {
                         far_mask |= 1u << j;
                         node = far_child;
                         level = j + 1;
@@ -215,7 +219,7 @@ struct Searcher {
 template <typename T, int D>
 void query_typed(const Tree& t, const T* q, int64_t nq, int k, T* dist, int32_t* idx, int64_t* orig,
                  int n_threads) {
-    Searcher<T, D> s{t, t.split_ptr<T>(), t.points_ptr<T>(), t.aabb_ptr<T>(), k};
+    Searcher<T, D> s{t, t.split_ptr<T>(), t.points_ptr<T>(), t.aabb_ptr<T>(), k, t.finite};
     if (n_threads <= 0) n_threads = omp_get_max_threads();
 #pragma omp parallel num_threads(n_threads)
     {

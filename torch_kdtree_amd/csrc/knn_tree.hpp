@@ -35,6 +35,11 @@ struct TreeView {
     int32_t n_nodes;
     // batched only: where this tree's nodes, boxes, leaf offsets and records start in the arena
     int32_t node_base = 0, box_base = 0, leaf_base = 0, rec_base = 0;
+    // Every split value separates its node's points (true when all coordinates and split values
+    // are finite).  A NaN coordinate leaves std::sort's order unspecified and an overflowing median
+    // is +-inf, so the lower-index policy then prunes with point boxes only (they hold the actual
+    // extents of every subtree whatever the order).
+    bool split_bounds = true;
 
     __device__ __forceinline__ T split_at(int32_t node) const { return split[kBatched ? node_base + node : node]; }
     __device__ __forceinline__ int32_t leaf_begin(int32_t leaf) const {
@@ -212,6 +217,48 @@ struct KBestLds {
     }
 };
 
+// The reference's own list, for trees with non-finite values under the reference tie policy.  Held
+// in LDS like KBestLds.  A candidate is rejected only when d >= worst (src/nndistance.hpp:77) and
+// is placed by the reference's binary search (upperBound, src/nndistance.hpp:41-60), so a NaN
+// distance is taken and goes to the front, and the search's path through a list holding NaN
+// entries decides where each later value lands; a NaN reaching the last slot makes worst NaN,
+// after which every candidate is taken and every bound test fails, as in the reference.
+template <typename T>
+struct KBestExact {
+    T* d;
+    int32_t* s;
+    int k;
+    int stride;
+
+    __device__ __forceinline__ void init() {
+        for (int j = 0; j < k; ++j) {
+            d[j * stride] = pos_inf<T>();
+            s[j * stride] = 0;
+        }
+    }
+    __device__ __forceinline__ T worst() const { return d[(k - 1) * stride]; }
+    __device__ __forceinline__ bool accepts(T dist, int32_t) const { return !(dist >= worst()); }
+    __device__ __forceinline__ void insert(T dist, int32_t sidx, int32_t) {
+        int first = 0, count = k;
+        while (count > 0) {
+            const int step = count / 2;
+            if (dist >= d[(first + step) * stride]) {
+                first += step + 1;
+                count -= step + 1;
+            } else {
+                count = step;
+            }
+        }
+        if (first >= k) return;  // not reached after accepts(); kept as the reference keeps it
+        for (int j = k - 1; j > first; --j) {
+            d[j * stride] = d[(j - 1) * stride];
+            s[j * stride] = s[(j - 1) * stride];
+        }
+        d[first * stride] = dist;
+        s[first * stride] = sidx;
+    }
+};
+
 template <typename T, int D, typename List, bool kB>
 __device__ __forceinline__ void scan_leaf(const TreeView<T, kB>& tv, int32_t leaf, const Pt<T, D>& q, List& kb) {
     const int32_t beg = tv.leaf_begin(leaf);
@@ -233,6 +280,20 @@ __device__ __forceinline__ uint32_t axis_levels(int a) {
     return m;
 }
 
+// Pruning test of the lower-index policy: a side is skipped only when its bound exceeds worst.  A
+// NaN bound (a NaN split value of a tree with non-finite coordinates) never prunes.
+template <typename T>
+__device__ __forceinline__ bool lower_passes(T bound, T worst) {
+    return !(bound > worst);
+}
+
+// The lower-index policy's split-plane test: passes on trees whose splits do not bound their points.
+template <typename T, int D, bool kB>
+__device__ __forceinline__ bool lower_split_passes(const TreeView<T, kB>& tv, const Pt<T, D>& q, const Pt<T, D>& pp,
+                                                   int a, T m, T worst) {
+    return !tv.split_bounds || lower_passes(clean_bound<T, D>(q, pp, a, m), worst);
+}
+
 // Far side test for the child `far_child` of split (a, m).  Reference policy: the reference's own
 // test (proj_bound < worst) decides whether the reference would look at the far side; the
 // distance to the box of the points there then tells whether that look could change anything:
@@ -244,8 +305,8 @@ template <typename T, int D, bool kLower, bool kB>
 __device__ __forceinline__ int far_side(const TreeView<T, kB>& tv, const Pt<T, D>& q, const Pt<T, D>& pp, int a, T m,
                                        int32_t far_child, T worst) {
     if constexpr (kLower) {
-        if (!(clean_bound<T, D>(q, pp, a, m) <= worst)) return 0;
-        return aabb_bound<T, D>(tv.boxes, tv.box_of(far_child), q) <= worst ? 1 : 0;
+        if (!lower_split_passes<T, D>(tv, q, pp, a, m, worst)) return 0;
+        return lower_passes(aabb_bound<T, D>(tv.boxes, tv.box_of(far_child), q), worst) ? 1 : 0;
     } else {
         if (!(proj_bound<T, D>(q, pp, a, m) < worst)) return 0;
         return aabb_bound<T, D>(tv.boxes, tv.box_of(far_child), q) < worst ? 1 : 2;
@@ -257,7 +318,7 @@ __device__ __forceinline__ int far_side(const TreeView<T, kB>& tv, const Pt<T, D
 template <typename T, int D, bool kLower, bool kB>
 __device__ __forceinline__ bool leaf_useful(const TreeView<T, kB>& tv, const Pt<T, D>& q, int32_t leaf, T worst) {
     const T b = aabb_bound<T, D>(tv.boxes, tv.box_of(tv.n_nodes + leaf), q);
-    return kLower ? b <= worst : b < worst;
+    return kLower ? lower_passes(b, worst) : b < worst;
 }
 
 // Run-time axis access.  The element is taken from / put into a copy of the whole vector, so
@@ -394,8 +455,8 @@ __device__ __forceinline__ void lane_descend(const TreeView<T, kB>& tv, const Pt
 // its descent reaches (a NaN coordinate never compares below a split) and prunes the rest.  Under
 // the reference policy the list is that leaf's last k records in reverse order, with their
 // distances, then the untouched (+inf, 0) entries when the leaf holds fewer than k points.  The
-// lower-index policy gives what the brute force gives (every point in index order, each to the
-// front): original indices n-1, n-2, ..., n-k.
+// lower-index policy gives what the brute force gives (src/nndistance_g.cu:57 inserts only when
+// d < worst, which a NaN never is): the empty list, (+inf, original index 0) in every slot.
 template <typename T, int D>
 __device__ __forceinline__ bool has_nan(const Pt<T, D>& q) {
     bool r = false;
@@ -419,33 +480,20 @@ __device__ __forceinline__ void descent_leaf(const TreeView<T, kB>& tv, const Pt
     end = tv.leaf_end(leaf);
 }
 
-// Output slot c (0 = nearest) of a NaN query: record position (arena-wide), key, whether filled.
-template <typename T, int D, bool kLower, bool kB>
-__device__ __forceinline__ bool nan_slot(const TreeView<T, kB>& tv, int32_t beg, int32_t end, int c,
-                                         int32_t& rec, int32_t& key) {
-    if constexpr (kLower) {
-        const int32_t first = tv.leaf_begin(0);
-        const int32_t n = tv.leaf_end(tv.n_nodes) - first;
-        key = n - 1 - c;  // original index within the tree (k <= n)
-        rec = first + load_i32_at(tv.inv_shuffled, first + key);
-        return true;
-    } else {
-        rec = end - 1 - c;
-        key = rec;
-        return rec >= beg;
-    }
-}
-
+// The list of a NaN query under the reference policy, from an initialised list: slot c (0 =
+// nearest) holds record end-1-c of the descent's leaf while that is in the leaf.  Under the
+// lower-index policy the initialised list is already the result.
 template <typename T, int D, int K, bool kLower, bool kB>
 __device__ __forceinline__ void nan_fill(const TreeView<T, kB>& tv, const Pt<T, D>& q, int k, KBest<T, K, kLower>& kb) {
-    int32_t beg = 0, end = 0;
-    if (!kLower) descent_leaf<T, D, kB>(tv, q, beg, end);
-    const int pad = K - k;
+    if constexpr (!kLower) {
+        int32_t beg = 0, end = 0;
+        descent_leaf<T, D, kB>(tv, q, beg, end);
+        const int pad = K - k;
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
-        int32_t rec, key;
-        if (j >= pad && nan_slot<T, D, kLower, kB>(tv, beg, end, j - pad, rec, key))
-            kb.set(j, sq_dist<T, D>(q, load_rec<T, D>(tv.recs, rec).c), key);
+        for (int j = 0; j < K; ++j) {
+            const int32_t rec = end - 1 - (j - pad);
+            if (j >= pad && rec >= beg) kb.set(j, sq_dist<T, D>(q, load_rec<T, D>(tv.recs, rec).c), rec);
+        }
     }
 }
 
@@ -481,7 +529,7 @@ __device__ __forceinline__ bool lane_step(const TreeView<T, kB>& tv, const Pt<T,
 #pragma nounroll
         for (int side = 0; side < 2; ++side) {
             const int32_t leaf = side ? far_leaf : near_leaf;
-            const bool looks = side == 0 || (kLower ? clean_bound<T, D>(q, pp, a_lp, m) <= kb.worst()
+            const bool looks = side == 0 || (kLower ? lower_split_passes<T, D>(tv, q, pp, a_lp, m, kb.worst())
                                                     : proj_bound<T, D>(q, pp, a_lp, m) < kb.worst());
             if (looks && leaf_useful<T, D, kLower>(tv, q, leaf, kb.worst()))
                 scan_leaf_staged<T, D, K, kLower, CH, LD>(tv, leaf, q, kb, st_d, st_o);
@@ -504,7 +552,7 @@ __device__ __forceinline__ bool lane_step(const TreeView<T, kB>& tv, const Pt<T,
                     const int32_t c = ((node + 1) >> (level - 1 - j)) - 1;
                     const int32_t fc = ((c - 1) ^ 1) + 1;
                     const T b = aabb_bound<T, D>(tv.boxes, tv.box_of(fc), q);
-                    if (kLower ? b <= kb.worst() : b < kb.worst()) {
+                    if (kLower ? lower_passes(b, kb.worst()) : b < kb.worst()) {
                         ax_set(pp, axis_of_level<D>(j), tv.split_at((c - 1) >> 1));
                         far_mask = (far_mask & ((1u << j) - 1u)) | (1u << j);
                         node = fc;
@@ -527,7 +575,7 @@ __device__ __forceinline__ bool lane_step(const TreeView<T, kB>& tv, const Pt<T,
                 const int32_t c2 = ((node + 1) >> (level - 1 - j2)) - 1;
                 const BoxVals<T, D> bx2 = load_box<T, D>(tv.boxes, tv.box_of(((c2 - 1) ^ 1) + 1));
                 const T b = box_dist<T, D>(bx, q);
-                if (kLower ? b <= kb.worst() : b < kb.worst()) {
+                if (kLower ? lower_passes(b, kb.worst()) : b < kb.worst()) {
                     ax_set(pp, axis_of_level<D>(j), tv.split_at((c - 1) >> 1));
                     far_mask = (far_mask & ((1u << j) - 1u)) | (1u << j);
                     node = ((c - 1) ^ 1) + 1;
@@ -584,9 +632,11 @@ __device__ __forceinline__ bool lane_step(const TreeView<T, kB>& tv, const Pt<T,
 }
 
 // Reference-order search with a generic list (used with the LDS list for large k): the same walk
-// as lane_start/lane_step, one node at a time.
-template <typename T, int D, bool kLower, typename List, bool kB>
+// as lane_start/lane_step, one node at a time.  kBoxes false: no point-box tests, so the walk is
+// the reference's own (src/kdtree.cpp:16-110), step for step (trees with non-finite values).
+template <typename T, int D, bool kLower, bool kBoxes = true, typename List, bool kB>
 __device__ __forceinline__ void knn_search_one(const TreeView<T, kB>& tv, const Pt<T, D>& q, List& kb) {
+    static_assert(kBoxes || !kLower, "the lower-index policy always prunes with boxes");
     Pt<T, D> pp = q;
     const int L = tv.levels;
     const int32_t nn = tv.n_nodes;
@@ -605,10 +655,11 @@ __device__ __forceinline__ void knn_search_one(const TreeView<T, kB>& tv, const 
             const bool lower = qa < m;
             const int32_t near_leaf = 2 * node + (lower ? 1 : 2) - nn;
             const int32_t far_leaf = 2 * node + (lower ? 2 : 1) - nn;
-            if (leaf_useful<T, D, kLower>(tv, q, near_leaf, kb.worst())) scan_leaf<T, D>(tv, near_leaf, q, kb);
-            if ((kLower ? clean_bound<T, D>(q, pp, a, m) <= kb.worst()
+            if (!kBoxes || leaf_useful<T, D, kLower>(tv, q, near_leaf, kb.worst()))
+                scan_leaf<T, D>(tv, near_leaf, q, kb);
+            if ((kLower ? lower_split_passes<T, D>(tv, q, pp, a, m, kb.worst())
                         : proj_bound<T, D>(q, pp, a, m) < kb.worst()) &&
-                leaf_useful<T, D, kLower>(tv, q, far_leaf, kb.worst()))
+                (!kBoxes || leaf_useful<T, D, kLower>(tv, q, far_leaf, kb.worst())))
                 scan_leaf<T, D>(tv, far_leaf, q, kb);
             if (level == 0) return;
             ax_set(pp, a, qa);
@@ -625,7 +676,8 @@ __device__ __forceinline__ void knn_search_one(const TreeView<T, kB>& tv, const 
             if (child == 2 * node + (lower2 ? 1 : 2)) {
                 const int32_t far_child = 2 * node + (lower2 ? 2 : 1);
                 ax_set(pp, a, m2);
-                const int f = far_side<T, D, kLower>(tv, q, pp, a, m2, far_child, kb.worst());
+                const int f = kBoxes ? far_side<T, D, kLower>(tv, q, pp, a, m2, far_child, kb.worst())
+                                     : (proj_bound<T, D>(q, pp, a, m2) < kb.worst() ? 1 : 0);
                 if (f == 1) {
                     node = far_child;
                     ++level;
@@ -784,14 +836,14 @@ __global__ __launch_bounds__(kLdsBlock) void knn_tree_lds_kernel(TreeView<T> tv,
     Pt<T, D> q;
     load_query<T, D>(queries, qi, q);
     if (has_nan<T, D>(q)) {
-        int32_t beg = 0, end = 0;
-        if (!kLower) descent_leaf<T, D>(tv, q, beg, end);
-        for (int c = 0; c < k; ++c) {
-            int32_t rec, key;
-            if (!nan_slot<T, D, kLower>(tv, beg, end, c, rec, key)) break;
-            kb.d[c * kLdsBlock] = sq_dist<T, D>(q, load_rec<T, D>(tv.recs, rec).c);
-            kb.s[c * kLdsBlock] = rec;
-            if constexpr (kLower) kb.o[c * kLdsBlock] = key;
+        // the reference policy's NaN result (nan_fill); the lower-index list stays empty
+        if constexpr (!kLower) {
+            int32_t beg = 0, end = 0;
+            descent_leaf<T, D>(tv, q, beg, end);
+            for (int c = 0; c < k && end - 1 - c >= beg; ++c) {
+                kb.d[c * kLdsBlock] = sq_dist<T, D>(q, load_rec<T, D>(tv.recs, end - 1 - c).c);
+                kb.s[c * kLdsBlock] = end - 1 - c;
+            }
         }
     } else {
         knn_search_one<T, D, kLower>(tv, q, kb);
@@ -802,6 +854,36 @@ __global__ __launch_bounds__(kLdsBlock) void knn_tree_lds_kernel(TreeView<T> tv,
         if (out_o) out_o[qi * k + j] = kLower ? kb.o[j * kLdsBlock] : tv.shuffled[kb.s[j * kLdsBlock]];
     }
 }
+
+// Reference tie policy on a tree with a non-finite coordinate or split value: the reference's walk
+// step for step (no point boxes) with the reference's list (KBestExact), one lane per query.
+// Queries with a NaN coordinate need no special case here.
+template <typename T, int D>
+__global__ __launch_bounds__(kLdsBlock) void knn_tree_exact_kernel(TreeView<T> tv, const T* __restrict__ queries,
+                                                                    int64_t nq, const int32_t* __restrict__ order,
+                                                                    int k, T* __restrict__ out_d,
+                                                                    int32_t* __restrict__ out_i,
+                                                                    int64_t* __restrict__ out_o) {
+    extern __shared__ unsigned char smem[];
+    const int64_t t = xcd_tile(blockIdx.x, gridDim.x) * kLdsBlock + threadIdx.x;
+    if (t >= nq) return;
+    const int64_t qi = order ? (int64_t)order[t] : t;
+    T* sd = reinterpret_cast<T*>(smem);
+    int32_t* ss = reinterpret_cast<int32_t*>(sd + (size_t)k * kLdsBlock);
+    KBestExact<T> kb{sd + threadIdx.x, ss + threadIdx.x, k, kLdsBlock};
+    kb.init();
+    Pt<T, D> q;
+    load_query<T, D>(queries, qi, q);
+    knn_search_one<T, D, false, false>(tv, q, kb);
+    for (int j = 0; j < k; ++j) {
+        const int32_t sj = kb.s[j * kLdsBlock];
+        out_d[qi * k + j] = kb.d[j * kLdsBlock];
+        out_i[qi * k + j] = sj;
+        if (out_o) out_o[qi * k + j] = tv.shuffled[sj];
+    }
+}
+
+inline size_t exact_list_bytes(int k, size_t tsize) { return (size_t)k * kLdsBlock * (tsize + 4); }
 
 inline size_t lds_list_bytes(int k, size_t tsize, bool lower) {
     return (size_t)k * kLdsBlock * (tsize + 4 + (lower ? 4 : 0));

@@ -9,8 +9,10 @@ namespace tk {
 
 template <typename T>
 TreeView<T> tree_view(const Tree& t) {
-    return TreeView<T>{static_cast<const T*>(t.dev->split), t.dev->recs, t.dev->leaf_off, t.dev->shuffled,
-                       t.dev->inv_shuffled, t.dev->boxes, t.levels, t.n_nodes};
+    TreeView<T> v{static_cast<const T*>(t.dev->split), t.dev->recs, t.dev->leaf_off, t.dev->shuffled,
+                  t.dev->inv_shuffled, t.dev->boxes, t.levels, t.n_nodes};
+    v.split_bounds = t.finite;
+    return v;
 }
 
 template <typename T, int D, bool kLower>
@@ -46,13 +48,27 @@ void launch_tree_td(const TreeView<T>& tv, const T* q, int64_t nq, const int32_t
     TK_HIP_CHECK(hipGetLastError());
 }
 
+// The reference's own walk and list (knn_tree_exact_kernel), for trees holding a non-finite value.
+template <typename T, int D>
+void launch_tree_exact(const TreeView<T>& tv, const T* q, int64_t nq, const int32_t* order, int k, T* dist,
+                       int32_t* idx, int64_t* orig, hipStream_t s) {
+    const size_t shmem = exact_list_bytes(k, sizeof(T));
+    const void* fn = reinterpret_cast<const void*>(&knn_tree_exact_kernel<T, D>);
+    if (shmem > 65536) TK_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    hipLaunchKernelGGL((knn_tree_exact_kernel<T, D>), dim3((unsigned)((nq + kLdsBlock - 1) / kLdsBlock)),
+                       dim3(kLdsBlock), shmem, s, tv, q, nq, order, k, dist, idx, orig);
+    TK_HIP_CHECK(hipGetLastError());
+}
+
 template <typename T>
 void launch_tree_search_t(const Tree& t, const T* q, int64_t nq, const int32_t* order, int k, T* dist, int32_t* idx,
                           int64_t* orig, int tie, hipStream_t s) {
     const TreeView<T> tv = tree_view<T>(t);
 #define TK_DIMS(DD)                                                            \
     case DD:                                                                   \
-        if (tie == kTieLowerIndex)                                             \
+        if (tie != kTieLowerIndex && !t.finite)                                \
+            launch_tree_exact<T, DD>(tv, q, nq, order, k, dist, idx, orig, s); \
+        else if (tie == kTieLowerIndex)                                        \
             launch_tree_td<T, DD, true>(tv, q, nq, order, k, dist, idx, orig, s); \
         else                                                                   \
             launch_tree_td<T, DD, false>(tv, q, nq, order, k, dist, idx, orig, s); \

@@ -60,7 +60,7 @@ struct KDTree {
     }
     // Adopts a built or loaded tree (uploading it for GPU classes unless it is on the GPU already).
     KDTree(tk_tree* loaded, int device, bool uploaded = false) : tree(loaded) {
-        int32_t info[5];
+        int32_t info[6];
         check(tk_tree_info(tree, info));
         levels = info[2];
         if (kGpu && !uploaded) {
@@ -88,8 +88,16 @@ struct KDTree {
         check(rc);
     }
 
+    // Every coordinate and split value finite (otherwise GPU queries under the reference tie
+    // policy run the reference's own steps, without point-box pruning).
+    bool is_finite() const {
+        int32_t info[6];
+        check(tk_tree_info(tree, info));
+        return info[5] != 0;
+    }
+
     int32_t n_points() const {
-        int32_t info[5];
+        int32_t info[6];
         check(tk_tree_info(tree, info));
         return info[3];
     }
@@ -107,7 +115,7 @@ struct KDTree {
     }
 
     py::tuple get_nodes() const {
-        int32_t info[5];
+        int32_t info[6];
         check(tk_tree_info(tree, info));
         const int L = info[2];
         const py::ssize_t nodes = (py::ssize_t(1) << L) - 1;
@@ -207,7 +215,7 @@ struct TreeSet {
     std::vector<int> info(int field) const {
         std::vector<int> out;
         for (tk_tree* t : trees) {
-            int32_t v[5];
+            int32_t v[6];
             check(tk_tree_info(t, v));
             out.push_back(v[field]);
         }
@@ -297,6 +305,7 @@ void bind_tree(py::module_& m, const char* name) {
         .def("get_structured_points", &C::get_structured_points,
              "Returns the ordered points how they are used in the KD-Tree")
         .def("get_nodes", &C::get_nodes, "Returns (axis[int32], split[T], leaf_offsets[int32]) of the tree")
+        .def("is_finite", &C::is_finite, "True when every coordinate and split value of the tree is finite")
         .def("query", &C::query, py::arg("points_query_ptr"), py::arg("nr_query_points"), py::arg("nr_nns_searches"),
              py::arg("dist_arr_ptr"), py::arg("knn_idx_ptr"), py::kw_only(), py::arg("orig_idx_ptr") = 0,
              py::arg("tie") = 0, py::arg("presort") = -1, py::arg("stream") = 0, py::arg("n_threads") = 0,
@@ -367,7 +376,7 @@ PYBIND11_MODULE(torch_knn, m) {
                 rc = tk_tree_load(path.c_str(), &t);
             }
             check(rc);
-            int32_t info[5];
+            int32_t info[6];
             if (tk_tree_info(t, info) != 0) {
                 tk_tree_free(t);
                 throw std::runtime_error(tk_last_error());

@@ -41,6 +41,10 @@ struct Tree {
     // (leaf j is heap node n_nodes + j): [(n_nodes + n_leaves) * 2 * dims] values of type T,
     // per node lo[dims] then hi[dims]; an empty leaf has lo = +inf, hi = -inf.
     std::vector<unsigned char> aabb;
+    // Every coordinate and every split value is finite.  Otherwise a distance or a pruning bound
+    // can be NaN, and the reference's insertion (src/nndistance.hpp:41-85) takes a NaN distance
+    // into the list; the searches then follow the reference step by step, without point boxes.
+    bool finite = true;
     DeviceTree* dev = nullptr;          // GPU copy, owned
 
     template <typename T> T* split_ptr() { return reinterpret_cast<T*>(split.data()); }

@@ -14,6 +14,7 @@
 // Subtrees are independent, so large ones are built as OpenMP tasks; the output does not
 // depend on the thread count.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -81,6 +82,9 @@ struct Builder {
 };
 
 // Point bounding boxes: leaves from their points, internal nodes as the union of their children.
+// NaN coordinates are left out of the boxes (a NaN distance is never below a bound, and the
+// searches that meet one, on trees that are not `finite`, use no boxes under the reference tie
+// policy).  Also sets t.finite: every coordinate and every split value finite.
 template <typename T, int D>
 void compute_aabb(Tree& t) {
     const int32_t nn = t.n_nodes, nl = t.n_leaves;
@@ -88,7 +92,8 @@ void compute_aabb(Tree& t) {
     T* box = reinterpret_cast<T*>(t.aabb.data());
     const T* pts = t.points_ptr<T>();
     const T inf = std::numeric_limits<T>::infinity();
-#pragma omp parallel for schedule(static)
+    bool finite = true;
+#pragma omp parallel for schedule(static) reduction(&& : finite)
     for (int32_t j = 0; j < nl; ++j) {
         T* b = box + (size_t)(nn + j) * 2 * D;
         for (int a = 0; a < D; ++a) {
@@ -97,8 +102,11 @@ void compute_aabb(Tree& t) {
         }
         for (int32_t s = t.leaf_off[j]; s < t.leaf_off[j + 1]; ++s)
             for (int a = 0; a < D; ++a) {
-                b[a] = std::min(b[a], pts[(size_t)s * D + a]);
-                b[D + a] = std::max(b[D + a], pts[(size_t)s * D + a]);
+                const T v = pts[(size_t)s * D + a];
+                finite = finite && std::isfinite(v);
+                if (std::isnan(v)) continue;
+                b[a] = std::min(b[a], v);
+                b[D + a] = std::max(b[D + a], v);
             }
     }
     for (int32_t i = nn - 1; i >= 0; --i) {
@@ -110,6 +118,9 @@ void compute_aabb(Tree& t) {
             b[D + a] = std::max(l[D + a], r[D + a]);
         }
     }
+    const T* split = t.split_ptr<T>();
+    for (int32_t i = 0; i < nn && finite; ++i) finite = std::isfinite(split[i]);
+    t.finite = finite;
 }
 
 template <typename T, int D>

@@ -14,9 +14,12 @@
 //
 // Exactness: the node sizes, split values and leaf contents equal those of the reference
 // whenever no node has two points with equal coordinates on its split axis.  With such ties the
-// reference's two std::sort calls order the tied points in an unspecified way (and may even order
-// the point array and the index array differently); this build orders them by their previous
-// position.  It is therefore opt-in; the default build stays the host restatement.
+// reference's two std::sort calls (one permutation, applied to both arrays: std::sort's moves
+// follow its comparison outcomes alone) order the tied points in the order libstdc++'s introsort
+// leaves them, which only a sequential emulation could follow; this build orders them by their
+// previous position.  It is therefore opt-in; the default build stays the host restatement.
+// Points with a NaN coordinate are refused: std::sort's order for them is the introsort's
+// accident too.  The host build accepts them and reproduces the reference.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -339,7 +342,10 @@ std::vector<Tree*> build_forest_typed(const T* pts, const int64_t* n_points, con
     TK_HIP_CHECK(hipMemcpyAsync(pts_h.data(), d_out_pts.p, sizeof(T) * pts_h.size(), hipMemcpyDeviceToHost, s));
     TK_HIP_CHECK(hipMemcpyAsync(idx_h.data(), d_out_idx.p, sizeof(int32_t) * idx_h.size(), hipMemcpyDeviceToHost, s));
     TK_HIP_CHECK(hipStreamSynchronize(s));
-    if (nan_flag) throw std::runtime_error("GPU tree build: the points contain NaN");
+    if (nan_flag)
+        throw std::runtime_error(
+            "GPU tree build: the points contain NaN (the host build, builder=\"cpu\", accepts them and orders "
+            "them as the reference does)");
 
     std::vector<Tree*> out(nc, nullptr);
     try {
