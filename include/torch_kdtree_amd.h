@@ -145,6 +145,18 @@ int tk_knn_query_gpu(const tk_tree* tree, const void* queries, int64_t n_queries
                      int64_t* orig_idx, int tie, int presort, void* workspace, size_t workspace_bytes, void* stream);
 int tk_knn_query_gpu_workspace(const tk_tree* tree, int64_t n_queries, int k, int tie, int presort, size_t* bytes);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  Same arguments,
+ * checks and results as tk_knn_query_gpu with workspace == NULL, but the presort workspace is a
+ * static device array of 64 MiB per GPU (allocated with the library's code object and held until
+ * the process ends) instead of an allocation: no hipMalloc, hipFree or allocating library call.
+ * At most TK_NOALLOC_MAX_QUERIES queries per call; more fail with an error.  Every call on one GPU
+ * shares that GPU's array, so calls that overlap in time on one GPU (different streams without
+ * ordering, or several host threads) race: serialise them. */
+#define TK_NOALLOC_MAX_QUERIES 2097152
+int tk_knn_query_gpu_noalloc(const tk_tree* tree, const void* queries, int64_t n_queries, int k, void* dist,
+                             int32_t* idx, int64_t* orig_idx, int tie, int presort, void* stream);
+
 /* Replaces NmDistanceKernelLauncher[Dynamic] (src/nndistance_g.cu:142-220), which the
  * reference compiles but never binds.  Exhaustive KNN over ref[n_ref*dims] in original
  * order; ties go to the lower index.  idx receives original int32 indices.  k <= 32. */

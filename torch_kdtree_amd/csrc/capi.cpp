@@ -250,6 +250,18 @@ int tk_knn_query_gpu(const tk_tree* tree, const void* queries, int64_t n_queries
     });
 }
 
+int tk_knn_query_gpu_noalloc(const tk_tree* tree, const void* queries, int64_t n_queries, int k, void* dist,
+                             int32_t* idx, int64_t* orig_idx, int tie, int presort, void* stream) {
+    static_assert(TK_NOALLOC_MAX_QUERIES == tk::kNoAllocMaxQueries, "header and library disagree on the limit");
+    return guarded([&] {
+        const tk::Tree& t = tree_of(tree);
+        tk::QueryOpts o;
+        o.tie = tie;
+        o.presort = presort;
+        tk::query_gpu_noalloc(t, queries, n_queries, k, dist, idx, orig_idx, o, static_cast<hipStream_t>(stream));
+    });
+}
+
 int tk_knn_query_gpu_workspace(const tk_tree* tree, int64_t n_queries, int k, int tie, int presort, size_t* bytes) {
     return guarded([&] {
         (void)k;

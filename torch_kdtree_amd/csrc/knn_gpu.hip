@@ -468,4 +468,49 @@ void query_gpu(const Tree& t, const void* queries, int64_t nq, int k, void* dist
                                orig, o.tie, stream);
 }
 
+// ------------------------------------------------------------------------------------------
+// The query that allocates nothing: its presort workspace is a static device array.  `scratch`
+// below names the host-side stand-in of that array; the device address of each GPU's copy comes
+// from hipGetSymbolAddress, taken once per device on the first call there (with that device
+// current), and only that address is ever handed to kernels.
+// ------------------------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on the current device.
+void* scratch_on_current_device() {
+    int dev = -1;
+    TK_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) throw std::runtime_error("no-allocation query: device out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw std::runtime_error(std::string("no-allocation query: hipGetSymbolAddress failed: ") +
+                                     hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+void query_gpu_noalloc(const Tree& t, const void* queries, int64_t nq, int k, void* dist, int32_t* idx, int64_t* orig,
+                       const QueryOpts& o, hipStream_t stream) {
+    if (!t.dev) throw std::runtime_error("The tree has no GPU copy (it was built for the CPU)");
+    if (nq > kNoAllocMaxQueries)
+        throw std::runtime_error("no-allocation query: " + std::to_string(nq) + " queries exceed the limit of " +
+                                 std::to_string(kNoAllocMaxQueries) + " per call (static scratch of " +
+                                 std::to_string(kNoAllocScratchBytes >> 20) + " MiB per GPU)");
+    const size_t need = query_gpu_workspace(t, nq, o);
+    if (need > kNoAllocScratchBytes) throw std::runtime_error("no-allocation query: workspace exceeds the static scratch");
+    DeviceGuard g(t.dev->device);
+    void* ws = need ? scratch_on_current_device() : nullptr;
+    query_gpu(t, queries, nq, k, dist, idx, orig, o, stream, ws, need ? kNoAllocScratchBytes : 0);
+}
+
 }  // namespace tk

@@ -48,6 +48,14 @@ size_t query_gpu_workspace(const Tree& t, int64_t nq, const QueryOpts& o);
 void query_gpu(const Tree& t, const void* queries, int64_t nq, int k, void* dist, int32_t* idx, int64_t* orig,
                const QueryOpts& o, hipStream_t stream, void* workspace, size_t workspace_bytes);
 
+// query_gpu with the presort workspace taken from a static device array instead of an allocation
+// (knn_gpu.hip): at most kNoAllocMaxQueries queries per call; calls that overlap on one GPU share
+// the array and race.
+constexpr int64_t kNoAllocMaxQueries = int64_t(1) << 21;
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+void query_gpu_noalloc(const Tree& t, const void* queries, int64_t nq, int k, void* dist, int32_t* idx, int64_t* orig,
+                       const QueryOpts& o, hipStream_t stream);
+
 // Per-dtype kernel launchers (knn_tree_f32.hip / knn_tree_f64.hip).
 void launch_tree_search_f32(const Tree& t, const float* q, int64_t nq, const int32_t* order, int k, float* dist,
                             int32_t* idx, int64_t* orig, int tie, hipStream_t s);
