@@ -168,8 +168,11 @@ int tk_knn_bruteforce_gpu_workspace(int32_t n_ref, int64_t n_queries, int dims, 
  * original-order queries[nq*dims], ref[n_ref*dims], neighbour indices idx[nq*k] (original,
  * int64) and the upstream gradient grad_dist[nq*k] of the squared distances, writes
  * grad_queries[nq*dims] and grad_ref[n_ref*dims] (either may be NULL).  grad_ref is
- * deterministic: no atomics, a fixed summation order.  n_queries * k <= 2^28 (the pairs are
- * ordered by one radix sort). */
+ * deterministic: no atomics, a fixed summation order (each reference point's terms subtracted in
+ * a canonical query order).  grad_queries alone has no size limit and needs no workspace;
+ * grad_ref takes at most 2^28 queries, any k: its (reference, slot) pairs are sorted in slices of
+ * at most 2^28 pairs, consecutive in the canonical order, each slice continuing the sums of the
+ * previous ones, which gives the same bits as one pass. */
 int tk_knn_backward_gpu(const void* queries, const void* ref, int64_t n_queries, int32_t n_ref, int dims, int dtype,
                         int k, const int64_t* idx, const void* grad_dist, void* grad_queries, void* grad_ref,
                         void* workspace, size_t workspace_bytes, void* stream);

@@ -1,5 +1,6 @@
 """GPU parity on the benchmark workloads at their full sizes (trees and query sets of the real
-shape), against the oracle on the same trees: bit-exact distances and indices."""
+shape), against the oracle on the same trees: bit-exact distances and indices.  The plain GPU
+workload's 16,000,000 queries and the batching workload are in test_gpu_fullsize.py."""
 import numpy as np
 import pytest
 import torch
@@ -18,15 +19,6 @@ def _check(tree, queries, k, cuda, what):
     assert_same(gi, oi, what + " structured indices")
     sh = tree.kdtree.get_shuffled_inds()
     assert_same(go, sh[oi].astype(np.int64), what + " original indices")
-
-
-def test_plain_gpu_full_tree_sample(cuda):
-    """4,000,000-point tree of the plain GPU workload (auto depth), 65,536 of its queries."""
-    w = workloads.plain_gpu()
-    half = w["halves"][0]
-    q = np.concatenate([w["queries"][:32768], w["queries"][half:half + 32768]])
-    tree = tk.build_kd_tree(torch.from_numpy(w["ref"]), device=cuda, levels="auto")
-    _check(tree, q, 16, cuda, "plain_gpu")
 
 
 def test_differentiation_workload_full(cuda):

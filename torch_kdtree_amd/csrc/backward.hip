@@ -18,6 +18,13 @@
 // canonical order, and one lane per reference point subtracts its run in that order.  Spatially close queries
 // are close in the canonical order, so a run's reads are close together.  The result is
 // bit-identical from run to run, and equal to the batched backward's on the same cloud.
+//
+// Size: the pairs of one sort are at most 2^28 (the radix sort's limit).  With more, the canonical
+// order is cut into consecutive slices of at most 2^28 / k queries, and each slice's runs are
+// subtracted from the sums the previous slices left (the first slice starts from zero).  Every
+// reference point's terms are then subtracted in the same canonical order as with one slice, one
+// after another into the same accumulator, so the sums are bit-identical to an unsliced pass.
+// dL/dq alone needs no sort and no workspace, at any size.
 #include <algorithm>
 #include <memory>
 
@@ -36,22 +43,22 @@ namespace {
 #endif
 constexpr int kBwUnroll = TK_BW_UNROLL;  // loads issued ahead per lane
 
-// dL/dq, terms added in neighbour order.  Without q_sorted one lane per query row.  With q_sorted
-// (for dL/dr) lane t takes the query i = order[t] of canonical position t, so that neighbouring
-// lanes gather neighbouring reference rows, and also copies the query to q_sorted[t] and, per
-// neighbour j, the upstream gradient and the pair (reference index, slot) to the canonical slot
-// t * k + j, written contiguously across lanes.
+// dL/dq, terms added in neighbour order.  Without q_sorted one lane per query row (nq rows).  With
+// q_sorted (for dL/dr) lane t takes the query i = order[t0 + t] of canonical position t0 + t (a
+// slice of nq positions), so that neighbouring lanes gather neighbouring reference rows, and also
+// copies the query to q_sorted[t] and, per neighbour j, the upstream gradient and the pair
+// (reference index, slot) to the slice's slot t * k + j, written contiguously across lanes.
 template <typename T, int D>
 __global__ __launch_bounds__(256) void grad_q_kernel(const T* __restrict__ q, const T* __restrict__ ref, int64_t nq,
                                                      int k, const int64_t* __restrict__ idx,
                                                      const T* __restrict__ g, const uint32_t* __restrict__ order,
-                                                     T* __restrict__ dq, T* __restrict__ q_sorted,
+                                                     int64_t t0, T* __restrict__ dq, T* __restrict__ q_sorted,
                                                      T* __restrict__ g_sorted, uint32_t* __restrict__ keys,
                                                      uint32_t* __restrict__ vals) {
     const int64_t lane = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (lane >= nq) return;
     const int64_t t = q_sorted ? lane : 0;
-    const int64_t i = q_sorted ? (int64_t)order[lane] : lane;
+    const int64_t i = q_sorted ? (int64_t)order[t0 + lane] : lane;
     T qi[D], acc[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
@@ -128,14 +135,15 @@ __global__ __launch_bounds__(256) void run_bounds_kernel(const uint32_t* __restr
     if (p == m - 1 || keys[p + 1] != key) run_end[key] = (uint32_t)(p + 1);
 }
 
-// dL/dr, one lane per reference point: its run of sorted pairs, subtracted in canonical order.
+// dL/dr, one lane per reference point: its run of sorted pairs, subtracted in canonical order,
+// from zero (the first slice) or from the sums in dr (later slices).
 template <typename T, int D>
 __global__ __launch_bounds__(256) void grad_r_kernel(const T* __restrict__ ref, int32_t n, int k,
                                                      const uint32_t* __restrict__ run_start,
                                                      const uint32_t* __restrict__ run_end,
                                                      const uint32_t* __restrict__ vals,
                                                      const T* __restrict__ q_sorted, const T* __restrict__ g_sorted,
-                                                     T* __restrict__ dr) {
+                                                     bool accumulate, T* __restrict__ dr) {
     const int64_t mi = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (mi >= n) return;
     const uint32_t e0 = run_start[mi], e1 = run_end[mi];
@@ -143,7 +151,7 @@ __global__ __launch_bounds__(256) void grad_r_kernel(const T* __restrict__ ref, 
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         r[a] = ref[mi * D + a];
-        acc[a] = T(0);
+        acc[a] = accumulate ? dr[mi * D + a] : T(0);
     }
     // loads kBwUnroll pairs ahead; the terms are still subtracted in run order
     for (uint32_t e0u = e0; e0u < e1; e0u += kBwUnroll) {
@@ -181,13 +189,20 @@ int key_bits(int64_t n) {
     return b;
 }
 
+// Pairs per sort (the radix sort's limit); queries per slice = kMaxPairs / k.
+constexpr int64_t kMaxPairs = int64_t(1) << 28;
+
+int64_t slice_queries(int64_t nq, int k) { return std::min<int64_t>(nq, kMaxPairs / k); }
+
 struct BwLayout {
     size_t qkey, qrow, qkey_s, order, run_start, run_end, bbox, q_sorted, g_sorted, keys, vals, keys_s, vals_s,
         temp, temp_bytes, total;
 };
 
+// Workspace of dL/dr: the canonical order of all nq queries, and the buffers of one slice.
 BwLayout bw_layout(int64_t nq, int32_t n, int k, int dims, int dtype) {
-    const int64_t m = nq * k;
+    const int64_t sq = slice_queries(nq, k);
+    const int64_t m = sq * k;
     const size_t ts = dtype_size(dtype);
     BwLayout l{};
     size_t off = 0;
@@ -198,7 +213,7 @@ BwLayout bw_layout(int64_t nq, int32_t n, int k, int dims, int dtype) {
     l.run_start = off; off += a256(4 * (size_t)n);
     l.run_end = off; off += a256(4 * (size_t)n);
     l.bbox = off; off += a256(16 * (size_t)dims);
-    l.q_sorted = off; off += a256(ts * (size_t)dims * (size_t)nq);
+    l.q_sorted = off; off += a256(ts * (size_t)dims * (size_t)sq);
     l.g_sorted = off; off += a256(ts * (size_t)m);
     l.keys = off; off += a256(4 * (size_t)m);
     l.vals = off; off += a256(4 * (size_t)m);
@@ -224,8 +239,8 @@ void backward_typed(const T* q, const T* ref, int64_t nq, int32_t n, int k, cons
     }
     if (nq == 0 || !(dq || pairs)) return;
     if (!pairs) {
-        hipLaunchKernelGGL((grad_q_kernel<T, D>), qgrid, dim3(256), 0, s, q, ref, nq, k, idx, g, nullptr, dq, nullptr,
-                           nullptr, nullptr, nullptr);
+        hipLaunchKernelGGL((grad_q_kernel<T, D>), qgrid, dim3(256), 0, s, q, ref, nq, k, idx, g, nullptr, (int64_t)0,
+                           dq, nullptr, nullptr, nullptr, nullptr);
         TK_HIP_CHECK(hipGetLastError());
         return;
     }
@@ -251,17 +266,22 @@ void backward_typed(const T* q, const T* ref, int64_t nq, int32_t n, int k, cons
     uint32_t* vals_s = reinterpret_cast<uint32_t*>(ws + l.vals_s);
     uint32_t* run_start = reinterpret_cast<uint32_t*>(ws + l.run_start);
     uint32_t* run_end = reinterpret_cast<uint32_t*>(ws + l.run_end);
-    hipLaunchKernelGGL((grad_q_kernel<T, D>), qgrid, dim3(256), 0, s, q, ref, nq, k, idx, g, order, dq, q_sorted,
-                       g_sorted, keys, vals);
-    TK_HIP_CHECK(hipGetLastError());
-    radix_sort_pairs(keys, vals, keys_s, vals_s, m, key_bits(n), ws + l.temp, l.temp_bytes, s);
     const dim3 rgrid((unsigned)(((int64_t)n + 255) / 256));
-    hipLaunchKernelGGL(zero_u32_kernel, rgrid, dim3(256), 0, s, run_start, run_end, (int64_t)n);
-    hipLaunchKernelGGL(run_bounds_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, keys_s, m, run_start,
-                       run_end);
-    hipLaunchKernelGGL((grad_r_kernel<T, D>), rgrid, dim3(256), 0, s, ref, n, k, run_start, run_end, vals_s, q_sorted,
-                       g_sorted, dr);
-    TK_HIP_CHECK(hipGetLastError());
+    const int64_t sq = slice_queries(nq, k);
+    for (int64_t t0 = 0; t0 < nq; t0 += sq) {
+        const int64_t cnt = std::min(sq, nq - t0);
+        const int64_t mm = cnt * k;
+        hipLaunchKernelGGL((grad_q_kernel<T, D>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, q, ref, cnt, k,
+                           idx, g, order, t0, dq, q_sorted, g_sorted, keys, vals);
+        TK_HIP_CHECK(hipGetLastError());
+        radix_sort_pairs(keys, vals, keys_s, vals_s, mm, key_bits(n), ws + l.temp, l.temp_bytes, s);
+        hipLaunchKernelGGL(zero_u32_kernel, rgrid, dim3(256), 0, s, run_start, run_end, (int64_t)n);
+        hipLaunchKernelGGL(run_bounds_kernel, dim3((unsigned)((mm + 255) / 256)), dim3(256), 0, s, keys_s, mm,
+                           run_start, run_end);
+        hipLaunchKernelGGL((grad_r_kernel<T, D>), rgrid, dim3(256), 0, s, ref, n, k, run_start, run_end, vals_s,
+                           q_sorted, g_sorted, t0 > 0, dr);
+        TK_HIP_CHECK(hipGetLastError());
+    }
 }
 
 }  // namespace
@@ -275,7 +295,9 @@ void knn_backward_gpu(const void* queries, const void* ref, int64_t nq, int32_t 
                       void* workspace, size_t workspace_bytes) {
     if (!dims_supported(dims)) throw std::runtime_error("unsupported dims");
     if (k < 1 || n < 1) throw std::runtime_error("invalid backward shape");
-    if (nq * (int64_t)k > (int64_t(1) << 28)) throw std::runtime_error("backward: nr_query * k must be <= 2^28");
+    if (nq < 0) throw std::runtime_error("invalid backward shape");
+    if (grad_ref && nq > kMaxPairs)
+        throw std::runtime_error("backward: the gradient of the reference points takes at most 2^28 queries");
     const BwLayout l = bw_layout(nq, n, k, dims, dtype);
     std::unique_ptr<StreamScratch> own;
     unsigned char* ws = static_cast<unsigned char*>(workspace);
