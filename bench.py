@@ -10,11 +10,17 @@ in the reference's benchmark (README.md:75).
 
 Prints one JSON line.  ``roofline`` refers to the traversal kernel: its average duration is
 measured with HIP events on the launch stream during the timed steps; ``achieved`` divides
-the algorithmic bytes of one launch by it; ``traffic`` is the HBM traffic of one launch from the
-rocprofv3 FETCH_SIZE / WRITE_SIZE summary under profiles/, reported when that summary was taken
-on the same configuration (tree depth, tie policy, presort).  ``cpu_baseline`` times the oracle (the C++
-restatement of the reference CPU search, OpenMP over queries) on a bounded sample of the same
-queries and the same tree on this host.
+the algorithmic bytes of one launch by it: per query d*4 (the query) + k*(4+4) (distance and
+int32 index written once) = 140 B at d=3, k=16, plus the tree read once (N*(d*4+4) record bytes
+and 5 B per node); ``peak`` is the 8 TB/s HBM3E figure, ``achievable_gbs`` the rate of the
+library's own streaming copy on this GPU (``frac_of_achievable``).  The int64 original indices
+the kernel also writes (the reference computes them afterwards in torch) are counted only in
+``achieved_with_fused_remap``.  ``traffic`` is the HBM traffic of one launch from the rocprofv3
+FETCH_SIZE / WRITE_SIZE summary under profiles/, reported when that summary was taken on the same
+configuration (tree depth, tie policy, presort).  ``cpu_baseline`` times the oracle (the C++
+restatement of the reference CPU search, OpenMP over queries) on bounded samples of the same
+queries, on the same tree and on the reference-default-depth tree, with all granted host threads
+and with one.
 """
 import argparse
 import json
@@ -54,6 +60,49 @@ def _events(torch_knn, stage, fn, reps):
     ms, n = torch_knn.profile_read(stage)
     torch_knn.profile_reset()
     return ms / max(n, 1)
+
+
+def _achievable_gbs(torch, torch_knn, dev, mib=1024, reps=5):
+    """Rate of the library's streaming copy (tk_copy_gpu) over `mib` MiB: bytes read + written per second."""
+    src = torch.empty(mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    nbytes = src.numel() * 4
+    torch_knn.copy_gpu(dst.data_ptr(), src.data_ptr(), nbytes, stream)
+    torch.cuda.synchronize()
+    ms = _events(torch_knn, "copy", lambda: torch_knn.copy_gpu(dst.data_ptr(), src.data_ptr(), nbytes, stream), reps)
+    del src, dst
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
+
+
+def _host_facts():
+    """nproc and the CPU model (lscpu) of this host."""
+    import subprocess
+    facts = {"nproc": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {a.strip(): b.strip() for a, b in (ln.split(":", 1) for ln in out.splitlines() if ":" in ln)}
+        facts.update({"model": kv.get("Model name"), "sockets": kv.get("Socket(s)"),
+                      "cores_per_socket": kv.get("Core(s) per socket"), "threads_per_core": kv.get("Thread(s) per core"),
+                      "lscpu_cpus": kv.get("CPU(s)")})
+    except Exception as e:  # lscpu missing: record why
+        facts["lscpu"] = "unavailable: %s" % e
+    try:
+        facts["sched_affinity"] = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    return facts
+
+
+def _cpu_rate(np, oracle_tree, sample, k, threads, runs):
+    """Queries/s of the oracle search on `sample`: (median, mean) over `runs` timed runs after one warm-up."""
+    oracle_tree.search(sample[:1024], k, nthreads=threads)
+    rates = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        oracle_tree.search(sample, k, nthreads=threads)
+        rates.append(sample.shape[0] / (time.perf_counter() - t0))
+    return float(np.median(rates)), float(np.mean(rates))
 
 
 def _extra_bruteforce(np, torch, torch_knn, dev, w):
@@ -380,10 +429,10 @@ def main():
         src = torch.empty(args.calibrate_copy_mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
-        dst.copy_(src)
+        torch_knn.copy_gpu(dst.data_ptr(), src.data_ptr(), src.numel() * 4, torch.cuda.current_stream(dev).cuda_stream)
         torch.cuda.synchronize()
         del src, dst
-        log("calibration copy of %d MiB done" % args.calibrate_copy_mib)
+        log("calibration copy of %d MiB done (stream_copy_kernel)" % args.calibrate_copy_mib)
     log("timing %d steps" % args.steps)
 
     torch_knn.profile_reset()
@@ -403,14 +452,19 @@ def main():
     value = nq * args.steps / elapsed
 
     # Algorithmic bytes of one traversal launch: every query read once (d*4 B) and its k results
-    # written once (4 B distance + 4 B structured index + 8 B original index), plus the tree read
-    # once (records 16 B/point, split values 4 B/node, leaf offsets 4 B/leaf).
-    rec_bytes = 16
-    tree_bytes = n * rec_bytes + ((1 << tree.levels) - 1) * 4 + ((1 << tree.levels) + 1) * 4
-    bytes_per_query = d * 4 + k * (4 + 4 + 8)
+    # written once (4 B distance + 4 B int32 index), plus the tree read once (N*(d*4+4) record
+    # bytes, 5 B per node: split value and axis).  The kernel also writes the int64 original index
+    # (8 B per result), which the reference computes afterwards in torch: counted only in the
+    # fused-remap figure.
+    n_nodes = (1 << tree.levels) - 1
+    tree_bytes = n * (d * 4 + 4) + n_nodes * 5
+    bytes_per_query = d * 4 + k * (4 + 4)
     launch_bytes = nq * bytes_per_query + tree_bytes
+    fused_bytes = launch_bytes + nq * k * 8
     trav_avg_ms = trav_ms / max(trav_n, 1)
     achieved = launch_bytes / (trav_avg_ms * 1e-3) / 1e9
+    log("measuring the achievable bandwidth (streaming copy, 1 GiB)")
+    achievable = _achievable_gbs(torch, torch_knn, dev)
 
     cpu = None
     if args.cpu_sample > 0:
@@ -419,17 +473,37 @@ def main():
 
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         half = w["halves"][0]
-        s = min(args.cpu_sample // 2, half)
-        sample = np.ascontiguousarray(np.concatenate([qry[:s], qry[half:half + s]]))
+
+        def balanced(count):
+            s = min(count // 2, half)
+            return np.ascontiguousarray(np.concatenate([qry[:s], qry[half:half + s]]))
+
+        sample = balanced(args.cpu_sample)
+        small = balanced(max(2048, args.cpu_sample // 16))
         ot = oracle.tree_from_product(tree)
-        ot.search(sample[:1024], k, nthreads=threads)
-        t0 = time.perf_counter()
-        ot.search(sample, k, nthreads=threads)
-        cpu_s = time.perf_counter() - t0
-        cpu = {"value": sample.shape[0] / cpu_s, "unit": "queries/s", "cores": threads, "kind": "port",
+        med, mean = _cpu_rate(np, ot, sample, k, threads, 5)
+        med1, mean1 = _cpu_rate(np, ot, small, k, 1, 3)
+        del ot
+        log("timing the CPU baseline on the reference-default-depth tree")
+        t13 = tk.build_kd_tree(torch.from_numpy(ref), device="cpu", levels=None)
+        ot13 = oracle.tree_from_product(t13)
+        s13 = balanced(max(4096, args.cpu_sample // 4))
+        med13, mean13 = _cpu_rate(np, ot13, s13, k, threads, 3)
+        s13_1 = balanced(max(2048, args.cpu_sample // 64))
+        med13_1, mean13_1 = _cpu_rate(np, ot13, s13_1, k, 1, 3)
+        del ot13, t13
+        cpu = {"value": med, "unit": "queries/s", "cores": threads, "kind": "port",
                "sample": "first %d queries of each half of the 16M-query set (%d total), same tree (levels=%d), k=%d; "
-                         "oracle = C++ restatement of KDTreeKNNSearch, OpenMP over queries, %d threads; %.2f s wall"
-                         % (s, sample.shape[0], tree.levels, k, threads, cpu_s)}
+                         "oracle = C++ restatement of KDTreeKNNSearch, OpenMP over queries, %d threads; median of 5 "
+                         "runs after a warm-up" % (sample.shape[0] // 2, sample.shape[0], tree.levels, k, threads),
+               "host": _host_facts(),
+               "runs": {"levels_%d_threads_%d" % (tree.levels, threads): {"queries": sample.shape[0], "median": med,
+                                                                        "mean": mean},
+                        "levels_%d_threads_1" % tree.levels: {"queries": small.shape[0], "median": med1, "mean": mean1},
+                        "levels_13_threads_%d" % threads: {"queries": s13.shape[0], "median": med13, "mean": mean13},
+                        "levels_13_threads_1": {"queries": s13_1.shape[0], "median": med13_1, "mean": mean13_1}},
+               "note": "levels 13 is the reference's default depth for 4M points (nn_distance.py:146-147); the GPU "
+                       "step above runs at levels %d" % tree.levels}
 
     extras = None
     if not args.no_extras:
@@ -472,7 +546,11 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": _measured_traffic(tree.levels, args.tie, args.presort),
                      "kernel": "knn_tree_kernel<float,3,16,false>",
-                     "algorithmic_bytes_per_launch": launch_bytes},
+                     "algorithmic_bytes_per_launch": launch_bytes,
+                     "bytes_per_query": bytes_per_query, "tree_bytes": tree_bytes,
+                     "traverse_ms": trav_avg_ms,
+                     "achievable_gbs": achievable, "frac_of_achievable": achieved / achievable,
+                     "achieved_with_fused_remap": fused_bytes / (trav_avg_ms * 1e-3) / 1e9},
         "cpu_baseline": cpu,
         "secondary": extras,
     }

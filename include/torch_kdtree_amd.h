@@ -231,6 +231,11 @@ int tk_profile_enable(int on);
 int tk_profile_reset(void);
 int tk_profile_read(const char* stage, double* total_ms, int64_t* calls);
 
+/* A streaming device-to-device copy (16-byte loads and stores; aligned pointers, bytes a multiple
+ * of 16), bracketed as stage "copy" when timing is on.  No counterpart in the reference: its rate
+ * is the achievable HBM bandwidth that the traversal's roofline is quoted against. */
+int tk_copy_gpu(void* dst, const void* src, size_t bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

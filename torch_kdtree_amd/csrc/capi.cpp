@@ -401,6 +401,13 @@ int tk_sort_pairs_gpu_workspace(int64_t n, size_t* bytes) {
     return guarded([&] { *bytes = tk::radix_sort_workspace(n); });
 }
 
+int tk_copy_gpu(void* dst, const void* src, size_t bytes, void* stream) {
+    return guarded([&] {
+        tk::ProfScope prof("copy", static_cast<hipStream_t>(stream));
+        tk::stream_copy_gpu(dst, src, bytes, static_cast<hipStream_t>(stream));
+    });
+}
+
 int tk_profile_enable(int on) {
     return guarded([&] { tk::prof_enable(on != 0); });
 }

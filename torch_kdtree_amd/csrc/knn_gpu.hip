@@ -197,6 +197,37 @@ void upload_tree(Tree& t, int device) {
 }
 
 // ------------------------------------------------------------------------------------------
+// Streaming copy: 16-byte loads and stores, each thread striding over the array.  Its rate is the
+// achievable HBM bandwidth the traversal's roofline is quoted against, and a copy of known size
+// scales the FETCH_SIZE / WRITE_SIZE counters.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void stream_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                          int64_t n16) {
+    constexpr int kU = 4;  // loads in flight per thread
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += kU * stride) {
+        uint4 v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (i + u * stride < n16) v[u] = src[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (i + u * stride < n16) dst[i + u * stride] = v[u];
+    }
+}
+
+void stream_copy_gpu(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+    if (bytes % 16 != 0 || (((uintptr_t)dst | (uintptr_t)src) & 15u) != 0)
+        throw std::runtime_error("stream copy: 16-byte aligned pointers and a multiple of 16 bytes needed");
+    const int64_t n16 = (int64_t)(bytes / 16);
+    if (n16 == 0) return;
+    const unsigned blocks = (unsigned)std::min<int64_t>((n16 + 255) / 256, 256 * 16);
+    hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, stream, static_cast<const uint4*>(src),
+                       static_cast<uint4*>(dst), n16);
+    TK_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------------------
 // Scan / scatter kernels for the presort
 // ------------------------------------------------------------------------------------------
 __global__ void fill_i32_kernel(int32_t* __restrict__ p, int64_t n, int32_t v) {

@@ -76,6 +76,9 @@ void init_bbox(void* bbox, int dtype, int dims, hipStream_t stream);
 // cell, at most the grid of the forward presort.
 int canonical_morton_bits(int64_t nq, int dims);
 
+// Streaming device copy (16-byte loads and stores) of `bytes` (a multiple of 16, aligned pointers).
+void stream_copy_gpu(void* dst, const void* src, size_t bytes, hipStream_t stream);
+
 // Brute force (bruteforce.hip).
 void bruteforce_gpu(const void* ref, int32_t n, const void* queries, int64_t nq, int dims, int dtype, int k,
                     void* dist, int32_t* idx, hipStream_t stream, void* workspace, size_t workspace_bytes);

@@ -496,6 +496,19 @@ PYBIND11_MODULE(torch_knn, m) {
     m.def("profile_enable", [](bool on) { check(tk_profile_enable(on ? 1 : 0)); }, py::arg("on"));
     m.def("profile_reset", [] { check(tk_profile_reset()); });
     m.def(
+        "copy_gpu",
+        [](size_t dst_ptr, size_t src_ptr, size_t bytes, size_t stream) {
+            int rc;
+            {
+                py::gil_scoped_release release;
+                rc = tk_copy_gpu(reinterpret_cast<void*>(dst_ptr), reinterpret_cast<const void*>(src_ptr), bytes,
+                                 reinterpret_cast<void*>(stream));
+            }
+            check(rc);
+        },
+        py::arg("dst_ptr"), py::arg("src_ptr"), py::arg("bytes"), py::arg("stream") = 0,
+        "Streaming device copy (16-byte loads and stores), timed as stage 'copy'");
+    m.def(
         "profile_read",
         [](const std::string& stage) {
             double ms = 0;

@@ -86,14 +86,14 @@ struct Builder {
 // searches that meet one, on trees that are not `finite`, use no boxes under the reference tie
 // policy).  Also sets t.finite: every coordinate and every split value finite.
 template <typename T, int D>
-void compute_aabb(Tree& t) {
+void compute_aabb(Tree& t, int n_threads) {
     const int32_t nn = t.n_nodes, nl = t.n_leaves;
     t.aabb.assign((size_t)(nn + nl) * 2 * D * sizeof(T), 0);
     T* box = reinterpret_cast<T*>(t.aabb.data());
     const T* pts = t.points_ptr<T>();
     const T inf = std::numeric_limits<T>::infinity();
     bool finite = true;
-#pragma omp parallel for schedule(static) reduction(&& : finite)
+#pragma omp parallel for schedule(static) reduction(&& : finite) num_threads(n_threads)
     for (int32_t j = 0; j < nl; ++j) {
         T* b = box + (size_t)(nn + j) * 2 * D;
         for (int a = 0; a < D; ++a) {
@@ -132,16 +132,17 @@ void build_typed(Tree& t, const T* src, int n_threads) {
 #pragma omp single
     b.node(0, 0, t.n_points, t.levels - 1, 0);
     t.leaf_off[t.n_leaves] = t.n_points;
-    compute_aabb<T, D>(t);
+    compute_aabb<T, D>(t, n_threads);
 }
 
 template <typename T>
 void compute_aabb_t(Tree& t) {
+    const int nt = omp_get_max_threads();
     switch (t.dims) {
-        case 1: compute_aabb<T, 1>(t); break;
-        case 2: compute_aabb<T, 2>(t); break;
-        case 3: compute_aabb<T, 3>(t); break;
-        case 8: compute_aabb<T, 8>(t); break;
+        case 1: compute_aabb<T, 1>(t, nt); break;
+        case 2: compute_aabb<T, 2>(t, nt); break;
+        case 3: compute_aabb<T, 3>(t, nt); break;
+        case 8: compute_aabb<T, 8>(t, nt); break;
         default: throw std::runtime_error("unsupported dims");
     }
 }
