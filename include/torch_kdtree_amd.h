@@ -119,6 +119,19 @@ void tk_tree_free(tk_tree* tree);
  * for step, without point-box pruning. */
 int tk_tree_info(const tk_tree* tree, int32_t out[6]);
 
+/* Launch variants of the GPU traversal (no counterpart in the reference, whose launch shape is
+ * fixed at compile time, src/kdtree_g.cu:6,281-283): prebuilt instantiations that differ in the
+ * leaf-scan chunk staged in LDS, the occupancy target and the threads per block, for an autotuner
+ * to measure and pick among.  Variant 0 is the built-in choice; variants 1..n-1 exist for float
+ * searches at d = 3 with k in 5..16 on trees whose values are finite, and any other search runs
+ * variant 0.  Results never depend on the variant.
+ * tk_tree_variant_info: out = {chunk, waves per SIMD (0 = no target), threads per block} of a
+ * variant (zeros for variant 0), *count = number of variants; fails for an unknown variant.
+ * tk_tree_set_variant: the variant the tree's later GPU queries launch. */
+int tk_tree_variant_info(int variant, int32_t out[3], int* count);
+int tk_tree_set_variant(tk_tree* tree, int variant);
+int tk_tree_get_variant(const tk_tree* tree, int* variant);
+
 /* Replaces KDTree::get_shuffled_inds (src/interface.cpp:45-50): out[n_points] receives the
  * original index of each structured position. */
 int tk_tree_get_shuffled_inds(const tk_tree* tree, int32_t* out);

@@ -430,3 +430,25 @@ def test_c_abi_serialize_roundtrip():
     np.testing.assert_array_equal(a, b)
     lib.tk_tree_free(h)
     lib.tk_tree_free(h2)
+
+
+def test_autotune_cache_lookup(tmp_path, monkeypatch):
+    """The tuning cache: leaf size per dims/dtype, launch variant per list width; missing entries
+    fall back to the built-in table and variant 0; CPU trees keep the reference's default depth."""
+    import json
+    from torch_kdtree_amd import autotune
+    cache = tmp_path / "t.json"
+    monkeypatch.setenv("TORCH_KDTREE_AMD_TUNING", str(cache))
+    assert autotune.variant_for(3, torch.float32, 16) == 0
+    assert autotune.target_leaf_size(3, torch.float32) == autotune.DEFAULT_LEAF[3]
+    cache.write_text(json.dumps({"d3_float32": {"leaf_size": 24, "variants": {"16": 5, "8": 2}}}))
+    assert autotune.target_leaf_size(3, torch.float32) == 24
+    assert autotune.variant_for(3, torch.float32, 9) == 5 and autotune.variant_for(3, torch.float32, 16) == 5
+    assert autotune.variant_for(3, torch.float32, 5) == 2 and autotune.variant_for(3, torch.float32, 4) == 0
+    assert autotune.variant_for(3, torch.float64, 16) == 0
+    assert autotune.pick_levels(4_000_000, 3, "cpu") == 13
+    assert autotune.pick_levels(4_000_000, 3, "cuda") == autotune._levels_for_leaf(4_000_000, 24)
+    from torch_kdtree_amd import torch_knn
+    table = torch_knn.tree_variants()
+    assert table[0] == {"chunk": 0, "waves": 0, "block": 0} and len(table) == 9
+    assert all(v["chunk"] in (4, 8, 16) and v["block"] in (64, 128, 256) for v in table[1:])

@@ -205,6 +205,30 @@ int tk_tree_info(const tk_tree* tree, int32_t out[6]) {
     });
 }
 
+int tk_tree_variant_info(int variant, int32_t out[3], int* count) {
+    return guarded([&] {
+        if (variant < 0 || variant >= tk::kNumTreeVariants) throw std::runtime_error("unknown launch variant");
+        if (out) {
+            out[0] = tk::kTreeVariants[variant].chunk;
+            out[1] = tk::kTreeVariants[variant].waves;
+            out[2] = tk::kTreeVariants[variant].block;
+        }
+        if (count) *count = tk::kNumTreeVariants;
+    });
+}
+
+int tk_tree_set_variant(tk_tree* tree, int variant) {
+    return guarded([&] {
+        if (!tree || !tree->t) throw std::runtime_error("null tree handle");
+        if (variant < 0 || variant >= tk::kNumTreeVariants) throw std::runtime_error("unknown launch variant");
+        tree->t->variant = variant;
+    });
+}
+
+int tk_tree_get_variant(const tk_tree* tree, int* variant) {
+    return guarded([&] { *variant = tree_of(tree).variant; });
+}
+
 int tk_tree_get_shuffled_inds(const tk_tree* tree, int32_t* out) {
     return guarded([&] {
         const tk::Tree& t = tree_of(tree);

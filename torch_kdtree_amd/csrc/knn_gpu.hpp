@@ -56,6 +56,22 @@ constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
 void query_gpu_noalloc(const Tree& t, const void* queries, int64_t nq, int k, void* dist, int32_t* idx, int64_t* orig,
                        const QueryOpts& o, hipStream_t stream);
 
+// Prebuilt launch variants of the traversal kernel that an autotuner picks among: leaf-scan chunk
+// (records staged in LDS per pass), occupancy target (waves per SIMD) and threads per block.
+// Variant 0 is the built-in choice (StageBytes / TreeOccupancy in knn_tree.hpp, blocks of 128).
+// The others exist for float searches at d = 3 with lists of 8 or 16 entries (k = 5..16); any
+// other search runs variant 0 whatever is asked.  Results never depend on the variant.
+struct TreeVariant {
+    int chunk, waves, block;
+};
+constexpr TreeVariant kTreeVariants[] = {{0, 0, 0},    {4, 7, 128},  {8, 7, 128},  {16, 7, 128}, {8, 5, 128},
+                                         {16, 5, 128}, {16, 7, 64},  {16, 7, 256}, {8, 7, 256}};
+constexpr int kNumTreeVariants = (int)(sizeof(kTreeVariants) / sizeof(kTreeVariants[0]));
+// Launches variant `variant` (> 0) for a float d = 3 search when it is prebuilt for list width kb
+// (knn_tree_variants.hip); returns false when it is not, and the caller launches variant 0.
+bool launch_tree_variant_f32_d3(const Tree& t, const float* q, int64_t nq, const int32_t* order, int k, int kb,
+                                float* dist, int32_t* idx, int64_t* orig, bool lower, int variant, hipStream_t s);
+
 // Per-dtype kernel launchers (knn_tree_f32.hip / knn_tree_f64.hip).
 void launch_tree_search_f32(const Tree& t, const float* q, int64_t nq, const int32_t* order, int k, float* dist,
                             int32_t* idx, int64_t* orig, int tie, hipStream_t s);

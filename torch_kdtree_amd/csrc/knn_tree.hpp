@@ -364,7 +364,7 @@ constexpr int kRecPad = 16;
 // by the stale worst is rejected by every later (smaller or equal) worst as well.
 // In pass 2 a candidate that the current list rejects is inserted as +inf, which leaves the list
 // unchanged: the insertion is then unconditional and updates the list in place.
-template <typename T, int D, int K, bool kLower, int CH, int LD, bool kB>
+template <typename T, int D, int K, bool kLower, int CH, int LD, int SB = kTreeBlockStride, bool kB>
 __device__ __forceinline__ void scan_leaf_staged(const TreeView<T, kB>& tv, int32_t leaf, const Pt<T, D>& q,
                                                  KBest<T, K, kLower>& kb, T* __restrict__ st_d,
                                                  int32_t* __restrict__ st_o) {
@@ -392,8 +392,8 @@ __device__ __forceinline__ void scan_leaf_staged(const TreeView<T, kB>& tv, int3
                 const T dd = sq_dist<T, D>(q, r.c);
                 const bool acc = kb.accepts_stale(w, dd, kLower ? r.o : 0);
                 mask |= acc ? (1u << j) : 0u;
-                st_d[j * kTreeBlockStride] = dd;
-                if constexpr (kLower) st_o[j * kTreeBlockStride] = r.o;
+                st_d[j * SB] = dd;
+                if constexpr (kLower) st_o[j * SB] = r.o;
             }
         }
         mask &= n >= 32 ? ~0u : ((1u << n) - 1u);  // only the leaf's points
@@ -401,8 +401,8 @@ __device__ __forceinline__ void scan_leaf_staged(const TreeView<T, kB>& tv, int3
         while (mask) {
             const int j = __builtin_ctz(mask);
             mask &= mask - 1u;
-            const T dd = st_d[j * kTreeBlockStride];
-            const int32_t o = kLower ? st_o[j * kTreeBlockStride] : 0;
+            const T dd = st_d[j * SB];
+            const int32_t o = kLower ? st_o[j * SB] : 0;
             const bool ok = kb.accepts(dd, o);
             kb.insert(ok ? dd : pos_inf<T>(), ok ? (kLower ? o : c0 + j) : 0x7fffffff);
         }
@@ -512,7 +512,7 @@ __device__ __forceinline__ void lane_start(const TreeView<T, kB>& tv, int k, con
 
 // Scans the leaf pair of the current leaf-parent, then climbs to the next far side to enter and
 // walks down to its leaf-parent.  Returns true when the search is complete.
-template <typename T, int D, int K, bool kLower, int CH, int LD, bool kB>
+template <typename T, int D, int K, bool kLower, int CH, int LD, int SB = kTreeBlockStride, bool kB>
 __device__ __forceinline__ bool lane_step(const TreeView<T, kB>& tv, const Pt<T, D>& q, Pt<T, D>& pp,
                                           KBest<T, K, kLower>& kb, uint32_t& far_mask, int32_t& node, int& level,
                                           T* __restrict__ st_d, int32_t* __restrict__ st_o) {
@@ -532,7 +532,7 @@ __device__ __forceinline__ bool lane_step(const TreeView<T, kB>& tv, const Pt<T,
             const bool looks = side == 0 || (kLower ? lower_split_passes<T, D>(tv, q, pp, a_lp, m, kb.worst())
                                                     : proj_bound<T, D>(q, pp, a_lp, m) < kb.worst());
             if (looks && leaf_useful<T, D, kLower>(tv, q, leaf, kb.worst()))
-                scan_leaf_staged<T, D, K, kLower, CH, LD>(tv, leaf, q, kb, st_d, st_o);
+                scan_leaf_staged<T, D, K, kLower, CH, LD, SB>(tv, leaf, q, kb, st_d, st_o);
         }
         if (level == 0) return true;
         ax_set(pp, a_lp, qa);
@@ -786,18 +786,16 @@ struct StageBytes {
 // One lane per query.  The lanes of a wavefront take 64 consecutive presorted queries and run
 // their searches to completion together: neighbouring queries walk the tree nearly in step, which
 // keeps the wavefront's control flow and loads coherent.  (Handing a finished lane the next query
-// was measured to be slower: it breaks that lockstep.)
-template <typename T, int D, int K, bool kLower>
-__global__ __launch_bounds__(kTreeBlock) TK_TREE_OCCUPANCY(T, D, K, kLower) void knn_tree_kernel(TreeView<T> tv, const T* __restrict__ queries,
-                                                               int64_t nq, const int32_t* __restrict__ order, int k,
-                                                               T* __restrict__ out_d, int32_t* __restrict__ out_i,
-                                                               int64_t* __restrict__ out_o) {
-    constexpr int CH = StageBytes<T, D, K, kLower>::kChunk;
-    constexpr int LD = StageBytes<T, D, K, kLower>::kLoads;
+// was measured to be slower: it breaks that lockstep.)  BLK threads per block, leaf chunks of CH
+// records with LD loads in flight.
+template <typename T, int D, int K, bool kLower, int CH, int LD, int BLK>
+__device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T* __restrict__ queries, int64_t nq,
+                                                  const int32_t* __restrict__ order, int k, T* __restrict__ out_d,
+                                                  int32_t* __restrict__ out_i, int64_t* __restrict__ out_o) {
     static_assert(CH <= kRecPad, "a chunk may read at most kRecPad records past a leaf's end");
-    __shared__ T stage_d[CH * kTreeBlock];
-    __shared__ int32_t stage_o[kLower ? CH * kTreeBlock : 1];
-    const int64_t t = xcd_tile(blockIdx.x, gridDim.x) * kTreeBlock + threadIdx.x;
+    __shared__ T stage_d[CH * BLK];
+    __shared__ int32_t stage_o[kLower ? CH * BLK : 1];
+    const int64_t t = xcd_tile(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
     if (t >= nq) return;
     const int64_t qi = order ? (int64_t)order[t] : t;
     Pt<T, D> q, pp;
@@ -810,10 +808,29 @@ __global__ __launch_bounds__(kTreeBlock) TK_TREE_OCCUPANCY(T, D, K, kLower) void
     if (has_nan<T, D>(q))
         nan_fill<T, D, K, kLower>(tv, q, k, kb);
     else
-        while (!lane_step<T, D, K, kLower, CH, LD>(tv, q, pp, kb, far_mask, node, level, stage_d + threadIdx.x,
-                                                stage_o + (kLower ? threadIdx.x : 0))) {
+        while (!lane_step<T, D, K, kLower, CH, LD, BLK>(tv, q, pp, kb, far_mask, node, level, stage_d + threadIdx.x,
+                                                      stage_o + (kLower ? threadIdx.x : 0))) {
         }
     store_list<T, K, kLower>(kb, k, tv, qi, out_d, out_i, out_o);
+}
+
+template <typename T, int D, int K, bool kLower>
+__global__ __launch_bounds__(kTreeBlock) TK_TREE_OCCUPANCY(T, D, K, kLower) void knn_tree_kernel(TreeView<T> tv, const T* __restrict__ queries,
+                                                               int64_t nq, const int32_t* __restrict__ order, int k,
+                                                               T* __restrict__ out_d, int32_t* __restrict__ out_i,
+                                                               int64_t* __restrict__ out_o) {
+    tree_search_lanes<T, D, K, kLower, StageBytes<T, D, K, kLower>::kChunk, StageBytes<T, D, K, kLower>::kLoads,
+                      kTreeBlock>(tv, queries, nq, order, k, out_d, out_i, out_o);
+}
+
+// The same search with its launch shape as template parameters: the prebuilt variants an autotuner
+// picks among (leaf chunk CH with LD loads in flight, BLK threads per block, an occupancy target of
+// WAVES waves per SIMD, 0 = none).  Results do not depend on them.
+template <typename T, int D, int K, bool kLower, int CH, int LD, int BLK, int WAVES>
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void knn_tree_kernel_v(
+    TreeView<T> tv, const T* __restrict__ queries, int64_t nq, const int32_t* __restrict__ order, int k,
+    T* __restrict__ out_d, int32_t* __restrict__ out_i, int64_t* __restrict__ out_o) {
+    tree_search_lanes<T, D, K, kLower, CH, LD, BLK>(tv, queries, nq, order, k, out_d, out_i, out_o);
 }
 
 constexpr int kLdsBlock = 64;

@@ -90,6 +90,13 @@ struct KDTree {
 
     // Every coordinate and split value finite (otherwise GPU queries under the reference tie
     // policy run the reference's own steps, without point-box pruning).
+    void set_variant(int v) { check(tk_tree_set_variant(tree, v)); }
+    int get_variant() const {
+        int v = 0;
+        check(tk_tree_get_variant(tree, &v));
+        return v;
+    }
+
     bool is_finite() const {
         int32_t info[6];
         check(tk_tree_info(tree, info));
@@ -306,6 +313,9 @@ void bind_tree(py::module_& m, const char* name) {
              "Returns the ordered points how they are used in the KD-Tree")
         .def("get_nodes", &C::get_nodes, "Returns (axis[int32], split[T], leaf_offsets[int32]) of the tree")
         .def("is_finite", &C::is_finite, "True when every coordinate and split value of the tree is finite")
+        .def("set_variant", &C::set_variant, py::arg("variant"),
+             "Launch variant of later GPU queries (see torch_knn.tree_variants())")
+        .def("get_variant", &C::get_variant)
         .def("query", &C::query, py::arg("points_query_ptr"), py::arg("nr_query_points"), py::arg("nr_nns_searches"),
              py::arg("dist_arr_ptr"), py::arg("knn_idx_ptr"), py::kw_only(), py::arg("orig_idx_ptr") = 0,
              py::arg("tie") = 0, py::arg("presort") = -1, py::arg("stream") = 0, py::arg("n_threads") = 0,
@@ -495,6 +505,24 @@ PYBIND11_MODULE(torch_knn, m) {
         "Stable GPU radix sort of uint32 (key, value) pairs by the low key_bits bits of the keys");
     m.def("profile_enable", [](bool on) { check(tk_profile_enable(on ? 1 : 0)); }, py::arg("on"));
     m.def("profile_reset", [] { check(tk_profile_reset()); });
+    m.def(
+        "tree_variants",
+        [] {
+            int n = 0;
+            int32_t v[3];
+            check(tk_tree_variant_info(0, v, &n));
+            py::list out;
+            for (int i = 0; i < n; ++i) {
+                check(tk_tree_variant_info(i, v, nullptr));
+                py::dict d;
+                d["chunk"] = v[0];
+                d["waves"] = v[1];
+                d["block"] = v[2];
+                out.append(d);
+            }
+            return out;
+        },
+        "Launch variants of the GPU traversal: [{chunk, waves, block}] (index 0 = the built-in choice)");
     m.def(
         "copy_gpu",
         [](size_t dst_ptr, size_t src_ptr, size_t bytes, size_t stream) {

@@ -45,6 +45,8 @@ struct Tree {
     // can be NaN, and the reference's insertion (src/nndistance.hpp:41-85) takes a NaN distance
     // into the list; the searches then follow the reference step by step, without point boxes.
     bool finite = true;
+    // Launch variant of the GPU traversal (kTreeVariants in knn_gpu.hpp; 0 = the built-in choice).
+    int variant = 0;
     DeviceTree* dev = nullptr;          // GPU copy, owned
 
     template <typename T> T* split_ptr() { return reinterpret_cast<T*>(split.data()); }

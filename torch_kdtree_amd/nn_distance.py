@@ -86,7 +86,7 @@ class _KnnSquaredDistances(torch.autograd.Function):
 
 class TorchKDTree:
     def __init__(self, points_ref: torch.Tensor, device: torch.device, levels: int, squared_distances: bool,
-                 tie_break: str = "reference", presort="auto", _kdtree=None, builder: str = "cpu"):
+                 tie_break: str = "reference", presort="auto", _kdtree=None, builder: str = "cpu", variant=None):
         """Builds the KDTree. See :ref:`build_kd_tree` for more details.  (``_kdtree``: an already
         built ``torch_knn`` tree of ``points_ref``, see ``load_kd_tree``.)"""
         assert device.type in ['cpu', 'cuda']
@@ -142,6 +142,9 @@ class TorchKDTree:
         if presort not in _PRESORT:
             raise ValueError("presort must be 'auto', 'on' ('morton'), 'leaf' or 'off'")
         self.presort = presort
+        # launch variant of GPU queries: None = the autotuner's choice for each k (autotune.variant_for)
+        self.variant = variant
+        self._applied_variant = 0
 
         if self.use_gpu:
             self.structured_points = self.structured_points.to(self.device)
@@ -158,21 +161,23 @@ class TorchKDTree:
 
         Parameters
         ----------
-        points_query : torch.Tensor of float or double precision
-            Points for which the KNNs will be computed
+        points_query : torch.Tensor, float32 or float64, shape (Q, d)
+            The query points.
         nr_nns_searches : int, optional
-            How many closest nearest neighbors will be queried (=k), by default 1
-        result_dists : torch.Tensor of float or double precision, optional
-            Target array that will hold the resulting distance. If not specified, this will be dynamically created.
-        result_idx : torch.Tensor of dtype_idx type, optional
-            Target array that will hold the resulting KNN indices (structured order, as in the reference).
-            If not specified, this will be dynamically created.
+            k, the number of neighbours returned per query (default 1).
+        result_dists : torch.Tensor, optional
+            Caller-owned (Q, k) buffer of the tree's dtype for the squared distances; allocated when
+            omitted.
+        result_idx : torch.Tensor, optional
+            Caller-owned (Q, k) int32 buffer for the neighbour positions in the tree's reordered
+            point array (what the reference's C++ layer writes); allocated when omitted.
 
         Returns
         -------
         tuple
-            * dists (tensor of float or double precision) : Quadratic distance of KD-Tree points to the queried points
-            * inds (tensor of int64) : Indices (rows of ``points_ref``) of the K closest neighbors
+            * dists (tensor of the tree's dtype, (Q, k)) : squared distances (distances when the tree
+              was built with ``squared_distances=False``), ascending per row
+            * inds (tensor of int64, (Q, k)) : rows of ``points_ref`` of the neighbours
 
         Raises
         ------
@@ -206,6 +211,11 @@ class TorchKDTree:
 
         inds = torch.empty(size=[nq, nr_nns_searches], dtype=torch.int64, device=self.device)
         if self.use_gpu:
+            v = self.variant if self.variant is not None else autotune.variant_for(self.dims, self.dtype,
+                                                                                   nr_nns_searches)
+            if v != self._applied_variant:
+                self.kdtree.set_variant(int(v))
+                self._applied_variant = v
             self.kdtree.query(points_query.data_ptr(), nq, nr_nns_searches, result_dists.data_ptr(),
                               result_idx.data_ptr(), orig_idx_ptr=inds.data_ptr(),
                               tie=_TIE_POLICIES[self.tie_break], presort=_PRESORT[self.presort],
@@ -235,7 +245,7 @@ class TorchKDTree:
 
 def build_kd_tree(points_ref: Union[torch.Tensor, np.ndarray], device: torch.device = None,
                   squared_distances=True, levels: int = None, tie_break: str = "reference", presort="auto",
-                  builder: str = "cpu"):
+                  builder: str = "cpu", variant=None):
     """Builds the KD-Tree for subsequent queries.
 
     By default the tree is built on the CPU (the reference's build, exactly) and then transferred
@@ -264,6 +274,10 @@ def build_kd_tree(points_ref: Union[torch.Tensor, np.ndarray], device: torch.dev
         node has two points with equal coordinates on its split axis; points with such ties may be
         ordered and split differently (the reference orders them by an unstable sort), so
         structured indices can differ while the neighbour distances do not.
+    variant : None (default) or int
+        Launch variant of the GPU traversal (``torch_knn.tree_variants()``): None takes the one
+        ``autotune.tune`` stored for this dims, dtype and k (0, the built-in shape, without one).
+        Results never depend on it.
 
     Returns
     -------
@@ -292,7 +306,7 @@ def build_kd_tree(points_ref: Union[torch.Tensor, np.ndarray], device: torch.dev
     assert device.type in ['cuda', 'cpu']
 
     return TorchKDTree(points_ref, device, levels, squared_distances, tie_break=tie_break, presort=presort,
-                       builder=builder)
+                       builder=builder, variant=variant)
 
 
 def load_kd_tree(path: str, device: torch.device = None, squared_distances=True, tie_break: str = "reference",
