@@ -124,9 +124,23 @@ def _extra_bruteforce(np, torch, torch_knn, dev, w):
     torch.cuda.synchronize()
     ms = _events(torch_knn, "bruteforce", run, 5)
     flops = 8.0 * n * n  # 3 sub + 3 mul + 2 add per (query, reference) pair
+    # lists beyond 32 entries (held in LDS): the same subset at k = 128
+    k2 = 128
+    d2 = torch.empty((n, k2), dtype=torch.float32, device=dev)
+    i2 = torch.empty((n, k2), dtype=torch.int32, device=dev)
+
+    def run128():
+        torch_knn.knn_bruteforce(ref.data_ptr(), n, q.data_ptr(), n, 3, "float32", k2, d2.data_ptr(), i2.data_ptr(),
+                                 stream)
+
+    run128()
+    torch.cuda.synchronize()
+    ms128 = _events(torch_knn, "bruteforce", run128, 3)
     return {"workload": "plain_gpu refs[:65536] x 65,536 queries (32,768 per half), k=16",
             "ms": ms, "tflops": flops / (ms * 1e-3) / 1e12, "peak_tflops": 157.3,
-            "frac": flops / (ms * 1e-3) / 1e12 / 157.3}
+            "frac": flops / (ms * 1e-3) / 1e12 / 157.3,
+            "k128": {"ms": ms128, "tflops": flops / (ms128 * 1e-3) / 1e12,
+                     "frac": flops / (ms128 * 1e-3) / 1e12 / 157.3}}
 
 
 def _extra_fwd_bwd(np, torch, torch_knn, tk, workloads, dev, steps):

@@ -172,7 +172,8 @@ int tk_knn_query_gpu_noalloc(const tk_tree* tree, const void* queries, int64_t n
 
 /* Replaces NmDistanceKernelLauncher[Dynamic] (src/nndistance_g.cu:142-220), which the
  * reference compiles but never binds.  Exhaustive KNN over ref[n_ref*dims] in original
- * order; ties go to the lower index.  idx receives original int32 indices.  k <= 32. */
+ * order; ties go to the lower index; a NaN distance is never inserted.  idx receives original
+ * int32 indices.  1 <= k <= 128 (lists beyond 32 entries are held in LDS). */
 int tk_knn_bruteforce_gpu(const void* ref, int32_t n_ref, const void* queries, int64_t n_queries, int dims, int dtype,
                           int k, void* dist, int32_t* idx, void* workspace, size_t workspace_bytes, void* stream);
 int tk_knn_bruteforce_gpu_workspace(int32_t n_ref, int64_t n_queries, int dims, int dtype, int k, size_t* bytes);

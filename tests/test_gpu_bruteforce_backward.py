@@ -26,7 +26,7 @@ def _bruteforce_gpu(ref, qry, k, device):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("d", [1, 2, 3, 8])
-@pytest.mark.parametrize("k", [1, 4, 16, 32])
+@pytest.mark.parametrize("k", [1, 4, 16, 32, 33, 64, 100, 128])
 def test_bruteforce_matches_oracle(cuda, dtype, d, k):
     rng = np.random.default_rng(300 + d + k)
     ref = (rng.standard_normal((3000, d)) * 1e3).astype(dtype)
@@ -37,8 +37,27 @@ def test_bruteforce_matches_oracle(cuda, dtype, d, k):
     assert_same(gi, oi, "indices")
 
 
+@pytest.mark.parametrize("k", [40, 128])
+@pytest.mark.parametrize("nq", [100, 70000])
+def test_bruteforce_large_k_chunked(cuda, k, nq):
+    """Lists beyond 32 entries with the reference set split over chunks (merged in chunk order) and
+    with one chunk; ties from an integer lattice go to the lower index, as in the oracle."""
+    rng = np.random.default_rng(k + nq)
+    ref = rng.integers(0, 12, (20000, 3)).astype(np.float32)
+    qry = (rng.integers(0, 12, (nq, 3)) + 0.5 * rng.integers(0, 2, (nq, 3))).astype(np.float32)
+    gd, gi = _bruteforce_gpu(ref, qry, k, cuda)
+    od, oi = oracle.bruteforce(ref, qry, k)
+    assert_same(gd, od, "distances")
+    assert_same(gi, oi, "indices")
+
+
+def test_bruteforce_k_above_128_is_refused(cuda):
+    with pytest.raises(RuntimeError, match="128"):
+        _bruteforce_gpu(np.zeros((300, 3), np.float32), np.zeros((5, 3), np.float32), 129, cuda)
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("k", [1, 7, 32])
+@pytest.mark.parametrize("k", [1, 7, 32, 64])
 def test_bruteforce_nan_query_rows(cuda, dtype, k):
     """A query with a NaN coordinate: the reference's brute force inserts only when d < worst
     (src/nndistance_g.cu:57), which a NaN never is, so the row stays (+inf, 0), as in the oracle;
