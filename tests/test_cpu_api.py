@@ -452,3 +452,32 @@ def test_autotune_cache_lookup(tmp_path, monkeypatch):
     table = torch_knn.tree_variants()
     assert table[0] == {"chunk": 0, "waves": 0, "block": 0} and len(table) == 9
     assert all(v["chunk"] in (4, 8, 16) and v["block"] in (64, 128, 256) for v in table[1:])
+
+
+def test_debug_flavour_library_runs_the_reference_cross_checks():
+    """libtorch_kdtree_amd_debug.so (make FLAVOUR=debug, built by __graft_entry__.build()): the same
+    C ABI with the reference's debug cross-checks compiled in; a CPU build and search on a tie-heavy
+    cloud pass them and equal the release library's results."""
+    lib_path = os.path.join(ROOT, "torch_kdtree_amd", "libtorch_kdtree_amd_debug.so")
+    if not os.path.exists(lib_path):
+        pytest.skip("debug flavour not built (make -C torch_kdtree_amd/csrc FLAVOUR=debug)")
+    dbg = ctypes.CDLL(lib_path)
+    rel = ctypes.CDLL(os.path.join(ROOT, "torch_kdtree_amd", "libtorch_kdtree_amd.so"))
+    rng = np.random.default_rng(4)
+    pts = rng.integers(0, 7, (20000, 3)).astype(np.float32)
+    qry = (rng.integers(0, 7, (3000, 3)) + 0.5 * rng.integers(0, 2, (3000, 3))).astype(np.float32)
+    outs = []
+    for lib in (dbg, rel):
+        lib.tk_last_error.restype = ctypes.c_char_p
+        tree = ctypes.c_void_p()
+        assert lib.tk_tree_build(pts.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(len(pts)), 3, 0, 9, 0,
+                                 ctypes.byref(tree)) == 0, lib.tk_last_error()
+        d = np.empty((3000, 12), np.float32)
+        i = np.empty((3000, 12), np.int32)
+        rc = lib.tk_knn_query_cpu(tree, qry.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(3000), 12,
+                                  d.ctypes.data_as(ctypes.c_void_p), i.ctypes.data_as(ctypes.c_void_p), None, 0)
+        assert rc == 0, lib.tk_last_error()
+        lib.tk_tree_free(tree)
+        outs.append((d, i))
+    assert_same(outs[0][0], outs[1][0], "debug vs release distances")
+    assert_same(outs[0][1], outs[1][1], "debug vs release indices")

@@ -31,8 +31,10 @@
 // for each axis a of the levels it walks (it always walks the near path down to the
 // leaf-parent level).  Skipping and applying exactly those resets keeps every later decision
 // identical to the reference's.
+#include <atomic>
 #include <cstring>
 #include <limits>
+#include <string>
 #include <vector>
 
 #include <omp.h>
@@ -138,6 +140,30 @@ struct Searcher {
         }
     }
 
+#ifdef TK_DEBUG_CHECKS
+    // The reference's recursive search (compQuadrDistNode, src/kdtree.cpp:112-152), for the debug
+    // cross-check of src/kdtree.cpp:182-189: no point boxes, and the far side of a node searched
+    // with pp = q except on that node's axis.
+    void recursive(const T* q, int32_t node, int cl, const T* pp, T* bd, int32_t* bi) const {
+        const int a = t.axis[node];
+        const T m = split[node];
+        const bool lower = q[a] < m;
+        if (cl == 0) {
+            const int32_t left_leaf = 2 * node + 1 - t.n_nodes;
+            scan(lower ? left_leaf : left_leaf + 1, q, bd, bi);
+            if (proj_bound<T, D>(q, pp, a, m) < bd[k - 1]) scan(lower ? left_leaf + 1 : left_leaf, q, bd, bi);
+            return;
+        }
+        recursive(q, lower ? 2 * node + 1 : 2 * node + 2, cl - 1, pp, bd, bi);
+        if (proj_bound<T, D>(q, pp, a, m) < bd[k - 1]) {
+            T pp2[D];
+            for (int i = 0; i < D; ++i) pp2[i] = q[i];
+            pp2[a] = m;
+            recursive(q, lower ? 2 * node + 2 : 2 * node + 1, cl - 1, pp2, bd, bi);
+        }
+    }
+#endif
+
     void one(const T* q, T* bd, int32_t* bi) const {
         if (has_nan(q)) {
             nan_query(q, bd, bi);
@@ -221,21 +247,45 @@ void query_typed(const Tree& t, const T* q, int64_t nq, int k, T* dist, int32_t*
                  int n_threads) {
     Searcher<T, D> s{t, t.split_ptr<T>(), t.points_ptr<T>(), t.aabb_ptr<T>(), k, t.finite};
     if (n_threads <= 0) n_threads = omp_get_max_threads();
+#ifdef TK_DEBUG_CHECKS
+    // Trees with a non-finite value are left out of the cross-check: there the two searches' different
+    // projected points can legitimately differ (a point with a NaN coordinate behind a bound one prunes).
+    std::atomic<int64_t> mismatch{-1};
+#endif
 #pragma omp parallel num_threads(n_threads)
     {
         std::vector<T> bd(k);
         std::vector<int32_t> bi(k);
+#ifdef TK_DEBUG_CHECKS
+        std::vector<T> bd2(k);
+        std::vector<int32_t> bi2(k);
+#endif
 #pragma omp for schedule(dynamic, 64)
         for (int64_t i = 0; i < nq; ++i) {
             std::fill(bd.begin(), bd.end(), std::numeric_limits<T>::infinity());
             std::fill(bi.begin(), bi.end(), 0);
             s.one(q + i * D, bd.data(), bi.data());
+#ifdef TK_DEBUG_CHECKS
+            if (t.finite) {
+                std::fill(bd2.begin(), bd2.end(), std::numeric_limits<T>::infinity());
+                std::fill(bi2.begin(), bi2.end(), 0);
+                s.recursive(q + i * D, 0, t.levels - 1, q + i * D, bd2.data(), bi2.data());
+                if (std::memcmp(bd.data(), bd2.data(), sizeof(T) * (size_t)k) != 0 ||
+                    std::memcmp(bi.data(), bi2.data(), sizeof(int32_t) * (size_t)k) != 0)
+                    mismatch = i;
+            }
+#endif
             std::memcpy(dist + i * k, bd.data(), sizeof(T) * (size_t)k);
             std::memcpy(idx + i * k, bi.data(), sizeof(int32_t) * (size_t)k);
             if (orig)
                 for (int j = 0; j < k; ++j) orig[i * k + j] = t.shuffled[bi[j]];
         }
     }
+#ifdef TK_DEBUG_CHECKS
+    if (mismatch >= 0)
+        throw std::runtime_error("debug check: the iterative and the recursive search differ at query " +
+                                 std::to_string((long long)mismatch.load()));
+#endif
 }
 
 }  // namespace

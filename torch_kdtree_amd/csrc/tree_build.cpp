@@ -205,6 +205,24 @@ Tree* build_tree(const void* points, int64_t n_points, int dims, int levels, int
         }
     }
 #undef TK_BUILD
+#ifdef TK_DEBUG_CHECKS
+    // The reference's sanity checks after createKDTree (src/kdtree.hpp:517-529): leaves contiguous,
+    // each starting where the previous one ends, all 2^levels of them; plus the index permutation.
+    if (t->leaf_off[0] != 0 || t->leaf_off[t->n_leaves] != t->n_points)
+        throw std::runtime_error("debug check: leaf offsets do not span the points");
+    for (int32_t j = 0; j < t->n_leaves; ++j)
+        if (t->leaf_off[j + 1] < t->leaf_off[j]) throw std::runtime_error("debug check: leaf offsets decrease");
+    std::vector<char> seen((size_t)n_points, 0);
+    for (int64_t i = 0; i < n_points; ++i) {
+        const int32_t o = t->shuffled[i];
+        if (o < 0 || o >= n_points || seen[(size_t)o]) throw std::runtime_error("debug check: shuffled is no permutation");
+        seen[(size_t)o] = 1;
+        if (std::memcmp(&t->points[(size_t)i * dims * dtype_size(dtype)],
+                        static_cast<const unsigned char*>(points) + (size_t)o * dims * dtype_size(dtype),
+                        (size_t)dims * dtype_size(dtype)) != 0)
+            throw std::runtime_error("debug check: structured point differs from its original row");
+    }
+#endif
     return t.release();
 }
 
