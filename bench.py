@@ -240,10 +240,20 @@ def _extra_stress(np, torch, torch_knn, tk, workloads, dev, reps):
     tr, n = torch_knn.profile_read("traverse")
     bw, m = torch_knn.profile_read("backward")
     torch_knn.profile_reset()
+    # work of the search: points scanned per query (host search on a sample; the same pruning
+    # decisions as the GPU's), and the distance arithmetic it implies, 3d - 1 = 23 flops per point
+    sample = np.ascontiguousarray(w["queries"][:1 << 15])
+    leaves, points = tree.kdtree.search_stats(sample, w["k"], 16)
+    ppq = points / sample.shape[0]
+    tr_ms = tr / max(n, 1)
+    useful = 23.0 * ppq * q.shape[0] / (tr_ms * 1e-3) / 1e12
     out["d8_fwd_bwd"] = {"workload": "d8_mixture: d=8 fp32, 2,000,000 refs and queries from a 64-component "
                                      "anisotropic mixture, k=32, loss sum(w*dists), grads w.r.t. queries and refs",
                          "levels": tree.levels, "fwd_bwd_per_s": q.shape[0] * reps / el,
-                         "ms_per_step": el * 1e3 / reps, "traverse_ms": tr / max(n, 1), "backward_ms": bw / max(m, 1)}
+                         "ms_per_step": el * 1e3 / reps, "traverse_ms": tr_ms, "backward_ms": bw / max(m, 1),
+                         "leaves_per_query": leaves / sample.shape[0], "points_per_query": ppq,
+                         "distance_tflops": useful, "distance_frac_of_fp32_peak": useful / 157.3,
+                         "work_sample": "first %d queries, host search" % sample.shape[0]}
     del tree, ref, q, wt
     w = workloads.fp64_normal()
     tree = tk.build_kd_tree(torch.from_numpy(w["ref"]), device=dev, levels="auto")
@@ -479,6 +489,16 @@ def main():
     achieved = launch_bytes / (trav_avg_ms * 1e-3) / 1e9
     log("measuring the achievable bandwidth (streaming copy, 1 GiB)")
     achievable = _achievable_gbs(torch, torch_knn, dev)
+    # work of the search: leaves and points scanned per query (host search of a balanced sample, the
+    # same pruning decisions as the GPU's), and the distance arithmetic it implies (8 flops per point)
+    half = w["halves"][0]
+    wsample = np.ascontiguousarray(np.concatenate([qry[:1 << 17], qry[half:half + (1 << 17)]]))
+    w_leaves, w_points = tree.kdtree.search_stats(wsample, k, 16)
+    ppq = w_points / wsample.shape[0]
+    work = {"leaves_per_query": w_leaves / wsample.shape[0], "points_per_query": ppq,
+            "distance_tflops": 8.0 * ppq * nq / (trav_avg_ms * 1e-3) / 1e12,
+            "sample": "first 131,072 queries of each half, host search (same pruning decisions as the GPU)"}
+    work["distance_frac_of_fp32_peak"] = work["distance_tflops"] / 157.3
 
     cpu = None
     if args.cpu_sample > 0:
@@ -564,7 +584,8 @@ def main():
                      "bytes_per_query": bytes_per_query, "tree_bytes": tree_bytes,
                      "traverse_ms": trav_avg_ms,
                      "achievable_gbs": achievable, "frac_of_achievable": achieved / achievable,
-                     "achieved_with_fused_remap": fused_bytes / (trav_avg_ms * 1e-3) / 1e9},
+                     "achieved_with_fused_remap": fused_bytes / (trav_avg_ms * 1e-3) / 1e9,
+                     "search_work": work},
         "cpu_baseline": cpu,
         "secondary": extras,
     }

@@ -48,7 +48,7 @@ enum { TK_TIE_REFERENCE = 0, TK_TIE_LOWER_INDEX = 1 };
 /* Query presort before the traversal, so that the lanes of a wavefront get nearby queries:
  * TK_PRESORT_MORTON orders queries along a space-filling curve over a grid on the queries'
  * bounding box (the Hilbert curve for d <= 3, the Morton code for d = 8; the name is kept),
- * TK_PRESORT_LEAF by the tree leaf they fall into; AUTO = MORTON from 4096 queries up.  The
+ * TK_PRESORT_LEAF by the tree leaf they fall into; AUTO = MORTON (LEAF at d = 8) from 4096 queries up.  The
  * presort changes the order of work only, never the results. */
 enum { TK_PRESORT_AUTO = -1, TK_PRESORT_OFF = 0, TK_PRESORT_MORTON = 1, TK_PRESORT_LEAF = 2 };
 
@@ -148,6 +148,13 @@ int tk_tree_get_nodes(const tk_tree* tree, int32_t* axis, void* split, int32_t* 
  * structured; orig_idx[nq*k] (may be NULL) original indices. */
 int tk_knn_query_cpu(const tk_tree* tree, const void* queries, int64_t n_queries, int k, void* dist, int32_t* idx,
                      int64_t* orig_idx, int n_threads);
+
+/* Search statistics (no counterpart in the reference): runs the host search of the n_queries host
+ * queries on the tree's host arrays (CPU or GPU tree) and returns out[0] = leaves scanned and
+ * out[1] = points scanned, summed over the queries.  The host search makes the same pruning
+ * decisions as the GPU traversal under the reference tie policy, so these are the GPU's work too. */
+int tk_knn_search_stats(const tk_tree* tree, const void* queries, int64_t n_queries, int k, int64_t out[2],
+                        int n_threads);
 
 /* Replaces KDTree::query / query_recast for GPU trees (src/interface.cpp:60-86) ->
  * KDTreeKNNGPUSearch (src/kdtree_g.cu:409-457).  Device pointers.  Writes squared

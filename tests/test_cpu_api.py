@@ -481,3 +481,17 @@ def test_debug_flavour_library_runs_the_reference_cross_checks():
         outs.append((d, i))
     assert_same(outs[0][0], outs[1][0], "debug vs release distances")
     assert_same(outs[0][1], outs[1][1], "debug vs release indices")
+
+
+def test_search_stats_counts_scanned_points():
+    """tk_knn_search_stats: with k equal to the cloud size nothing can be pruned, so every query scans
+    both leaves of a one-level tree; with small k on a deep tree it scans few points, at least k."""
+    rng = np.random.default_rng(2)
+    ref = rng.random((500, 3)).astype(np.float32)
+    qry = rng.random((40, 3)).astype(np.float32)
+    t = tk.build_kd_tree(torch.from_numpy(ref), device="cpu", levels=1)
+    assert t.kdtree.search_stats(qry, 500) == (80, 40 * 500)
+    big = rng.random((100000, 3)).astype(np.float32)
+    t2 = tk.build_kd_tree(torch.from_numpy(big), device="cpu", levels=12)
+    leaves, points = t2.kdtree.search_stats(qry, 8)
+    assert 40 <= leaves <= 40 * 64 and 40 * 8 <= points <= 40 * 3000

@@ -262,6 +262,18 @@ int tk_knn_query_cpu(const tk_tree* tree, const void* queries, int64_t n_queries
     });
 }
 
+int tk_knn_search_stats(const tk_tree* tree, const void* queries, int64_t n_queries, int k, int64_t out[2],
+                        int n_threads) {
+    return guarded([&] {
+        const tk::Tree& t = tree_of(tree);
+        if (n_queries < 0 || !out) throw std::runtime_error("invalid arguments");
+        const size_t kk = (size_t)std::max(k, 1);
+        std::vector<unsigned char> dist((size_t)n_queries * kk * tk::dtype_size(t.dtype));
+        std::vector<int32_t> idx((size_t)n_queries * kk);
+        tk::query_cpu(t, queries, n_queries, k, dist.data(), idx.data(), nullptr, n_threads, out);
+    });
+}
+
 int tk_knn_query_gpu(const tk_tree* tree, const void* queries, int64_t n_queries, int k, void* dist, int32_t* idx,
                      int64_t* orig_idx, int tie, int presort, void* workspace, size_t workspace_bytes, void* stream) {
     return guarded([&] {

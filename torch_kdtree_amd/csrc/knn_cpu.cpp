@@ -44,6 +44,9 @@
 namespace tk {
 namespace {
 
+// Search statistics (leaves and points scanned), counted per thread when asked for.
+thread_local int64_t tl_leaves = 0, tl_points = 0;
+
 template <typename T, int D>
 inline T sq_dist(const T* q, const T* r) {
     T s = T(0);
@@ -101,6 +104,8 @@ struct Searcher {
 
     void scan(int32_t leaf, const T* q, T* bd, int32_t* bi) const {
         const int32_t beg = t.leaf_off[leaf], end = t.leaf_off[leaf + 1];
+        ++tl_leaves;
+        tl_points += end - beg;
         for (int32_t s = beg; s < end; ++s) {
             const T d = sq_dist<T, D>(q, pts + (int64_t)s * D);
             if (d >= bd[k - 1]) continue;
@@ -244,7 +249,7 @@ struct Searcher {
 
 template <typename T, int D>
 void query_typed(const Tree& t, const T* q, int64_t nq, int k, T* dist, int32_t* idx, int64_t* orig,
-                 int n_threads) {
+                 int n_threads, int64_t* stats) {
     Searcher<T, D> s{t, t.split_ptr<T>(), t.points_ptr<T>(), t.aabb_ptr<T>(), k, t.finite};
     if (n_threads <= 0) n_threads = omp_get_max_threads();
 #ifdef TK_DEBUG_CHECKS
@@ -252,8 +257,10 @@ void query_typed(const Tree& t, const T* q, int64_t nq, int k, T* dist, int32_t*
     // projected points can legitimately differ (a point with a NaN coordinate behind a bound one prunes).
     std::atomic<int64_t> mismatch{-1};
 #endif
+    std::atomic<int64_t> leaves{0}, points{0};
 #pragma omp parallel num_threads(n_threads)
     {
+        tl_leaves = tl_points = 0;
         std::vector<T> bd(k);
         std::vector<int32_t> bi(k);
 #ifdef TK_DEBUG_CHECKS
@@ -280,6 +287,12 @@ void query_typed(const Tree& t, const T* q, int64_t nq, int k, T* dist, int32_t*
             if (orig)
                 for (int j = 0; j < k; ++j) orig[i * k + j] = t.shuffled[bi[j]];
         }
+        leaves += tl_leaves;
+        points += tl_points;
+    }
+    if (stats) {
+        stats[0] = leaves.load();
+        stats[1] = points.load();
     }
 #ifdef TK_DEBUG_CHECKS
     if (mismatch >= 0)
@@ -291,11 +304,12 @@ void query_typed(const Tree& t, const T* q, int64_t nq, int k, T* dist, int32_t*
 }  // namespace
 
 void query_cpu(const Tree& t, const void* queries, int64_t n_queries, int k, void* dist, int32_t* idx,
-               int64_t* orig_idx, int n_threads) {
+               int64_t* orig_idx, int n_threads, int64_t* stats) {
     if (k < 1) throw std::runtime_error("nr_nns_searches must be >= 1");
     if (k > t.n_points) throw std::runtime_error("You requested more nearest neighbors than there are in the KD-Tree");
 #define TK_Q(TT, DD) \
-    query_typed<TT, DD>(t, static_cast<const TT*>(queries), n_queries, k, static_cast<TT*>(dist), idx, orig_idx, n_threads)
+    query_typed<TT, DD>(t, static_cast<const TT*>(queries), n_queries, k, static_cast<TT*>(dist), idx, orig_idx, n_threads, \
+                        stats)
     if (t.dtype == kF32) {
         switch (t.dims) {
             case 1: TK_Q(float, 1); break;

@@ -351,7 +351,11 @@ inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 int presort_mode(const Tree& t, int64_t nq, const QueryOpts& o) {
     if (nq <= 0) return kPresortOff;
-    if (o.presort == kPresortAuto) return (nq >= kPresortMinQueries && t.levels > 1) ? kPresortMorton : kPresortOff;
+    // auto: the space-filling curve at d <= 3; the near-leaf order at d = 8, where the curve over a
+    // 3-bit-per-axis grid groups queries more coarsely (d8_mixture, 2M queries, k = 32, depth 16:
+    // 222 ms per query step against 231 ms, and 246 ms without a presort)
+    if (o.presort == kPresortAuto)
+        return (nq >= kPresortMinQueries && t.levels > 1) ? (t.dims == 8 ? kPresortLeaf : kPresortMorton) : kPresortOff;
     if (o.presort != kPresortOff && o.presort != kPresortMorton && o.presort != kPresortLeaf)
         throw std::runtime_error("unknown presort mode");
     return o.presort;

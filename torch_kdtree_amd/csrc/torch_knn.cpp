@@ -97,6 +97,18 @@ struct KDTree {
         return v;
     }
 
+    // Leaves and points scanned by the search of host queries (summed), see tk_knn_search_stats.
+    py::tuple search_stats(const py::array_t<T, py::array::c_style | py::array::forcecast>& q, int k, int n_threads) {
+        int64_t out[2] = {0, 0};
+        int rc;
+        {
+            py::gil_scoped_release release;
+            rc = tk_knn_search_stats(tree, q.data(), q.shape(0), k, out, n_threads);
+        }
+        check(rc);
+        return py::make_tuple(out[0], out[1]);
+    }
+
     bool is_finite() const {
         int32_t info[6];
         check(tk_tree_info(tree, info));
@@ -316,6 +328,8 @@ void bind_tree(py::module_& m, const char* name) {
         .def("set_variant", &C::set_variant, py::arg("variant"),
              "Launch variant of later GPU queries (see torch_knn.tree_variants())")
         .def("get_variant", &C::get_variant)
+        .def("search_stats", &C::search_stats, py::arg("queries"), py::arg("k"), py::arg("n_threads") = 0,
+             "(leaves, points) scanned by the search of these host queries, summed")
         .def("query", &C::query, py::arg("points_query_ptr"), py::arg("nr_query_points"), py::arg("nr_nns_searches"),
              py::arg("dist_arr_ptr"), py::arg("knn_idx_ptr"), py::kw_only(), py::arg("orig_idx_ptr") = 0,
              py::arg("tie") = 0, py::arg("presort") = -1, py::arg("stream") = 0, py::arg("n_threads") = 0,

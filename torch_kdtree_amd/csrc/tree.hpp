@@ -87,7 +87,8 @@ Tree* load_tree(const char* path);
 
 // CPU KNN search with the reference's semantics (iterative near-first DFS, upper-bound
 // insertion; src/kdtree.cpp:16-194).
+// stats (may be null): stats[0] = leaves scanned, stats[1] = points scanned, over all queries.
 void query_cpu(const Tree& t, const void* queries, int64_t n_queries, int k, void* dist, int32_t* idx,
-               int64_t* orig_idx, int n_threads);
+               int64_t* orig_idx, int n_threads, int64_t* stats = nullptr);
 
 }  // namespace tk
