@@ -320,6 +320,20 @@ def _extra_batch(np, torch, torch_knn, tk, workloads, dev, reps):
         loop()
     torch.cuda.synchronize()
     loop_ms = (time.perf_counter() - t0) * 1e3 / reps
+    # the same step with the other traversal launch shape (persistent work queues / one launch per k bucket)
+    batch_pb = tk.build_kd_tree_batch(refs, device=dev, levels=batch.levels, persistent=not batch.persistent)
+    step_pb = tk.BatchStep(batch_pb, [torch.from_numpy(q) for q in qs], ks, [torch.from_numpy(w) for w in ws])
+    step_pb.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step_pb.replay()
+    torch.cuda.synchronize()
+    per_bucket_ms = (time.perf_counter() - t0) * 1e3 / reps
+    same_pb = all(torch.equal(a, b) for a, b in ((step.dist, step_pb.dist), (step.idx, step_pb.idx),
+                                                (step.grad_queries, step_pb.grad_queries),
+                                                (step.grad_refs, step_pb.grad_refs)))
+    del step_pb, batch_pb
     same = (torch.equal(step.dist, torch.cat([b[3].reshape(-1) for b in bufs]))
             and torch.equal(step.orig, torch.cat([b[5].reshape(-1) for b in bufs]))
             and torch.equal(step.grad_queries, torch.cat([b[7] for b in bufs]))
@@ -331,6 +345,9 @@ def _extra_batch(np, torch, torch_knn, tk, workloads, dev, reps):
                         "k in {1,4,16}; fwd + bwd, loss sum(w * dists)",
             "clouds": len(clouds), "sum_refs": sum_n, "sum_queries": sum_q, "plan_and_capture_s": plan_s,
             "batched_graph_ms": batched_ms, "batched_fwd_bwd_per_s": sum_q / (batched_ms * 1e-3),
+            "traversal_launch": "persistent" if batch.persistent else "per k bucket",
+            "other_launch_shape": "per k bucket" if batch.persistent else "persistent",
+            "other_launch_shape_graph_ms": per_bucket_ms, "other_launch_shape_same_results": bool(same_pb),
             "per_cloud_loop_ms": loop_ms, "per_cloud_fwd_bwd_per_s": sum_q / (loop_ms * 1e-3),
             "batched_vs_per_cloud": loop_ms / batched_ms, "same_results": bool(same),
             "batched_algorithmic_gbs": alg / (batched_ms * 1e-3) / 1e9}

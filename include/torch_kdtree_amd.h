@@ -233,6 +233,10 @@ int tk_batch_backward(tk_batch* batch, const void* queries, const int32_t* idx, 
 int tk_batch_capture(tk_batch* batch, const void* queries, void* dist, int32_t* idx, int64_t* orig_idx,
                      const void* grad_dist, void* grad_queries, void* grad_ref, void* stream);
 int tk_batch_launch(tk_batch* batch, void* stream);
+/* Traversal launch shape of the plan's later forwards and captures: 0 (default) one launch per k
+ * bucket; 1 one persistent launch for every bucket, whose wavefronts take units of 64 presorted
+ * queries from per-XCD work queues until all are done.  Results are identical either way. */
+int tk_batch_set_persistent(tk_batch* batch, int on);
 
 /* Stable radix sort of n (key, value) uint32 pairs on the GPU by the low key_bits bits of the
  * keys (no counterpart in the reference: it is the sort behind the deterministic grad_ref of

@@ -323,3 +323,35 @@ def test_batch_refuses_nonfinite_clouds_under_reference_policy(cuda):
         bd, bi = oracle.bruteforce(pts, qry, 4)
         assert_same(d.cpu().numpy(), bd, "distances")
         assert_same(i.cpu().numpy(), bi.astype(np.int64), "indices")
+
+
+@pytest.mark.parametrize("tie", ["reference", "lower_index"])
+def test_persistent_launch_equals_per_bucket_launches(cuda, tie):
+    """The persistent traversal (one launch, per-XCD work queues of 64-query units) gives the
+    per-bucket launches' outputs bit for bit, eager and in a captured fwd+bwd replayed twice, with k
+    buckets 1 to 32 present."""
+    rng = np.random.default_rng(404 + (tie == "lower_index"))
+    clouds, queries, _ = _clouds(rng, 14, 3, np.float32, sizes=(40, 9000))
+    ks = [[1, 2, 3, 4, 7, 8, 12, 16, 20, 32][c % 10] for c in range(len(clouds))]
+    ks = [min(k, c.shape[0]) for k, c in zip(ks, clouds)]
+    weights = [torch.from_numpy(rng.random((q.shape[0], k)).astype(np.float32)) for q, k in zip(queries, ks)]
+    qt = [torch.from_numpy(q) for q in queries]
+    outs = {}
+    for persistent in (False, True):
+        batch = tk.build_kd_tree_batch(clouds, device=cuda, tie_break=tie, persistent=persistent)
+        step = tk.BatchStep(batch, qt, ks, weights)
+        got = []
+        for _ in range(2):
+            step.replay()
+            torch.cuda.synchronize()
+            got.append([t.clone() for t in (step.dist, step.idx, step.orig, step.grad_queries, step.grad_refs)])
+        for a, b in zip(got[0], got[1]):
+            assert torch.equal(a, b), "replays differ"
+        plan = batch.new_plan([q.shape[0] for q in queries], ks)
+        total_q, total_out, _, _ = plan.sizes()
+        q_cat = torch.cat([t.to(cuda) for t in qt])
+        ed, ei, eo = _plan_forward(plan, q_cat, total_out, torch.float32, cuda)
+        outs[persistent] = got[0] + [ed, ei, eo]
+    for what, a, b in zip(("dist", "idx", "orig", "dL/dq", "dL/dr", "eager dist", "eager idx", "eager orig"),
+                          outs[False], outs[True]):
+        assert torch.equal(a, b), "persistent vs per-bucket: " + what

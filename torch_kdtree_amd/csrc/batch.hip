@@ -294,6 +294,7 @@ BatchPlan* batch_create(const Tree* const* trees, int n_clouds, const int64_t* n
     dmalloc(&p->vals_s, 4 * no);
     p->radix_ws_bytes = std::max(radix_sort_workspace((int64_t)no), radix_sort_workspace((int64_t)nq));
     dmalloc(reinterpret_cast<unsigned char**>(&p->radix_ws), p->radix_ws_bytes);
+    dmalloc(&p->work, 4 * 8);
     batch_init_qcloud(*p, nullptr);
     TK_HIP_CHECK(hipDeviceSynchronize());
     return p.release();
@@ -329,6 +330,7 @@ void batch_free(BatchPlan* p) noexcept {
     dfree(p->keys_s);
     dfree(p->vals_s);
     dfree(p->radix_ws);
+    dfree(p->work);
     if (prev >= 0) (void)hipSetDevice(prev);
     delete p;
 }

@@ -85,6 +85,8 @@ struct BatchPlan {
     uint32_t* vals = nullptr;
     uint32_t* keys_s = nullptr;
     uint32_t* vals_s = nullptr;
+    bool persistent = false;    // one work-queue traversal launch for every k bucket
+    uint32_t* work = nullptr;   // [8] work counters of the persistent traversal
     int key_bits = 1;           // bits of the presort keys
     int pair_bits = 1;          // bits the dL/dr pair sort orders by
     int pair_tag_shift = 0;     // pair keys: 0 = arena record; else cloud << shift | record within the cloud

@@ -58,29 +58,45 @@ __device__ __forceinline__ uint32_t pair_key(int32_t cloud, int32_t local, int32
     return tag_shift ? (((uint32_t)cloud << tag_shift) | (uint32_t)local) : (uint32_t)(rec_base + local);
 }
 
-// One lane per query of the k bucket K; `order` holds the presorted query rows.
-template <typename T, int D, int K, bool kLower>
-__global__ __launch_bounds__(kTreeBlock) TK_BATCH_OCCUPANCY(T, D, K, kLower) void knn_batch_kernel(TreeView<T, true> arena,
-                                                                const CloudDesc* __restrict__ clouds,
-                                                                const int32_t* __restrict__ qcloud,
-                                                                const T* __restrict__ queries,
-                                                                const int32_t* __restrict__ order, int64_t begin,
-                                                                int64_t count, T* __restrict__ out_d,
-                                                                int32_t* __restrict__ out_i, int64_t* __restrict__ out_o,
-                                                                uint32_t* __restrict__ pair_keys,
-                                                                uint32_t* __restrict__ pair_vals,
-                                                                T* __restrict__ q_sorted, int tag_shift,
-                                                                const T* __restrict__ fuse_g,
-                                                                T* __restrict__ fuse_dq, T* __restrict__ g_sorted) {
-    constexpr int CH = StageBytes<T, D, K, kLower, true>::kChunk;
-    constexpr int LD = StageBytes<T, D, K, kLower, true>::kLoads;
-    static_assert(CH <= kRecPad, "a chunk may read at most kRecPad records past a leaf's end");
-    __shared__ T stage_d[CH * kTreeBlock];
-    __shared__ int32_t stage_o[kLower ? CH * kTreeBlock : 1];
-    const int64_t t = xcd_tile(blockIdx.x, gridDim.x) * kTreeBlock + threadIdx.x;
-    if (t >= count) return;
-    const int64_t qi = order[begin + t];
-    const int32_t cloud = qcloud[qi];
+// Arguments of the batched traversal shared by its per-bucket and persistent kernels.
+template <typename T>
+struct BatchArgs {
+    TreeView<T, true> arena;
+    const CloudDesc* __restrict__ clouds;
+    const int32_t* __restrict__ qcloud;
+    const T* __restrict__ queries;
+    const int32_t* __restrict__ order;
+    T* __restrict__ out_d;
+    int32_t* __restrict__ out_i;
+    int64_t* __restrict__ out_o;
+    uint32_t* __restrict__ pair_keys;
+    uint32_t* __restrict__ pair_vals;
+    T* __restrict__ q_sorted;
+    int tag_shift;
+    const T* __restrict__ fuse_g;
+    T* __restrict__ fuse_dq;
+    T* __restrict__ g_sorted;
+};
+
+// The search of the query at sorted position tpos (in the k bucket K), with its outputs.
+template <typename T, int D, int K, bool kLower, int CH, int LD>
+__device__ __forceinline__ void batch_lane(const BatchArgs<T>& ba, int64_t tpos, T* __restrict__ st_d,
+                                           int32_t* __restrict__ st_o) {
+    const TreeView<T, true>& arena = ba.arena;
+    const CloudDesc* __restrict__ clouds = ba.clouds;
+    const T* __restrict__ queries = ba.queries;
+    T* __restrict__ out_d = ba.out_d;
+    int32_t* __restrict__ out_i = ba.out_i;
+    int64_t* __restrict__ out_o = ba.out_o;
+    uint32_t* __restrict__ pair_keys = ba.pair_keys;
+    uint32_t* __restrict__ pair_vals = ba.pair_vals;
+    T* __restrict__ q_sorted = ba.q_sorted;
+    const int tag_shift = ba.tag_shift;
+    const T* __restrict__ fuse_g = ba.fuse_g;
+    T* __restrict__ fuse_dq = ba.fuse_dq;
+    T* __restrict__ g_sorted = ba.g_sorted;
+    const int64_t qi = ba.order[tpos];
+    const int32_t cloud = ba.qcloud[qi];
     const CloudDesc cd = clouds[cloud];
     TreeView<T, true> tv = arena;
     tv.node_base = cd.node_base;
@@ -100,8 +116,7 @@ __global__ __launch_bounds__(kTreeBlock) TK_BATCH_OCCUPANCY(T, D, K, kLower) voi
     if (has_nan<T, D>(q))
         nan_fill<T, D, K, kLower>(tv, q, k, kb);
     else
-        while (!lane_step<T, D, K, kLower, CH, LD>(tv, q, pp, kb, far_mask, node, level, stage_d + threadIdx.x,
-                                                    stage_o + (kLower ? threadIdx.x : 0))) {
+        while (!lane_step<T, D, K, kLower, CH, LD>(tv, q, pp, kb, far_mask, node, level, st_d, st_o)) {
         }
     // keys: reference policy = arena record position, lower-index policy = the cloud's original index.
     // Every gather (index maps, and the neighbour records of the fused dL/dq) is issued for all K
@@ -112,7 +127,6 @@ __global__ __launch_bounds__(kTreeBlock) TK_BATCH_OCCUPANCY(T, D, K, kLower) voi
     const int pad = K - k;
     const int64_t ob = cd.out_base + (qi - cd.q_base) * k;
     // training outputs: this query at its sorted position, its pairs at their canonical slots
-    const int64_t tpos = begin + t;
     const int64_t sb = cd.sorted_out_base + (tpos - cd.sorted_q_base) * k;
     if (q_sorted) {
 #pragma unroll
@@ -168,6 +182,99 @@ __global__ __launch_bounds__(kTreeBlock) TK_BATCH_OCCUPANCY(T, D, K, kLower) voi
         }
 #pragma unroll
         for (int a = 0; a < D; ++a) fuse_dq[qi * D + a] = acc[a];
+    }
+}
+
+// One lane per query of the k bucket K (sorted positions begin .. begin + count).
+template <typename T, int D, int K, bool kLower>
+__global__ __launch_bounds__(kTreeBlock) TK_BATCH_OCCUPANCY(T, D, K, kLower) void knn_batch_kernel(
+    BatchArgs<T> ba, int64_t begin, int64_t count) {
+    constexpr int CH = StageBytes<T, D, K, kLower, true>::kChunk;
+    constexpr int LD = StageBytes<T, D, K, kLower, true>::kLoads;
+    static_assert(CH <= kRecPad, "a chunk may read at most kRecPad records past a leaf's end");
+    __shared__ T stage_d[CH * kTreeBlock];
+    __shared__ int32_t stage_o[kLower ? CH * kTreeBlock : 1];
+    const int64_t t = xcd_tile(blockIdx.x, gridDim.x) * kTreeBlock + threadIdx.x;
+    if (t >= count) return;
+    batch_lane<T, D, K, kLower, CH, LD>(ba, begin + t, stage_d + threadIdx.x, stage_o + (kLower ? threadIdx.x : 0));
+}
+
+// ---------------------------------------------------------------------------------------------
+// Persistent traversal: one launch for every k bucket.  The sorted order is cut into units of 64
+// queries (one wavefront), each inside one bucket, largest lists first.  Units are dealt to the
+// kXcdQueues XCDs (workgroup b runs on XCD b % 8) in runs of kUnitRun consecutive units, run r
+// going to XCD r % 8, so every XCD gets a share of every bucket and neighbouring units, which read
+// the same leaves, share one L2.  Each XCD has a counter over its own units; a wavefront takes the
+// next one with one atomic add, and once its XCD's units are exhausted those of the other XCDs in
+// turn.  Every wavefront loops until all are exhausted, so the launch drains.  The bucket of a
+// unit is wavefront-uniform.
+// ---------------------------------------------------------------------------------------------
+constexpr int kXcdQueues = 8;
+constexpr int kBatchMaxRanges = 6;
+constexpr int64_t kUnitRun = 256;
+
+// The c-th unit of XCD x.
+__host__ __device__ __forceinline__ int64_t xcd_unit(int x, int64_t c) {
+    return ((c / kUnitRun) * kXcdQueues + x) * kUnitRun + c % kUnitRun;
+}
+
+struct BatchUnits {
+    int n_ranges;
+    int K[kBatchMaxRanges];
+    int64_t begin[kBatchMaxRanges];  // sorted position of the range's first query
+    int64_t count[kBatchMaxRanges];
+    int64_t unit0[kBatchMaxRanges];  // first unit of the range
+    int64_t n_units;
+};
+
+// Work counters of the persistent traversal (one per XCD) back to zero.
+__global__ void batch_units_reset_kernel(uint32_t* __restrict__ counters) {
+    if (threadIdx.x < kXcdQueues) counters[threadIdx.x] = 0;
+}
+
+template <typename T, int D, int KMAX, bool kLower>
+__global__ __launch_bounds__(kTreeBlock) __attribute__((amdgpu_waves_per_eu(TK_BATCH_WAVES_K16 > 0 ? TK_BATCH_WAVES_K16 : 1))) void knn_batch_persistent_kernel(
+    BatchArgs<T> ba, BatchUnits u, uint32_t* __restrict__ counters) {
+    constexpr int CHMAX = StageBytes<T, D, KMAX, kLower, true>::kChunk > StageBytes<T, D, 1, kLower, true>::kChunk
+                              ? StageBytes<T, D, KMAX, kLower, true>::kChunk
+                              : StageBytes<T, D, 1, kLower, true>::kChunk;
+    __shared__ T stage_d[CHMAX * kTreeBlock];
+    __shared__ int32_t stage_o[kLower ? CHMAX * kTreeBlock : 1];
+    const int lane = threadIdx.x & 63;
+    const int home = blockIdx.x % kXcdQueues;
+    T* st_d = stage_d + threadIdx.x;
+    int32_t* st_o = stage_o + (kLower ? threadIdx.x : 0);
+    for (int probe = 0; probe < kXcdQueues;) {
+        const int x = (home + probe) % kXcdQueues;
+        uint32_t got = 0;
+        if (lane == 0) got = atomicAdd(&counters[x], 1u);
+        const int64_t unit = xcd_unit(x, (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)got));
+        if (unit >= u.n_units) {
+            ++probe;  // this XCD's units are exhausted: take the next XCD's
+            continue;
+        }
+        int r = 0;
+        while (r + 1 < u.n_ranges && unit >= u.unit0[r + 1]) ++r;
+        const int64_t t = (unit - u.unit0[r]) * 64 + lane;
+        if (t < u.count[r]) {
+            const int64_t tpos = u.begin[r] + t;
+#define TK_PERSIST_K(KK)                                                                                          \
+    case KK:                                                                                                      \
+        if constexpr (KK <= KMAX)                                                                                 \
+            batch_lane<T, D, KK, kLower, StageBytes<T, D, KK, kLower, true>::kChunk,                              \
+                       StageBytes<T, D, KK, kLower, true>::kLoads>(ba, tpos, st_d, st_o);                         \
+        break;
+            switch (u.K[r]) {
+                TK_PERSIST_K(1)
+                TK_PERSIST_K(2)
+                TK_PERSIST_K(4)
+                TK_PERSIST_K(8)
+                TK_PERSIST_K(16)
+                TK_PERSIST_K(32)
+                default: break;
+            }
+#undef TK_PERSIST_K
+        }
     }
 }
 
@@ -418,15 +525,50 @@ void batch_traverse_td(const BatchPlan& p, const T* q, T* dist, int32_t* idx, in
     uint32_t* pk = emit ? p.keys : nullptr;
     uint32_t* pv = emit ? p.vals : nullptr;
     T* qs = emit ? static_cast<T*>(p.q_sorted) : nullptr;
-    const TreeView<T, true> arena = batch_arena_view<T>(p);
+    const BatchArgs<T> ba{batch_arena_view<T>(p), p.d_clouds, p.qcloud, q, p.order, dist, idx, orig, pk, pv, qs,
+                          p.pair_tag_shift, g, dq, gs};
+    if (p.persistent) {
+        // units of 64 queries per range, largest lists first (the longest work starts first)
+        BatchUnits u{};
+        int kmax = 1;
+        for (int i = (int)p.ranges.size() - 1; i >= 0; --i) {
+            const BatchPlan::Range& r = p.ranges[(size_t)i];
+            if (r.count == 0) continue;
+            u.K[u.n_ranges] = r.K;
+            u.begin[u.n_ranges] = r.begin;
+            u.count[u.n_ranges] = r.count;
+            u.unit0[u.n_ranges] = u.n_units;
+            u.n_units += (r.count + 63) / 64;
+            kmax = std::max(kmax, r.K);
+            ++u.n_ranges;
+        }
+        if (u.n_units == 0) return;
+        if (u.n_units >= (int64_t(1) << 31)) throw std::runtime_error("batch: too many units for the persistent launch");
+        hipLaunchKernelGGL(batch_units_reset_kernel, dim3(1), dim3(64), 0, s, p.work);
+        int dev = 0, cus = 0;
+        TK_HIP_CHECK(hipGetDevice(&dev));
+        TK_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        // a resident grid: 8 workgroups of 128 threads per CU (4 waves per SIMD), a multiple of 8
+        const unsigned blocks = (unsigned)std::max(8, (cus * 8 + 7) / 8 * 8);
+#define TK_PERSIST_LAUNCH(KM)                                                                                     \
+    if (kmax <= KM) {                                                                                             \
+        hipLaunchKernelGGL((knn_batch_persistent_kernel<T, D, KM, kLower>), dim3(blocks), dim3(kTreeBlock), 0, s,  \
+                           ba, u, p.work);                                                                        \
+        TK_HIP_CHECK(hipGetLastError());                                                                          \
+        return;                                                                                                   \
+    }
+        TK_PERSIST_LAUNCH(4)
+        TK_PERSIST_LAUNCH(16)
+        TK_PERSIST_LAUNCH(32)
+#undef TK_PERSIST_LAUNCH
+        throw std::runtime_error("batch: k bucket out of range");
+    }
     for (const BatchPlan::Range& r : p.ranges) {
         if (r.count == 0) continue;
         const dim3 grid((unsigned)((r.count + kTreeBlock - 1) / kTreeBlock));
 #define TK_LAUNCH_K(KK)                                                                                              \
     case KK:                                                                                                         \
-        hipLaunchKernelGGL((knn_batch_kernel<T, D, KK, kLower>), grid, dim3(kTreeBlock), 0, s, arena, p.d_clouds,   \
-                           p.qcloud, q, p.order, r.begin, r.count, dist, idx, orig, pk, pv, qs, p.pair_tag_shift,    \
-                           g, dq, gs);                                                                               \
+        hipLaunchKernelGGL((knn_batch_kernel<T, D, KK, kLower>), grid, dim3(kTreeBlock), 0, s, ba, r.begin, r.count); \
         break;
         switch (r.K) {
             TK_LAUNCH_K(1)

@@ -411,6 +411,13 @@ int tk_batch_capture(tk_batch* batch, const void* queries, void* dist, int32_t* 
     });
 }
 
+int tk_batch_set_persistent(tk_batch* batch, int on) {
+    return guarded([&] {
+        if (!batch || !batch->p) throw std::runtime_error("null batch handle");
+        batch->p->persistent = on != 0;
+    });
+}
+
 int tk_batch_launch(tk_batch* batch, void* stream) {
     return guarded([&] {
         if (!batch || !batch->p) throw std::runtime_error("null batch handle");

@@ -310,6 +310,7 @@ struct BatchPlan {
         }
         check(rc);
     }
+    void set_persistent(bool on) { check(tk_batch_set_persistent(b, on ? 1 : 0)); }
 };
 
 template <typename T, int D, bool kGpu>
@@ -388,7 +389,9 @@ PYBIND11_MODULE(torch_knn, m) {
         .def("capture", &BatchPlan::capture, py::arg("queries_ptr"), py::arg("dist_ptr"), py::arg("idx_ptr"),
              py::arg("orig_idx_ptr") = 0, py::arg("grad_dist_ptr") = 0, py::arg("grad_queries_ptr") = 0,
              py::arg("grad_ref_ptr") = 0, py::arg("stream") = 0)
-        .def("launch", &BatchPlan::launch, py::arg("stream") = 0);
+        .def("launch", &BatchPlan::launch, py::arg("stream") = 0)
+        .def("set_persistent", &BatchPlan::set_persistent, py::arg("on"),
+             "One persistent traversal launch for every k bucket (work queues) instead of one launch per bucket");
 
     m.def(
         "load_tree",
