@@ -754,11 +754,122 @@ __device__ __forceinline__ void store_list(const KBest<T, K, kLower>& kb, int k,
     if constexpr (J0 + N < K) store_list<T, K, kLower, J0 + N>(kb, k, tv, qi, out_d, out_i, out_o);
 }
 
+// ---------------------------------------------------------------------------------------------
+// Row stores through LDS for a full wavefront.  Lane l holds output row qi(l); stored lane by lane,
+// every 16-byte store instruction touches 64 different rows (64 partial lines).  Passing a slab of
+// SW bytes of each row through the wave's part of the leaf-scan staging buffer lets one
+// instruction write 64 consecutive 16-byte chunks of the slab matrix instead: SW/16 lanes per row
+// and 1024/SW whole row slabs per instruction.
+// ---------------------------------------------------------------------------------------------
+
+// The wave's part of a block's staging buffer T[CH * BLK] (lane i of the block owns elements i,
+// i + BLK, i + 2*BLK, ...): CH segments of 64 elements, BLK elements apart.  Chunk c (16 bytes) of
+// the wave's logical buffer lies in one segment, since a segment is a multiple of 16 bytes.
+template <typename T, int BLK>
+struct WaveStage {
+    unsigned char* base;  // the wave's part of segment 0
+
+    __device__ __forceinline__ uint4* chunk(int c) const {
+        constexpr int kSeg = 64 * (int)sizeof(T);
+        const int byte = c * 16;
+        return reinterpret_cast<uint4*>(base + (byte / kSeg) * (BLK * (int)sizeof(T)) + byte % kSeg);
+    }
+};
+
+// Orders the wave's LDS writes before its reads and the reads before the next writes.  A
+// wavefront's LDS instructions execute in issue order, so only the compiler must keep them there.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Writes bytes [col, col + 16*NC) of every lane's row: v holds this lane's, row_bytes is the row
+// stride of `base`, qr[p] the row of the lanes whose chunk pass p stores (row p*64/C + lane/C of
+// the wave, C = SW/16 chunks per lane and slab).
+template <typename T, int BLK, int SW, int NC>
+__device__ __forceinline__ void wave_store_part(const WaveStage<T, BLK>& st, int lane, const int32_t (&qr)[SW / 16],
+                                                unsigned char* __restrict__ base, int64_t row_bytes, int col,
+                                                const uint4 (&v)[NC]) {
+    constexpr int C = SW / 16;
+    static_assert(NC % C == 0, "a row part is a whole number of slabs");
+#pragma unroll
+    for (int s = 0; s < NC / C; ++s) {
+#pragma unroll
+        for (int u = 0; u < C; ++u) *st.chunk(lane * C + u) = v[s * C + u];
+        wave_lds_sync();
+#pragma unroll
+        for (int p = 0; p < C; ++p) {
+            const uint4 w = *st.chunk(p * 64 + lane);
+            *reinterpret_cast<uint4*>(base + qr[p] * row_bytes + col + s * SW + (lane % C) * 16) = w;
+        }
+        wave_lds_sync();
+    }
+}
+
+// Slab width of the wave stores: at most 64 bytes, what the wave's staging part holds per lane
+// (CH elements), and a divisor of every row part (up to 16 entries of 4, 8 or sizeof(T) bytes).
+template <typename T, int K, int CH>
+constexpr int wave_slab_bytes() {
+    const int a = CH * (int)sizeof(T) < 64 ? CH * (int)sizeof(T) : 64;
+    const int n4 = (K < 16 ? K : 16) * 4;
+    return a < n4 ? a : n4;
+}
+
+// store_list for a full wavefront whose rows are k == K entries long and 16-byte aligned: the same
+// values, through wave_store_part.  The distances go out before the index gathers are issued, so
+// the two are not live at once.
+template <typename T, int K, bool kLower, int BLK, int SW, int J0 = 0>
+__device__ __forceinline__ void store_list_wave(const KBest<T, K, kLower>& kb, const TreeView<T>& tv,
+                                                const WaveStage<T, BLK>& st, int lane,
+                                                const int32_t (&qr)[SW / 16], T* __restrict__ out_d,
+                                                int32_t* __restrict__ out_i, int64_t* __restrict__ out_o) {
+    constexpr int N = K - J0 < 16 ? K - J0 : 16;
+    {
+        T tmp[N];
+#pragma unroll
+        for (int u = 0; u < N; ++u) tmp[u] = kb.dist(J0 + u);
+        uint4 v[N * sizeof(T) / 16];
+        __builtin_memcpy(v, tmp, sizeof(tmp));
+        wave_store_part<T, BLK, SW>(st, lane, qr, reinterpret_cast<unsigned char*>(out_d), (int64_t)K * sizeof(T),
+                                    J0 * (int)sizeof(T), v);
+    }
+    int32_t m[N] = {};
+    if (kLower || out_o) {
+        const int32_t* map = kLower ? tv.inv_shuffled : tv.shuffled;
+#pragma unroll
+        for (int u = 0; u < N; ++u) m[u] = load_i32_at(map, kb.key(J0 + u));
+    }
+    {
+        int32_t tmp[N];
+#pragma unroll
+        for (int u = 0; u < N; ++u) tmp[u] = kLower ? m[u] : kb.key(J0 + u);
+        uint4 v[N * 4 / 16];
+        __builtin_memcpy(v, tmp, sizeof(tmp));
+        wave_store_part<T, BLK, SW>(st, lane, qr, reinterpret_cast<unsigned char*>(out_i), (int64_t)K * 4, J0 * 4, v);
+    }
+    if (out_o) {
+        int64_t tmp[N];
+#pragma unroll
+        for (int u = 0; u < N; ++u) tmp[u] = (int64_t)(kLower ? kb.key(J0 + u) : m[u]);
+        uint4 v[N * 8 / 16];
+        __builtin_memcpy(v, tmp, sizeof(tmp));
+        wave_store_part<T, BLK, SW>(st, lane, qr, reinterpret_cast<unsigned char*>(out_o), (int64_t)K * 8, J0 * 8, v);
+    }
+    if constexpr (J0 + N < K)
+        store_list_wave<T, K, kLower, BLK, SW, J0 + N>(kb, tv, st, lane, qr, out_d, out_i, out_o);
+}
+
 template <typename T, int D>
 __device__ __forceinline__ void load_query(const T* __restrict__ queries, int64_t qi, Pt<T, D>& q) {
 #pragma unroll
     for (int i = 0; i < D; ++i) q[i] = queries[qi * D + i];
 }
+
+// Output rows of full wavefronts written through LDS (1) or lane by lane (0).
+#ifndef TK_WAVE_STORES
+#define TK_WAVE_STORES 1
+#endif
 
 // Points per staged chunk of the leaf scan (measured on MI355X), for float searches with d <= 3
 // and lists of 8 or more: TK_STAGE_CHUNK_F32_K16 (16) for the single-tree reference-policy
@@ -793,11 +904,12 @@ __device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T
                                                   const int32_t* __restrict__ order, int k, T* __restrict__ out_d,
                                                   int32_t* __restrict__ out_i, int64_t* __restrict__ out_o) {
     static_assert(CH <= kRecPad, "a chunk may read at most kRecPad records past a leaf's end");
-    __shared__ T stage_d[CH * BLK];
+    static_assert(BLK % 64 == 0, "whole wavefronts per block");
+    __shared__ __align__(16) T stage_d[CH * BLK];
     __shared__ int32_t stage_o[kLower ? CH * BLK : 1];
     const int64_t t = xcd_tile(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
     if (t >= nq) return;
-    const int64_t qi = order ? (int64_t)order[t] : t;
+    const int32_t qi = order ? order[t] : (int32_t)t;  // nq < 2^31
     Pt<T, D> q, pp;
     KBest<T, K, kLower> kb;
     uint32_t far_mask;
@@ -811,6 +923,27 @@ __device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T
         while (!lane_step<T, D, K, kLower, CH, LD, BLK>(tv, q, pp, kb, far_mask, node, level, stage_d + threadIdx.x,
                                                       stage_o + (kLower ? threadIdx.x : 0))) {
         }
+    constexpr int SW = wave_slab_bytes<T, K, CH>();
+    // (float only: the double kernels already run at 2 to 5 waves per SIMD, where the stores' extra
+    // registers cost a wave for some lists)
+    if constexpr (TK_WAVE_STORES && sizeof(T) == 4 && K >= 8 && SW >= 32) {
+        // When the wave's 64 queries all exist, every lane of the wave is here (the search loop has
+        // ended for all of them) and the wave's rows go out through its part of stage_d.  The wave's
+        // first position comes from scalars, so that t need not stay live through the search.
+        const int lane = (int)(threadIdx.x & 63);
+        const int wave_off = __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
+        const bool full = xcd_tile(blockIdx.x, gridDim.x) * BLK + wave_off + 64 <= nq;
+        const bool vec = k == K && ((((uintptr_t)out_d | (uintptr_t)out_i | (uintptr_t)out_o) & 15u) == 0);
+        if (full && vec) {
+            constexpr int C = SW / 16;
+            int32_t qr[C];
+#pragma unroll
+            for (int p = 0; p < C; ++p) qr[p] = __shfl(qi, p * (64 / C) + lane / C, 64);
+            const WaveStage<T, BLK> st{reinterpret_cast<unsigned char*>(stage_d + wave_off)};
+            store_list_wave<T, K, kLower, BLK, SW>(kb, tv, st, lane, qr, out_d, out_i, out_o);
+            return;
+        }
+    }
     store_list<T, K, kLower>(kb, k, tv, qi, out_d, out_i, out_o);
 }
 
