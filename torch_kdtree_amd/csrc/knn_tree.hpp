@@ -907,15 +907,20 @@ __device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T
     static_assert(BLK % 64 == 0, "whole wavefronts per block");
     __shared__ __align__(16) T stage_d[CH * BLK];
     __shared__ int32_t stage_o[kLower ? CH * BLK : 1];
-    const int64_t t = xcd_tile(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
-    if (t >= nq) return;
-    const int32_t qi = order ? order[t] : (int32_t)t;  // nq < 2^31
+    // t = wave0 + lane with wave0 in scalars; the lane index and the row are found again after the
+    // search, so that no per-lane index has to stay live through it
+    const int wave_off = __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
+    const int64_t wave0 = xcd_tile(blockIdx.x, gridDim.x) * BLK + wave_off;
+    if (wave0 + (int)(threadIdx.x & 63u) >= nq) return;
     Pt<T, D> q, pp;
     KBest<T, K, kLower> kb;
     uint32_t far_mask;
     int32_t node;
     int level;
-    load_query<T, D>(queries, qi, q);
+    {
+        const int64_t t = wave0 + (int)(threadIdx.x & 63u);
+        load_query<T, D>(queries, order ? (int64_t)order[t] : t, q);
+    }
     lane_start<T, D, K, kLower>(tv, k, q, pp, kb, far_mask, node, level);
     if (has_nan<T, D>(q))
         nan_fill<T, D, K, kLower>(tv, q, k, kb);
@@ -923,6 +928,12 @@ __device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T
         while (!lane_step<T, D, K, kLower, CH, LD, BLK>(tv, q, pp, kb, far_mask, node, level, stage_d + threadIdx.x,
                                                       stage_o + (kLower ? threadIdx.x : 0))) {
         }
+    // (the lane index from mbcnt in a volatile asm, and the row by an atomic load, so that the
+    // compiler recomputes both here rather than carrying the first values through the search)
+    int lane;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lane));
+    const int32_t qi = order ? __hip_atomic_load(order + wave0 + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT)
+                             : (int32_t)(wave0 + lane);  // nq < 2^31
     constexpr int SW = wave_slab_bytes<T, K, CH>();
     // (float only: the double kernels already run at 2 to 5 waves per SIMD, where the stores' extra
     // registers cost a wave for some lists)
@@ -930,9 +941,7 @@ __device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T
         // When the wave's 64 queries all exist, every lane of the wave is here (the search loop has
         // ended for all of them) and the wave's rows go out through its part of stage_d.  The wave's
         // first position comes from scalars, so that t need not stay live through the search.
-        const int lane = (int)(threadIdx.x & 63);
-        const int wave_off = __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
-        const bool full = xcd_tile(blockIdx.x, gridDim.x) * BLK + wave_off + 64 <= nq;
+        const bool full = wave0 + 64 <= nq;
         const bool vec = k == K && ((((uintptr_t)out_d | (uintptr_t)out_i | (uintptr_t)out_o) & 15u) == 0);
         if (full && vec) {
             constexpr int C = SW / 16;
