@@ -36,8 +36,9 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "knn_tree_plain_gpu_traffic.txt")
 
 
-def _measured_traffic(levels, tie, presort):
-    """HBM bytes per traversal launch from the committed counter summary (key: value lines)."""
+def _measured_traffic(levels, tie, presort, variant):
+    """HBM bytes per traversal launch from the committed counter summary (key: value lines), when it
+    was taken on the same tree depth, tie policy, presort and launch variant."""
     try:
         with open(TRAFFIC_SUMMARY) as f:
             kv = dict(line.split(":", 1) for line in f if ":" in line and not line.startswith("#"))
@@ -45,7 +46,7 @@ def _measured_traffic(levels, tie, presort):
         return None
     kv = {k.strip(): v.strip() for k, v in kv.items()}
     if (kv.get("levels") != str(levels) or kv.get("tie_break") != tie or kv.get("presort") != presort
-            or "traffic_bytes_per_launch" not in kv):
+            or kv.get("variant", "0") != str(variant) or "traffic_bytes_per_launch" not in kv):
         return None
     return int(kv["traffic_bytes_per_launch"])
 
@@ -491,6 +492,14 @@ def main():
 
     ms_per_step = elapsed * 1e3 / args.steps
     value = nq * args.steps / elapsed
+    # the traversal kernel that ran: the built-in launch shape, or a tuned variant of it
+    variant = tree.kdtree.get_variant()
+    if variant > 0:
+        vi = torch_knn.tree_variants()[variant]
+        kernel_name = "knn_tree_kernel_v<float,3,16,%s> (chunk %d, %d waves per SIMD, block %d)" % (
+            "true" if args.tie == "lower_index" else "false", vi["chunk"], vi["waves"], vi["block"])
+    else:
+        kernel_name = "knn_tree_kernel<float,3,16,%s>" % ("true" if args.tie == "lower_index" else "false")
 
     # Algorithmic bytes of one traversal launch: every query read once (d*4 B) and its k results
     # written once (4 B distance + 4 B int32 index), plus the tree read once (N*(d*4+4) record
@@ -595,8 +604,8 @@ def main():
                    "presort_ms_per_step": pre_ms / max(pre_n, 1), "traverse_ms_per_step": trav_avg_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": _measured_traffic(tree.levels, args.tie, args.presort),
-                     "kernel": "knn_tree_kernel<float,3,16,false>",
+                     "traffic": _measured_traffic(tree.levels, args.tie, args.presort, variant),
+                     "kernel": kernel_name, "launch_variant": variant,
                      "algorithmic_bytes_per_launch": launch_bytes,
                      "bytes_per_query": bytes_per_query, "tree_bytes": tree_bytes,
                      "traverse_ms": trav_avg_ms,
