@@ -36,19 +36,19 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "knn_tree_plain_gpu_traffic.txt")
 
 
-def _measured_traffic(levels, tie, presort, variant):
-    """HBM bytes per traversal launch from the committed counter summary (key: value lines), when it
-    was taken on the same tree depth, tie policy, presort and launch variant."""
+def _measured_counters(levels, tie, presort, variant):
+    """Counter figures of one traversal launch from the committed summary (key: value lines), when it
+    was taken on the same tree depth, tie policy, presort and launch variant; else {}."""
     try:
         with open(TRAFFIC_SUMMARY) as f:
             kv = dict(line.split(":", 1) for line in f if ":" in line and not line.startswith("#"))
     except OSError:
-        return None
+        return {}
     kv = {k.strip(): v.strip() for k, v in kv.items()}
     if (kv.get("levels") != str(levels) or kv.get("tie_break") != tie or kv.get("presort") != presort
-            or kv.get("variant", "0") != str(variant) or "traffic_bytes_per_launch" not in kv):
-        return None
-    return int(kv["traffic_bytes_per_launch"])
+            or kv.get("variant", "0") != str(variant)):
+        return {}
+    return kv
 
 
 def _events(torch_knn, stage, fn, reps):
@@ -61,6 +61,21 @@ def _events(torch_knn, stage, fn, reps):
     ms, n = torch_knn.profile_read(stage)
     torch_knn.profile_reset()
     return ms / max(n, 1)
+
+
+def _valu_issue_rate(torch, torch_knn, dev, mode, reps=3):
+    """VALU wave instructions per second of the library's issue probe (tk_valu_probe_gpu: 8
+    independent chains of 32-bit VALU instructions per lane; mode 0 multiply/add, mode 2
+    compare/select/add) over 128 wavefronts per CU."""
+    threads = 256 * 128 * 64
+    out = torch.empty(threads, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    n = torch_knn.valu_probe(out.data_ptr(), threads, 2048, mode, stream)
+    torch.cuda.synchronize()
+    ms = _events(torch_knn, "valu_probe",
+                 lambda: torch_knn.valu_probe(out.data_ptr(), threads, 2048, mode, stream), reps)
+    del out
+    return n / (ms * 1e-3)
 
 
 def _achievable_gbs(torch, torch_knn, dev, mib=1024, reps=5):
@@ -515,6 +530,22 @@ def main():
     achieved = launch_bytes / (trav_avg_ms * 1e-3) / 1e9
     log("measuring the achievable bandwidth (streaming copy, 1 GiB)")
     achievable = _achievable_gbs(torch, torch_knn, dev)
+    log("measuring the VALU issue rate (probe kernel)")
+    valu_peak = _valu_issue_rate(torch, torch_knn, dev, 0)
+    valu_mix = _valu_issue_rate(torch, torch_knn, dev, 2)
+    counters = _measured_counters(tree.levels, args.tie, args.presort, variant)
+    traffic = int(counters["traffic_bytes_per_launch"]) if "traffic_bytes_per_launch" in counters else None
+    # the traversal's own bound: VALU issue.  Its instruction count per launch comes from the committed
+    # SQ_INSTS_VALU summary of the same configuration; the ceiling from the probe, measured here.
+    valu = {"probe_wave_insts_per_s": valu_peak, "probe_cmp_select_wave_insts_per_s": valu_mix,
+            "probe": "tk_valu_probe_gpu, 128 waves per CU: mode 0 = 8 independent v_mul_f32/v_add_f32 chains "
+                     "per lane (4-cycle instructions), mode 2 = v_cmp/v_cndmask/v_add chains"}
+    if "valu_insts_per_launch" in counters:
+        n_valu = float(counters["valu_insts_per_launch"])
+        valu.update({"insts_per_launch": n_valu, "wave_insts_per_s": n_valu / (trav_avg_ms * 1e-3),
+                     "frac_of_probe": n_valu / (trav_avg_ms * 1e-3) / valu_peak,
+                     "frac_of_cmp_select_probe": n_valu / (trav_avg_ms * 1e-3) / valu_mix,
+                     "lanes_active_per_inst": float(counters.get("valu_lanes_active", "nan"))})
     # work of the search: leaves and points scanned per query (host search of a balanced sample, the
     # same pruning decisions as the GPU's), and the distance arithmetic it implies (8 flops per point)
     half = w["halves"][0]
@@ -604,14 +635,14 @@ def main():
                    "presort_ms_per_step": pre_ms / max(pre_n, 1), "traverse_ms_per_step": trav_avg_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": _measured_traffic(tree.levels, args.tie, args.presort, variant),
+                     "traffic": traffic,
                      "kernel": kernel_name, "launch_variant": variant,
                      "algorithmic_bytes_per_launch": launch_bytes,
                      "bytes_per_query": bytes_per_query, "tree_bytes": tree_bytes,
                      "traverse_ms": trav_avg_ms,
                      "achievable_gbs": achievable, "frac_of_achievable": achieved / achievable,
                      "achieved_with_fused_remap": fused_bytes / (trav_avg_ms * 1e-3) / 1e9,
-                     "search_work": work},
+                     "search_work": work, "valu_issue": valu},
         "cpu_baseline": cpu,
         "secondary": extras,
     }

@@ -261,6 +261,14 @@ int tk_profile_read(const char* stage, double* total_ms, int64_t* calls);
  * is the achievable HBM bandwidth that the traversal's roofline is quoted against. */
 int tk_copy_gpu(void* dst, const void* src, size_t bytes, void* stream);
 
+/* A VALU issue probe (no counterpart in the reference): `threads` lanes (a positive multiple of
+ * 256) each run `iters` iterations of 8 independent chains of 32-bit VALU instructions (mode 0: a
+ * multiply and an add, mode 1: two med3, mode 2: a compare, a select and an add per chain) and write
+ * one float to out[lane].  *wave_insts receives the VALU wave
+ * instructions of the loop.  Bracketed as stage "valu_probe" when timing is on: instructions per
+ * second over all CUs is the issue ceiling the traversal's instruction count is quoted against. */
+int tk_valu_probe_gpu(float* out, int64_t threads, int iters, int mode, void* stream, int64_t* wave_insts);
+
 #ifdef __cplusplus
 }
 #endif

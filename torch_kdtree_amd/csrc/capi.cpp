@@ -451,6 +451,14 @@ int tk_copy_gpu(void* dst, const void* src, size_t bytes, void* stream) {
     });
 }
 
+int tk_valu_probe_gpu(float* out, int64_t threads, int iters, int mode, void* stream, int64_t* wave_insts) {
+    return guarded([&] {
+        tk::ProfScope prof("valu_probe", static_cast<hipStream_t>(stream));
+        const int64_t n = tk::valu_probe_gpu(out, threads, iters, mode, static_cast<hipStream_t>(stream));
+        if (wave_insts) *wave_insts = n;
+    });
+}
+
 int tk_profile_enable(int on) {
     return guarded([&] { tk::prof_enable(on != 0); });
 }

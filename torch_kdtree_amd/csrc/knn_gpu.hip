@@ -228,6 +228,55 @@ void stream_copy_gpu(void* dst, const void* src, size_t bytes, hipStream_t strea
 }
 
 // ------------------------------------------------------------------------------------------
+// VALU issue probe: 8 independent chains per lane of 32-bit VALU operations (no FMA under
+// -ffp-contract=off).  Mode 0: a v_mul_f32 and a v_add_f32 per chain, mode 1: two v_med3_f32 per
+// chain (16 VALU instructions per loop iteration); mode 2: a compare, a select and an add per chain
+// (v_cmp_gt_f32, v_cndmask_b32, v_add_f32: 24 per iteration), the mix of the searches' list and
+// pruning code.  Launched over every CU, mode 0's rate is the issue ceiling of 4-cycle
+// instructions (one wave instruction per SIMD every 4 cycles); compares and selects issue faster,
+// which mode 2 shows.  The traversal's VALU instruction count is quoted against both.
+// ------------------------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(256) void valu_probe_kernel(float* __restrict__ out, int iters, float a, float b) {
+    float x[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) x[i] = (float)threadIdx.x * 0.001f + (float)i;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if constexpr (MODE == 0) {
+                x[i] = x[i] * a;
+                x[i] = x[i] + b;
+            } else if constexpr (MODE == 1) {
+                x[i] = __builtin_amdgcn_fmed3f(x[i], a, b);
+                x[i] = __builtin_amdgcn_fmed3f(x[i], b, a);
+            } else {
+                x[i] = x[i] + (x[i] < a ? b : -b);
+            }
+        }
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s += x[i];
+    out[(int64_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+int64_t valu_probe_gpu(float* out, int64_t threads, int iters, int mode, hipStream_t stream) {
+    if (threads <= 0 || threads % 256 != 0 || threads / 256 > INT32_MAX || iters <= 0)
+        throw std::runtime_error("VALU probe: threads must be a positive multiple of 256, iters positive");
+    if (mode < 0 || mode > 2) throw std::runtime_error("VALU probe: mode 0 (mul + add), 1 (med3) or 2 (compare + select)");
+    const dim3 grid((unsigned)(threads / 256));
+    if (mode == 0)
+        hipLaunchKernelGGL(valu_probe_kernel<0>, grid, dim3(256), 0, stream, out, iters, 0.999f, 0.001f);
+    else if (mode == 1)
+        hipLaunchKernelGGL(valu_probe_kernel<1>, grid, dim3(256), 0, stream, out, iters, 0.999f, 0.001f);
+    else
+        hipLaunchKernelGGL(valu_probe_kernel<2>, grid, dim3(256), 0, stream, out, iters, 0.999f, 0.001f);
+    TK_HIP_CHECK(hipGetLastError());
+    return threads / 64 * (int64_t)iters * (mode == 2 ? 24 : 16);
+}
+
+// ------------------------------------------------------------------------------------------
 // Scan / scatter kernels for the presort
 // ------------------------------------------------------------------------------------------
 __global__ void fill_i32_kernel(int32_t* __restrict__ p, int64_t n, int32_t v) {

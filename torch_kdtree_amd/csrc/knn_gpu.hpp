@@ -94,6 +94,9 @@ int canonical_morton_bits(int64_t nq, int dims);
 
 // Streaming device copy (16-byte loads and stores) of `bytes` (a multiple of 16, aligned pointers).
 void stream_copy_gpu(void* dst, const void* src, size_t bytes, hipStream_t stream);
+// VALU issue probe over `threads` lanes (a multiple of 256), writing one float per lane to `out`;
+// returns the wave instructions of its loop (16 per iteration per wavefront, 24 in mode 2).
+int64_t valu_probe_gpu(float* out, int64_t threads, int iters, int mode, hipStream_t stream);
 
 // Brute force (bruteforce.hip).
 void bruteforce_gpu(const void* ref, int32_t n, const void* queries, int64_t nq, int dims, int dtype, int k,

@@ -554,6 +554,22 @@ PYBIND11_MODULE(torch_knn, m) {
         py::arg("dst_ptr"), py::arg("src_ptr"), py::arg("bytes"), py::arg("stream") = 0,
         "Streaming device copy (16-byte loads and stores), timed as stage 'copy'");
     m.def(
+        "valu_probe",
+        [](size_t out_ptr, int64_t threads, int iters, int mode, size_t stream) {
+            int rc;
+            int64_t n = 0;
+            {
+                py::gil_scoped_release release;
+                rc = tk_valu_probe_gpu(reinterpret_cast<float*>(out_ptr), threads, iters, mode,
+                                       reinterpret_cast<void*>(stream), &n);
+            }
+            check(rc);
+            return n;
+        },
+        py::arg("out_ptr"), py::arg("threads"), py::arg("iters"), py::arg("mode") = 0, py::arg("stream") = 0,
+        "VALU issue probe (16 independent 32-bit VALU instructions per iteration and lane), timed as stage "
+        "'valu_probe'; returns the VALU wave instructions of its loop");
+    m.def(
         "profile_read",
         [](const std::string& stage) {
             double ms = 0;
