@@ -63,7 +63,7 @@ void stage_cloud(const Tree& t, const CloudDesc& cd, unsigned char* recs, T* spl
     for (int32_t s = 0; s < t.n_points; ++s) {
         unsigned char* r = recs + (size_t)(cd.rec_base + s) * R::kBytes;
         std::memcpy(r, pts + (size_t)s * D, sizeof(T) * D);
-        std::memcpy(r + sizeof(T) * D, &t.shuffled[s], 4);
+        if constexpr (R::kHasIdx) std::memcpy(r + sizeof(T) * D, &t.shuffled[s], 4);
         shuffled[cd.rec_base + s] = t.shuffled[s];
         inv[cd.rec_base + t.shuffled[s]] = s;
     }

@@ -49,15 +49,20 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t n_tiles) {
 
 // ---------------------------------------------------------------------------------------------
 // Point records in HBM.  A leaf is a contiguous run of records in structured order; each record
-// holds the point's coordinates followed by its original index, padded so that one record is
-// one (or a few) 16-byte loads: fp32 d=3 is exactly one float4 (x, y, z, index bits).
+// holds the point's coordinates, padded so that one record is one (or a few) 16-byte loads, and
+// the point's original index where the coordinates leave room for it in the padding: fp32 d=3 is
+// exactly one float4 (x, y, z, index bits).  Coordinates that fill whole 16-byte words (fp32 d=8,
+// fp64 d=2 and d=8) are stored alone, 32 instead of 48 bytes at fp32 d=8; a search that needs the
+// original index of such a record reads it from the tree's `shuffled` map (rec_orig).
 // ---------------------------------------------------------------------------------------------
 template <typename T, int D>
 struct RecLayout {
-    static constexpr int kRaw = D * (int)sizeof(T) + 4;
+    static constexpr int kCoord = D * (int)sizeof(T);
+    static constexpr bool kHasIdx = kCoord <= 4 || kCoord % 16 != 0;
+    static constexpr int kRaw = kCoord + (kHasIdx ? 4 : 0);
     static constexpr int kBytes = kRaw <= 8 ? 8 : ((kRaw + 15) / 16) * 16;
     static constexpr int kWords = kBytes / 4;
-    static constexpr int kIdxWord = D * (int)sizeof(T) / 4;
+    static constexpr int kIdxWord = kCoord / 4;  // when kHasIdx
 };
 
 template <typename T, int D>
@@ -87,7 +92,10 @@ __device__ __forceinline__ Rec<T, D> load_rec(const unsigned char* __restrict__ 
     }
     Rec<T, D> r;
     __builtin_memcpy(r.c, w, sizeof(T) * D);
-    r.o = (int32_t)w[L::kIdxWord];
+    if constexpr (L::kHasIdx)
+        r.o = (int32_t)w[L::kIdxWord];
+    else
+        r.o = 0;  // not stored: rec_orig reads it from the tree's map
     return r;
 }
 

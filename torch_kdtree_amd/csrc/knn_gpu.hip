@@ -154,10 +154,11 @@ void upload_tree(Tree& t, int device) {
     std::vector<unsigned char> recs((n + kRecPad) * (size_t)d->rec_bytes, 0);
     std::vector<int32_t> inv(n);
     for (size_t s = 0; s < n; ++s) inv[(size_t)t.shuffled[s]] = (int32_t)s;
+    const bool rec_idx = (size_t)d->rec_bytes >= coord_bytes + 4;  // RecLayout::kHasIdx
     for (size_t s = 0; s < n; ++s) {
         unsigned char* r = &recs[s * (size_t)d->rec_bytes];
         std::memcpy(r, &t.points[s * coord_bytes], coord_bytes);
-        std::memcpy(r + coord_bytes, &t.shuffled[s], 4);
+        if (rec_idx) std::memcpy(r + coord_bytes, &t.shuffled[s], 4);
     }
     // boxes: lo then hi, each padded from dims to box_bytes / (2 * ts) elements
     d->box_bytes = box_bytes_for(t.dtype, t.dims);

@@ -259,6 +259,16 @@ struct KBestExact {
     }
 };
 
+// The original index of the record at structured position s: from the record where its layout
+// has room for it, else from the tree's `shuffled` map (indexed, like the records, by position).
+template <typename T, int D, bool kB>
+__device__ __forceinline__ int32_t rec_orig(const TreeView<T, kB>& tv, int32_t s, const Rec<T, D>& r) {
+    if constexpr (RecLayout<T, D>::kHasIdx)
+        return r.o;
+    else
+        return load_i32_at(tv.shuffled, s);
+}
+
 template <typename T, int D, typename List, bool kB>
 __device__ __forceinline__ void scan_leaf(const TreeView<T, kB>& tv, int32_t leaf, const Pt<T, D>& q, List& kb) {
     const int32_t beg = tv.leaf_begin(leaf);
@@ -266,7 +276,8 @@ __device__ __forceinline__ void scan_leaf(const TreeView<T, kB>& tv, int32_t lea
     for (int32_t s = beg; s < end; ++s) {
         const Rec<T, D> r = load_rec<T, D>(tv.recs, s);
         const T dd = sq_dist<T, D>(q, r.c);
-        if (kb.accepts(dd, r.o)) kb.insert(dd, s, r.o);
+        const int32_t o = rec_orig(tv, s, r);
+        if (kb.accepts(dd, o)) kb.insert(dd, s, o);
     }
 }
 
@@ -390,10 +401,11 @@ __device__ __forceinline__ void scan_leaf_staged(const TreeView<T, kB>& tv, int3
                 const int j = g + jj;
                 const Rec<T, D>& r = rg[jj];
                 const T dd = sq_dist<T, D>(q, r.c);
-                const bool acc = kb.accepts_stale(w, dd, kLower ? r.o : 0);
+                const int32_t o = kLower ? rec_orig(tv, c0 + j, r) : 0;
+                const bool acc = kb.accepts_stale(w, dd, o);
                 mask |= acc ? (1u << j) : 0u;
                 st_d[j * SB] = dd;
-                if constexpr (kLower) st_o[j * SB] = r.o;
+                if constexpr (kLower) st_o[j * SB] = o;
             }
         }
         mask &= n >= 32 ? ~0u : ((1u << n) - 1u);  // only the leaf's points
