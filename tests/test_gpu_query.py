@@ -154,6 +154,38 @@ def test_exact_ties_on_integer_grid(cuda, policy):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("d", [1, 2, 3, 8])
+def test_record_group_boxes_big_leaves(cuda, dtype, d):
+    """Trees whose leaves hold 64 or more points on average carry a box per run of 16 records of a
+    leaf, and a leaf scan skips the runs whose box lies at least the worst distance away.  Random
+    points with queries on and off them, and a tie-heavy integer lattice with duplicate points, at
+    3,000 and 375 points a leaf: every result equals the oracle's (reference policy) and brute
+    force's (lower-index policy)."""
+    rng = np.random.default_rng(40 + d)
+    rnd_ref, rnd_q = _data(rng, 12000, 1500, d, dtype)
+    lat_ref = rng.integers(0, 5, (12000, d)).astype(dtype)
+    lat_q = (rng.integers(0, 5, (1500, d)) + 0.5 * rng.integers(0, 2, (1500, d))).astype(dtype)
+    cases = (("random", rnd_ref, np.ascontiguousarray(np.concatenate([rnd_q, rnd_ref[:300]]))),
+             ("lattice", lat_ref, lat_q))
+    for name, ref, qry in cases:
+        for levels in (2, 5):
+            for policy in ("reference", "lower_index"):
+                tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels=levels, tie_break=policy)
+                ot = oracle_for(tree) if policy == "reference" else None
+                for k in (1, 5, 16, 32, 40):
+                    what = "%s levels %d %s k %d" % (name, levels, policy, k)
+                    dist, sidx, oidx = gpu_query(tree, qry, k, cuda)
+                    if policy == "reference":
+                        od, oi = ot.search(qry, k)
+                        assert_same(dist, od, what + " distances")
+                        assert_same(sidx, oi, what + " structured indices")
+                    else:
+                        bd, bi = oracle.bruteforce(ref, qry, k)
+                        assert_same(dist, bd, what + " distances")
+                        assert_same(oidx, bi.astype(np.int64), what + " original indices")
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [1, 2, 3, 8])
 def test_presort_does_not_change_results(cuda, dtype, d):
     rng = np.random.default_rng(8 + d)
     ref, qry = _data(rng, 50000, 20000, d, dtype)

@@ -19,6 +19,12 @@ struct DeviceTree {
     int32_t* shuffled = nullptr;      // [n_points] original index of each structured position
     int32_t* inv_shuffled = nullptr;  // [n_points] structured position of each original index
     unsigned char* boxes = nullptr;   // [n_nodes + n_leaves] point boxes, BoxLayout<T, dims>::kBytes each
+    // point boxes of the record groups: each leaf's records in runs of kGroupRecs (knn_tree.hpp)
+    // from its first (the last run may be shorter); group_base[leaf] = index of the leaf's first
+    // group box.  Null for trees holding a non-finite value (their searches use no boxes) and for
+    // trees whose leaves hold fewer than kGroupMinLeaf points on average.
+    unsigned char* group_boxes = nullptr;
+    int32_t* group_base = nullptr;    // [n_leaves + 1]
     int rec_bytes = 0;
     int box_bytes = 0;
 };

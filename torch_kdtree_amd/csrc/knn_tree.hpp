@@ -35,6 +35,10 @@ struct TreeView {
     int32_t n_nodes;
     // batched only: where this tree's nodes, boxes, leaf offsets and records start in the arena
     int32_t node_base = 0, box_base = 0, leaf_base = 0, rec_base = 0;
+    // point boxes of each leaf's record groups (kGroupRecs records from the leaf's first), and the
+    // index of each leaf's first group box; null: no group tests
+    const unsigned char* __restrict__ group_boxes = nullptr;
+    const int32_t* __restrict__ group_base = nullptr;
     // Every split value separates its node's points (true when all coordinates and split values
     // are finite).  A NaN coordinate leaves std::sort's order unspecified and an overflowing median
     // is +-inf, so the lower-index policy then prunes with point boxes only (they hold the actual
@@ -58,6 +62,11 @@ struct TreeView {
 __device__ __forceinline__ int32_t load_i32_at(const int32_t* __restrict__ base, int32_t i) {
     return *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(base) + ((uint32_t)i << 2));
 }
+
+// Records per group box (the host writes them in runs of the same length), and the mean leaf
+// size from which a tree gets group boxes.
+constexpr int kGroupRecs = 16;
+constexpr int kGroupMinLeaf = 4 * kGroupRecs;
 
 // Threads per block of the traversal kernel; per-lane LDS staging slots are strided by it.
 #ifndef TK_TREE_BLOCK
@@ -375,14 +384,27 @@ constexpr int kRecPad = 16;
 // by the stale worst is rejected by every later (smaller or equal) worst as well.
 // In pass 2 a candidate that the current list rejects is inserted as +inf, which leaves the list
 // unchanged: the insertion is then unconditional and updates the list in place.
-template <typename T, int D, int K, bool kLower, int CH, int LD, int SB = kTreeBlockStride, bool kB>
+template <typename T, int D, int K, bool kLower, int CH, int LD, int SB = kTreeBlockStride, bool kGroups = false,
+          bool kB>
 __device__ __forceinline__ void scan_leaf_staged(const TreeView<T, kB>& tv, int32_t leaf, const Pt<T, D>& q,
                                                  KBest<T, K, kLower>& kb, T* __restrict__ st_d,
                                                  int32_t* __restrict__ st_o) {
     static_assert(CH % LD == 0 && CH <= 32, "a chunk is a whole number of load groups, at most 32 points");
+    static_assert(!kGroups || kGroupRecs % CH == 0, "chunks nest in record groups");
     const int32_t beg = tv.leaf_begin(leaf);
     const int32_t end = tv.leaf_end(leaf);
+    const int32_t gb = kGroups ? tv.group_base[leaf] : 0;
     for (int32_t c0 = beg; c0 < end; c0 += CH) {
+        // With kGroups (the tree has group boxes): a group of records whose box lies at least the
+        // worst distance away holds no point the list would take (every distance is at least the box
+        // distance, in rounded arithmetic too), so skipping it changes nothing but the work.
+        if (kGroups && (c0 - beg) % kGroupRecs == 0) {
+            const T b = aabb_bound<T, D>(tv.group_boxes, gb + (c0 - beg) / kGroupRecs, q);
+            if (kLower ? !lower_passes(b, kb.worst()) : !(b < kb.worst())) {
+                c0 += kGroupRecs - CH;
+                continue;
+            }
+        }
         const int n = min(CH, end - c0);
         const auto w = kb.stale();
         uint32_t mask = 0;
@@ -524,7 +546,8 @@ __device__ __forceinline__ void lane_start(const TreeView<T, kB>& tv, int k, con
 
 // Scans the leaf pair of the current leaf-parent, then climbs to the next far side to enter and
 // walks down to its leaf-parent.  Returns true when the search is complete.
-template <typename T, int D, int K, bool kLower, int CH, int LD, int SB = kTreeBlockStride, bool kB>
+template <typename T, int D, int K, bool kLower, int CH, int LD, int SB = kTreeBlockStride, bool kGroups = false,
+          bool kB>
 __device__ __forceinline__ bool lane_step(const TreeView<T, kB>& tv, const Pt<T, D>& q, Pt<T, D>& pp,
                                           KBest<T, K, kLower>& kb, uint32_t& far_mask, int32_t& node, int& level,
                                           T* __restrict__ st_d, int32_t* __restrict__ st_o) {
@@ -544,7 +567,7 @@ __device__ __forceinline__ bool lane_step(const TreeView<T, kB>& tv, const Pt<T,
             const bool looks = side == 0 || (kLower ? lower_split_passes<T, D>(tv, q, pp, a_lp, m, kb.worst())
                                                     : proj_bound<T, D>(q, pp, a_lp, m) < kb.worst());
             if (looks && leaf_useful<T, D, kLower>(tv, q, leaf, kb.worst()))
-                scan_leaf_staged<T, D, K, kLower, CH, LD, SB>(tv, leaf, q, kb, st_d, st_o);
+                scan_leaf_staged<T, D, K, kLower, CH, LD, SB, kGroups>(tv, leaf, q, kb, st_d, st_o);
         }
         if (level == 0) return true;
         ax_set(pp, a_lp, qa);
@@ -911,7 +934,7 @@ struct StageBytes {
 // keeps the wavefront's control flow and loads coherent.  (Handing a finished lane the next query
 // was measured to be slower: it breaks that lockstep.)  BLK threads per block, leaf chunks of CH
 // records with LD loads in flight.
-template <typename T, int D, int K, bool kLower, int CH, int LD, int BLK>
+template <typename T, int D, int K, bool kLower, int CH, int LD, int BLK, bool kGroups = false>
 __device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T* __restrict__ queries, int64_t nq,
                                                   const int32_t* __restrict__ order, int k, T* __restrict__ out_d,
                                                   int32_t* __restrict__ out_i, int64_t* __restrict__ out_o) {
@@ -937,7 +960,7 @@ __device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T
     if (has_nan<T, D>(q))
         nan_fill<T, D, K, kLower>(tv, q, k, kb);
     else
-        while (!lane_step<T, D, K, kLower, CH, LD, BLK>(tv, q, pp, kb, far_mask, node, level, stage_d + threadIdx.x,
+        while (!lane_step<T, D, K, kLower, CH, LD, BLK, kGroups>(tv, q, pp, kb, far_mask, node, level, stage_d + threadIdx.x,
                                                       stage_o + (kLower ? threadIdx.x : 0))) {
         }
     // (the lane index from mbcnt in a volatile asm, and the row by an atomic load, so that the
@@ -968,13 +991,14 @@ __device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T
     store_list<T, K, kLower>(kb, k, tv, qi, out_d, out_i, out_o);
 }
 
-template <typename T, int D, int K, bool kLower>
+// kGroups: the tree has record-group boxes (tv.group_boxes), and leaf scans test them.
+template <typename T, int D, int K, bool kLower, bool kGroups = false>
 __global__ __launch_bounds__(kTreeBlock) TK_TREE_OCCUPANCY(T, D, K, kLower) void knn_tree_kernel(TreeView<T> tv, const T* __restrict__ queries,
                                                                int64_t nq, const int32_t* __restrict__ order, int k,
                                                                T* __restrict__ out_d, int32_t* __restrict__ out_i,
                                                                int64_t* __restrict__ out_o) {
     tree_search_lanes<T, D, K, kLower, StageBytes<T, D, K, kLower>::kChunk, StageBytes<T, D, K, kLower>::kLoads,
-                      kTreeBlock>(tv, queries, nq, order, k, out_d, out_i, out_o);
+                      kTreeBlock, kGroups>(tv, queries, nq, order, k, out_d, out_i, out_o);
 }
 
 // The same search with its launch shape as template parameters: the prebuilt variants an autotuner

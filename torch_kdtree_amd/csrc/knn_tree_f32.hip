@@ -9,8 +9,9 @@ void launch_tree_search_f32(const Tree& t, const float* q, int64_t nq, const int
                             int32_t* idx, int64_t* orig, int tie, hipStream_t s) {
     int kb = 1;
     while (kb < k) kb <<= 1;
-    // a tuned launch variant, where one is prebuilt (finite trees: the others run the exact kernel)
-    if (t.variant > 0 && t.finite &&
+    // a tuned launch variant, where one is prebuilt (finite trees without record-group boxes: the
+    // others run the exact kernel or the group-testing one)
+    if (t.variant > 0 && t.finite && !t.dev->group_boxes &&
         launch_tree_variant_f32_d3(t, q, nq, order, k, kb, dist, idx, orig, tie == kTieLowerIndex, t.variant, s))
         return;
     launch_tree_search_t<float>(t, q, nq, order, k, dist, idx, orig, tie, s);

@@ -12,6 +12,8 @@ TreeView<T> tree_view(const Tree& t) {
     TreeView<T> v{static_cast<const T*>(t.dev->split), t.dev->recs, t.dev->leaf_off, t.dev->shuffled,
                   t.dev->inv_shuffled, t.dev->boxes, t.levels, t.n_nodes};
     v.split_bounds = t.finite;
+    v.group_boxes = t.dev->group_boxes;
+    v.group_base = t.dev->group_base;
     return v;
 }
 
@@ -23,10 +25,14 @@ void launch_tree_td(const TreeView<T>& tv, const T* q, int64_t nq, const int32_t
     int kb = 1;
     while (kb < k) kb <<= 1;
     if (kb > TK_REG_LIST_MAX_K) kb = 0;  // the LDS-list kernel
-#define TK_LAUNCH_K(KK)                                                                                        \
-    case KK:                                                                                                   \
-        hipLaunchKernelGGL((knn_tree_kernel<T, D, KK, kLower>), grid, block, 0, s, tv, q, nq, order, k, dist, \
-                           idx, orig);                                                                         \
+#define TK_LAUNCH_K(KK)                                                                                         \
+    case KK:                                                                                                    \
+        if (tv.group_boxes)                                                                                     \
+            hipLaunchKernelGGL((knn_tree_kernel<T, D, KK, kLower, true>), grid, block, 0, s, tv, q, nq, order, \
+                               k, dist, idx, orig);                                                             \
+        else                                                                                                    \
+            hipLaunchKernelGGL((knn_tree_kernel<T, D, KK, kLower, false>), grid, block, 0, s, tv, q, nq, order, \
+                               k, dist, idx, orig);                                                             \
         break;
     switch (kb) {
         TK_LAUNCH_K(1)
