@@ -446,6 +446,12 @@ def test_autotune_cache_lookup(tmp_path, monkeypatch):
     assert autotune.variant_for(3, torch.float32, 9) == 5 and autotune.variant_for(3, torch.float32, 16) == 5
     assert autotune.variant_for(3, torch.float32, 5) == 2 and autotune.variant_for(3, torch.float32, 4) == 0
     assert autotune.variant_for(3, torch.float64, 16) == 0
+    # launch variants are kept per tie policy: the reference policy's do not apply to the other
+    assert autotune.variant_for(3, torch.float32, 16, "lower_index") == 0
+    cache.write_text(json.dumps({"d3_float32": {"leaf_size": 24, "variants": {"16": 5},
+                                                "variants_lower_index": {"16": 1}}}))
+    assert autotune.variant_for(3, torch.float32, 16, "lower_index") == 1
+    assert autotune.variant_for(3, torch.float32, 16) == 5
     assert autotune.pick_levels(4_000_000, 3, "cpu") == 13
     assert autotune.pick_levels(4_000_000, 3, "cuda") == autotune._levels_for_leaf(4_000_000, 24)
     from torch_kdtree_amd import torch_knn

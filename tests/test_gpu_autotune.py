@@ -44,6 +44,29 @@ def test_tune_writes_and_reads_back(cuda, tmp_path, monkeypatch):
     assert_same(sidx, oi, "structured indices with the tuned variant")
 
 
+def test_tune_lower_index_policy_kept_apart(cuda, tmp_path, monkeypatch):
+    """Tuning under the lower-index policy stores its launch variant under its own key and leaves the
+    reference policy's variants and the leaf size alone; queries of each policy read their own."""
+    cache = tmp_path / "tuning.json"
+    monkeypatch.setenv("TORCH_KDTREE_AMD_TUNING", str(cache))
+    cache.write_text(json.dumps({"d3_float32": {"leaf_size": 32, "variants": {"8": 2}}}))
+    rng = np.random.default_rng(22)
+    ref = torch.from_numpy(rng.random((100_000, 3)).astype(np.float32))
+    qry = torch.from_numpy(rng.random((50_000, 3)).astype(np.float32)).to(cuda)
+    out = autotune.tune(ref, qry, 8, leaf_sizes=(32,), repeats=1, tie_break="lower_index")
+    entry = json.loads(cache.read_text())["d3_float32"]
+    assert entry["leaf_size"] == 32 and entry["variants"] == {"8": 2}
+    assert entry["variants_lower_index"]["8"] == out["variant"]
+    assert autotune.variant_for(3, torch.float32, 8, "lower_index") == out["variant"]
+    assert autotune.variant_for(3, torch.float32, 8) == 2
+    low = tk.build_kd_tree(ref, device=cuda, levels="auto", tie_break="lower_index")
+    gpu_query(low, qry.cpu().numpy(), 8, cuda)
+    assert low.kdtree.get_variant() == out["variant"]
+    refp = tk.build_kd_tree(ref, device=cuda, levels="auto")
+    gpu_query(refp, qry.cpu().numpy(), 8, cuda)
+    assert refp.kdtree.get_variant() == 2
+
+
 @pytest.mark.parametrize("tie", ["reference", "lower_index"])
 @pytest.mark.parametrize("k", [6, 16])
 def test_every_variant_gives_the_same_results(cuda, tie, k):

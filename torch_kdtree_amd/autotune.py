@@ -101,12 +101,19 @@ def pick_levels(n: int, dims: int, device=None, dtype=torch.float32) -> int:
     return _levels_for_leaf(n, target_leaf_size(dims, dtype))
 
 
-def variant_for(dims: int, dtype, k: int) -> int:
-    """The cached launch variant for GPU queries of this dims, dtype and k (0 without an entry)."""
+def _variants_key(tie_break: str) -> str:
+    # launch variants are measured per tie policy: their kernels differ (the lower-index list holds
+    # 64-bit keys), and so does the fastest launch shape
+    return "variants" if tie_break == "reference" else "variants_" + tie_break
+
+
+def variant_for(dims: int, dtype, k: int, tie_break: str = "reference") -> int:
+    """The cached launch variant for GPU queries of this dims, dtype, k and tie policy (0 without an
+    entry)."""
     entry = _load_cache().get(_key(dims, dtype))
     if not entry:
         return 0
-    return int(entry.get("variants", {}).get(str(_list_width(k)), 0))
+    return int(entry.get(_variants_key(tie_break), {}).get(str(_list_width(k)), 0))
 
 
 def _time_query(tree, queries, k, repeats):
@@ -122,13 +129,14 @@ def _time_query(tree, queries, k, repeats):
 
 
 def tune(points: torch.Tensor, queries: torch.Tensor, k: int, leaf_sizes=(8, 16, 24, 32, 48),
-         variants=None, repeats: int = 3, write: bool = True) -> dict:
+         variants=None, repeats: int = 3, write: bool = True, tie_break: str = "reference") -> dict:
     """Measures GPU queries of ``queries`` against ``points`` (k neighbours) for each leaf size with the
     built-in launch shape, then each launch variant on the fastest depth, and keeps the fastest of
     each.  Every variant's distances and indices must equal variant 0's on the same tree, and every
     depth's distances must equal the first depth's (the neighbour sets are exact; only tie order and
     structured indices depend on the tree); a mismatch raises.  With ``write`` the choice is stored
-    for this dims/dtype (leaf size) and list width (variant)."""
+    for this dims/dtype (leaf size, measured under the reference tie policy only) and, for the
+    tie policy ``tie_break``, list width (variant)."""
     from .nn_distance import build_kd_tree
     from . import torch_knn
 
@@ -141,7 +149,7 @@ def tune(points: torch.Tensor, queries: torch.Tensor, k: int, leaf_sizes=(8, 16,
     leaf_res, ref_dist = {}, None
     for leaf in leaf_sizes:
         lv = _levels_for_leaf(n, leaf)
-        tree = build_kd_tree(points, device=dev, levels=lv, variant=0)
+        tree = build_kd_tree(points, device=dev, levels=lv, variant=0, tie_break=tie_break)
         d, _ = tree.query(queries, nr_nns_searches=k)
         if ref_dist is None:
             ref_dist = d.clone()
@@ -149,7 +157,7 @@ def tune(points: torch.Tensor, queries: torch.Tensor, k: int, leaf_sizes=(8, 16,
             raise RuntimeError("tune: leaf size %s changed the distances" % leaf)
         leaf_res[leaf] = {"levels": lv, "seconds": _time_query(tree, queries, k, repeats)}
     leaf_best = min(leaf_res, key=lambda s: leaf_res[s]["seconds"])
-    tree = build_kd_tree(points, device=dev, levels=leaf_res[leaf_best]["levels"], variant=0)
+    tree = build_kd_tree(points, device=dev, levels=leaf_res[leaf_best]["levels"], variant=0, tie_break=tie_break)
     table = torch_knn.tree_variants()
     var_res, ref_out = {}, None
     for v in variants:
@@ -165,11 +173,13 @@ def tune(points: torch.Tensor, queries: torch.Tensor, k: int, leaf_sizes=(8, 16,
     var_best = min(var_res, key=lambda v: var_res[v]["seconds"])
     kb = _list_width(k)
     entry = dict(_load_cache().get(_key(dims, points.dtype), {}))
-    entry.update(leaf_size=leaf_best, n=int(n), queries=int(queries.shape[0]),
-                 measured_leaf={str(s): r for s, r in leaf_res.items()})
-    entry.setdefault("variants", {})[str(kb)] = var_best
-    entry.setdefault("measured_variants", {})[str(kb)] = {str(v): r for v, r in var_res.items()}
+    if tie_break == "reference":
+        entry.update(leaf_size=leaf_best, n=int(n), queries=int(queries.shape[0]),
+                     measured_leaf={str(s): r for s, r in leaf_res.items()})
+    vk = _variants_key(tie_break)
+    entry.setdefault(vk, {})[str(kb)] = var_best
+    entry.setdefault("measured_" + vk, {})[str(kb)] = {str(v): r for v, r in var_res.items()}
     if write:
         _store(_key(dims, points.dtype), entry)
-    return {"leaf_size": leaf_best, "variant": var_best, "list_width": kb, "k": k,
+    return {"leaf_size": leaf_best, "variant": var_best, "list_width": kb, "k": k, "tie_break": tie_break,
             "measured_leaf": leaf_res, "measured_variants": var_res}

@@ -212,7 +212,7 @@ class TorchKDTree:
         inds = torch.empty(size=[nq, nr_nns_searches], dtype=torch.int64, device=self.device)
         if self.use_gpu:
             v = self.variant if self.variant is not None else autotune.variant_for(self.dims, self.dtype,
-                                                                                   nr_nns_searches)
+                                                                                   nr_nns_searches, self.tie_break)
             if v != self._applied_variant:
                 self.kdtree.set_variant(int(v))
                 self._applied_variant = v
