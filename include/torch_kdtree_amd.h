@@ -199,6 +199,18 @@ int tk_knn_backward_gpu(const void* queries, const void* ref, int64_t n_queries,
                         void* workspace, size_t workspace_bytes, void* stream);
 int tk_knn_backward_gpu_workspace(int64_t n_queries, int32_t n_ref, int dims, int dtype, int k, size_t* bytes);
 
+/* The same backward for the neighbours of a GPU query on `tree` given by their structured indices
+ * sidx[nq*k] (int32, the query's idx output) instead of original ones; ref[n_ref*dims] holds the
+ * current coordinates of the tree's points in original order (n_ref = the tree's point count,
+ * dims and dtype the tree's).  Same outputs, bit for bit, as tk_knn_backward_gpu with the matching
+ * original indices: the points are first gathered into the tree's structured order, so the
+ * gathers of spatially close queries read close rows.  Replaces the same recomputation
+ * (nn_distance.py:111-112); always takes a workspace (NULL: allocated stream-ordered). */
+int tk_knn_backward_tree_gpu(const tk_tree* tree, const void* queries, const void* ref, int64_t n_queries, int k,
+                             const int32_t* sidx, const void* grad_dist, void* grad_queries, void* grad_ref,
+                             void* workspace, size_t workspace_bytes, void* stream);
+int tk_knn_backward_tree_gpu_workspace(const tk_tree* tree, int64_t n_queries, int k, size_t* bytes);
+
 /* Batched many-cloud mode (no counterpart in the reference, which queries one tree per call;
  * it replaces a loop of KDTree::query calls, src/interface.cpp:60-86, over many clouds).
  * A plan packs n_trees trees (same dims and dtype; built, not necessarily uploaded) with their

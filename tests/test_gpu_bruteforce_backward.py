@@ -220,3 +220,43 @@ def test_grad_matches_reference_formula_and_is_deterministic(cuda):
         err = (got.double() - want).abs().max().item()
         scale = want.abs().max().item()
         assert err <= 1e-5 * scale, (err, scale)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("d", [1, 2, 3, 8])
+@pytest.mark.parametrize("k", [1, 5, 16, 40])
+def test_tree_backward_equals_original_index_backward(cuda, dtype, d, k):
+    """tk_knn_backward_tree_gpu (neighbours by structured index, points gathered into the tree's
+    structured order) gives tk_knn_backward_gpu's dL/dq and dL/dr bit for bit, with and without
+    dL/dr, and the autograd path of a GPU query (which takes the tree form) gives the same."""
+    gen = torch.Generator(device=cuda).manual_seed(50 + d + k)
+    n, nq = 20000, 7000
+    ref = torch.randn((n, d), device=cuda, generator=gen, dtype=dtype) * 10
+    q = torch.randn((nq, d), device=cuda, generator=gen, dtype=dtype) * 10
+    g = torch.rand((nq, k), device=cuda, generator=gen, dtype=dtype)
+    tree = tk.build_kd_tree(ref, device=cuda, levels="auto")
+    sidx = torch.empty((nq, k), dtype=torch.int32, device=cuda)
+    _, idx = tree.query(q, nr_nns_searches=k, result_idx=sidx)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(cuda).cuda_stream
+    name = str(dtype).replace("torch.", "")
+    full = None
+    for with_r in (True, False):
+        dq0, dr0 = torch.empty_like(q), torch.empty_like(ref)
+        torch_knn.knn_backward(q.data_ptr(), ref.data_ptr(), nq, n, d, name, k, idx.data_ptr(), g.data_ptr(),
+                               dq0.data_ptr(), dr0.data_ptr() if with_r else 0, stream)
+        dq1, dr1 = torch.empty_like(q), torch.empty_like(ref)
+        tree.kdtree.backward(q.data_ptr(), ref.data_ptr(), nq, k, sidx.data_ptr(), g.data_ptr(), dq1.data_ptr(),
+                             dr1.data_ptr() if with_r else 0, stream)
+        torch.cuda.synchronize()
+        assert torch.equal(dq0, dq1), "dL/dq"
+        if with_r:
+            assert torch.equal(dr0, dr1), "dL/dr"
+            full = (dq0, dr0)
+    qq = q.clone().requires_grad_(True)
+    rr = ref.clone().requires_grad_(True)
+    t2 = tk.build_kd_tree(rr, device=cuda, levels="auto")
+    dist, _ = t2.query(qq, nr_nns_searches=k)
+    (g * dist).sum().backward()
+    assert torch.equal(qq.grad, full[0]) and torch.equal(rr.grad, full[1]), "autograd path"
+

@@ -48,9 +48,9 @@ constexpr int kBwUnroll = TK_BW_UNROLL;  // loads issued ahead per lane
 // slice of nq positions), so that neighbouring lanes gather neighbouring reference rows, and also
 // copies the query to q_sorted[t] and, per neighbour j, the upstream gradient and the pair
 // (reference index, slot) to the slice's slot t * k + j, written contiguously across lanes.
-template <typename T, int D>
+template <typename T, int D, typename IdxT>
 __global__ __launch_bounds__(256) void grad_q_kernel(const T* __restrict__ q, const T* __restrict__ ref, int64_t nq,
-                                                     int k, const int64_t* __restrict__ idx,
+                                                     int k, const IdxT* __restrict__ idx,
                                                      const T* __restrict__ g, const uint32_t* __restrict__ order,
                                                      int64_t t0, T* __restrict__ dq, T* __restrict__ q_sorted,
                                                      T* __restrict__ g_sorted, uint32_t* __restrict__ keys,
@@ -136,22 +136,26 @@ __global__ __launch_bounds__(256) void run_bounds_kernel(const uint32_t* __restr
 }
 
 // dL/dr, one lane per reference point: its run of sorted pairs, subtracted in canonical order,
-// from zero (the first slice) or from the sums in dr (later slices).
+// from zero (the first slice) or from the sums in dr (later slices).  With out_row the points are
+// in structured order (lane s takes the point of tree position s) and the lane's sums go to row
+// out_row[s] of dr; else lane m takes row m.
 template <typename T, int D>
 __global__ __launch_bounds__(256) void grad_r_kernel(const T* __restrict__ ref, int32_t n, int k,
                                                      const uint32_t* __restrict__ run_start,
                                                      const uint32_t* __restrict__ run_end,
                                                      const uint32_t* __restrict__ vals,
                                                      const T* __restrict__ q_sorted, const T* __restrict__ g_sorted,
-                                                     bool accumulate, T* __restrict__ dr) {
+                                                     bool accumulate, const int32_t* __restrict__ out_row,
+                                                     T* __restrict__ dr) {
     const int64_t mi = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (mi >= n) return;
+    const int64_t row = out_row ? (int64_t)out_row[mi] : mi;
     const uint32_t e0 = run_start[mi], e1 = run_end[mi];
     T r[D], acc[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         r[a] = ref[mi * D + a];
-        acc[a] = accumulate ? dr[mi * D + a] : T(0);
+        acc[a] = accumulate ? dr[row * D + a] : T(0);
     }
     // loads kBwUnroll pairs ahead; the terms are still subtracted in run order
     for (uint32_t e0u = e0; e0u < e1; e0u += kBwUnroll) {
@@ -178,7 +182,19 @@ __global__ __launch_bounds__(256) void grad_r_kernel(const T* __restrict__ ref, 
         }
     }
 #pragma unroll
-    for (int a = 0; a < D; ++a) dr[mi * D + a] = acc[a];
+    for (int a = 0; a < D; ++a) dr[row * D + a] = acc[a];
+}
+
+// out[s] = in[row[s]] for rows of D coordinates: the reference points in the tree's structured
+// order, so that the gathers of neighbouring queries' neighbours read neighbouring rows.
+template <typename T, int D>
+__global__ __launch_bounds__(256) void gather_rows_kernel(const T* __restrict__ in, const int32_t* __restrict__ row,
+                                                          int32_t n, T* __restrict__ out) {
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= n) return;
+    const int64_t r = row[s];
+#pragma unroll
+    for (int a = 0; a < D; ++a) out[s * D + a] = in[r * D + a];
 }
 
 inline size_t a256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -196,11 +212,12 @@ int64_t slice_queries(int64_t nq, int k) { return std::min<int64_t>(nq, kMaxPair
 
 struct BwLayout {
     size_t qkey, qrow, qkey_s, order, run_start, run_end, bbox, q_sorted, g_sorted, keys, vals, keys_s, vals_s,
-        temp, temp_bytes, total;
+        ref_s, temp, temp_bytes, total;
 };
 
-// Workspace of dL/dr: the canonical order of all nq queries, and the buffers of one slice.
-BwLayout bw_layout(int64_t nq, int32_t n, int k, int dims, int dtype) {
+// Workspace of dL/dr: the canonical order of all nq queries, and the buffers of one slice; with
+// `structured` also the reference points in structured order.
+BwLayout bw_layout(int64_t nq, int32_t n, int k, int dims, int dtype, bool structured = false) {
     const int64_t sq = slice_queries(nq, k);
     const int64_t m = sq * k;
     const size_t ts = dtype_size(dtype);
@@ -219,6 +236,7 @@ BwLayout bw_layout(int64_t nq, int32_t n, int k, int dims, int dtype) {
     l.vals = off; off += a256(4 * (size_t)m);
     l.keys_s = off; off += a256(4 * (size_t)m);
     l.vals_s = off; off += a256(4 * (size_t)m);
+    l.ref_s = off; off += structured ? a256(ts * (size_t)dims * (size_t)n) : 0;
     l.temp = off;
     l.temp_bytes = std::max(radix_sort_workspace(m), radix_sort_workspace(nq));
     off += a256(l.temp_bytes);
@@ -226,9 +244,16 @@ BwLayout bw_layout(int64_t nq, int32_t n, int k, int dims, int dtype) {
     return l;
 }
 
-template <typename T, int D>
-void backward_typed(const T* q, const T* ref, int64_t nq, int32_t n, int k, const int64_t* idx, const T* g, T* dq,
-                    T* dr, hipStream_t s, unsigned char* ws, const BwLayout& l) {
+// idx: the neighbours' rows of `ref` (original indices, int64), or with `shuffled` their
+// structured positions (int32), shuffled[s] being the original row of structured position s.  In
+// the structured form the points are first gathered into structured order, every pair is keyed by
+// its structured position and dL/dr is summed per structured position and written to its original
+// row: the same terms in the same order, so the same bits, with the gathers of spatially close
+// queries reading close rows.
+template <typename T, int D, typename IdxT>
+void backward_typed(const T* q, const T* ref, int64_t nq, int32_t n, int k, const IdxT* idx,
+                    const int32_t* shuffled, const T* g, T* dq, T* dr, hipStream_t s, unsigned char* ws,
+                    const BwLayout& l) {
     const int64_t m = nq * k;
     const bool pairs = dr && m > 0;
     const dim3 qgrid((unsigned)((nq + 255) / 256));
@@ -238,9 +263,17 @@ void backward_typed(const T* q, const T* ref, int64_t nq, int32_t n, int k, cons
         TK_HIP_CHECK(hipGetLastError());
     }
     if (nq == 0 || !(dq || pairs)) return;
+    const T* rows = ref;
+    if (shuffled) {
+        T* ref_s = reinterpret_cast<T*>(ws + l.ref_s);
+        hipLaunchKernelGGL((gather_rows_kernel<T, D>), dim3((unsigned)(((int64_t)n + 255) / 256)), dim3(256), 0, s, ref,
+                           shuffled, n, ref_s);
+        TK_HIP_CHECK(hipGetLastError());
+        rows = ref_s;
+    }
     if (!pairs) {
-        hipLaunchKernelGGL((grad_q_kernel<T, D>), qgrid, dim3(256), 0, s, q, ref, nq, k, idx, g, nullptr, (int64_t)0,
-                           dq, nullptr, nullptr, nullptr, nullptr);
+        hipLaunchKernelGGL((grad_q_kernel<T, D, IdxT>), qgrid, dim3(256), 0, s, q, rows, nq, k, idx, g, nullptr,
+                           (int64_t)0, dq, nullptr, nullptr, nullptr, nullptr);
         TK_HIP_CHECK(hipGetLastError());
         return;
     }
@@ -271,15 +304,15 @@ void backward_typed(const T* q, const T* ref, int64_t nq, int32_t n, int k, cons
     for (int64_t t0 = 0; t0 < nq; t0 += sq) {
         const int64_t cnt = std::min(sq, nq - t0);
         const int64_t mm = cnt * k;
-        hipLaunchKernelGGL((grad_q_kernel<T, D>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, q, ref, cnt, k,
-                           idx, g, order, t0, dq, q_sorted, g_sorted, keys, vals);
+        hipLaunchKernelGGL((grad_q_kernel<T, D, IdxT>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, q, rows,
+                           cnt, k, idx, g, order, t0, dq, q_sorted, g_sorted, keys, vals);
         TK_HIP_CHECK(hipGetLastError());
         radix_sort_pairs(keys, vals, keys_s, vals_s, mm, key_bits(n), ws + l.temp, l.temp_bytes, s);
         hipLaunchKernelGGL(zero_u32_kernel, rgrid, dim3(256), 0, s, run_start, run_end, (int64_t)n);
         hipLaunchKernelGGL(run_bounds_kernel, dim3((unsigned)((mm + 255) / 256)), dim3(256), 0, s, keys_s, mm,
                            run_start, run_end);
-        hipLaunchKernelGGL((grad_r_kernel<T, D>), rgrid, dim3(256), 0, s, ref, n, k, run_start, run_end, vals_s,
-                           q_sorted, g_sorted, t0 > 0, dr);
+        hipLaunchKernelGGL((grad_r_kernel<T, D>), rgrid, dim3(256), 0, s, rows, n, k, run_start, run_end, vals_s,
+                           q_sorted, g_sorted, t0 > 0, shuffled, dr);
         TK_HIP_CHECK(hipGetLastError());
     }
 }
@@ -310,10 +343,56 @@ void knn_backward_gpu(const void* queries, const void* ref, int64_t nq, int32_t 
         }
     }
     ProfScope prof("backward", stream);
-#define TK_BW(TT, DD)                                                                                                 \
-    backward_typed<TT, DD>(static_cast<const TT*>(queries), static_cast<const TT*>(ref), nq, n, k, idx,              \
-                           static_cast<const TT*>(grad_dist), static_cast<TT*>(grad_q), static_cast<TT*>(grad_ref), \
-                           stream, ws, l)
+#define TK_BW(TT, DD)                                                                                           \
+    backward_typed<TT, DD, int64_t>(static_cast<const TT*>(queries), static_cast<const TT*>(ref), nq, n, k, idx,   \
+                                    nullptr, static_cast<const TT*>(grad_dist), static_cast<TT*>(grad_q),           \
+                                    static_cast<TT*>(grad_ref), stream, ws, l)
+    if (dtype == kF32) {
+        switch (dims) {
+            case 1: TK_BW(float, 1); break;
+            case 2: TK_BW(float, 2); break;
+            case 3: TK_BW(float, 3); break;
+            case 8: TK_BW(float, 8); break;
+        }
+    } else {
+        switch (dims) {
+            case 1: TK_BW(double, 1); break;
+            case 2: TK_BW(double, 2); break;
+            case 3: TK_BW(double, 3); break;
+            case 8: TK_BW(double, 8); break;
+        }
+    }
+#undef TK_BW
+}
+
+size_t knn_backward_structured_workspace(int64_t nq, int32_t n, int k, int dims, int dtype) {
+    return bw_layout(nq, n, k, dims, dtype, true).total;
+}
+
+void knn_backward_structured_gpu(const void* queries, const void* ref, int64_t nq, int32_t n, int dims, int dtype,
+                                 int k, const int32_t* sidx, const int32_t* shuffled, const void* grad_dist,
+                                 void* grad_q, void* grad_ref, hipStream_t stream, void* workspace,
+                                 size_t workspace_bytes) {
+    if (!dims_supported(dims)) throw std::runtime_error("unsupported dims");
+    if (k < 1 || n < 1) throw std::runtime_error("invalid backward shape");
+    if (nq < 0) throw std::runtime_error("invalid backward shape");
+    if (!shuffled) throw std::runtime_error("backward: no structured-to-original map");
+    if (grad_ref && nq > kMaxPairs)
+        throw std::runtime_error("backward: the gradient of the reference points takes at most 2^28 queries");
+    const BwLayout l = bw_layout(nq, n, k, dims, dtype, true);
+    std::unique_ptr<StreamScratch> own;
+    unsigned char* ws = static_cast<unsigned char*>(workspace);
+    if (!ws) {
+        own.reset(new StreamScratch(l.total, stream));
+        ws = static_cast<unsigned char*>(own->ptr);
+    } else if (workspace_bytes < l.total) {
+        throw std::runtime_error("backward workspace too small");
+    }
+    ProfScope prof("backward", stream);
+#define TK_BW(TT, DD)                                                                                           \
+    backward_typed<TT, DD, int32_t>(static_cast<const TT*>(queries), static_cast<const TT*>(ref), nq, n, k, sidx,  \
+                                    shuffled, static_cast<const TT*>(grad_dist), static_cast<TT*>(grad_q),          \
+                                    static_cast<TT*>(grad_ref), stream, ws, l)
     if (dtype == kF32) {
         switch (dims) {
             case 1: TK_BW(float, 1); break;

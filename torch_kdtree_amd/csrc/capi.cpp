@@ -335,6 +335,26 @@ int tk_knn_backward_gpu(const void* queries, const void* ref, int64_t n_queries,
     });
 }
 
+int tk_knn_backward_tree_gpu(const tk_tree* tree, const void* queries, const void* ref, int64_t n_queries, int k,
+                             const int32_t* sidx, const void* grad_dist, void* grad_queries, void* grad_ref,
+                             void* workspace, size_t workspace_bytes, void* stream) {
+    return guarded([&] {
+        const tk::Tree& t = tree_of(tree);
+        if (!t.dev) throw std::runtime_error("The tree has no GPU copy (it was built for the CPU)");
+        tk::DeviceGuard g(t.dev->device);
+        tk::knn_backward_structured_gpu(queries, ref, n_queries, t.n_points, t.dims, t.dtype, k, sidx, t.dev->shuffled,
+                                        grad_dist, grad_queries, grad_ref, static_cast<hipStream_t>(stream), workspace,
+                                        workspace_bytes);
+    });
+}
+
+int tk_knn_backward_tree_gpu_workspace(const tk_tree* tree, int64_t n_queries, int k, size_t* bytes) {
+    return guarded([&] {
+        const tk::Tree& t = tree_of(tree);
+        *bytes = tk::knn_backward_structured_workspace(n_queries, t.n_points, k, t.dims, t.dtype);
+    });
+}
+
 int tk_knn_backward_gpu_workspace(int64_t n_queries, int32_t n_ref, int dims, int dtype, int k, size_t* bytes) {
     return guarded([&] {
         check_dtype(dtype);

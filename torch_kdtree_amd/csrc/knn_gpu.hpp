@@ -114,6 +114,15 @@ void knn_backward_gpu(const void* queries, const void* ref, int64_t nq, int32_t 
                       const int64_t* idx, const void* grad_dist, void* grad_q, void* grad_ref, hipStream_t stream,
                       void* workspace, size_t workspace_bytes);
 size_t knn_backward_workspace(int64_t nq, int32_t n, int k, int dims, int dtype);
+// The same backward with the neighbours given by structured position (sidx[nq*k], int32) and the
+// tree's structured-to-original map shuffled[n] (device memory): the same results, with the
+// reference points gathered in structured order (spatially coherent reads).  Always takes a
+// workspace (the reordered points); workspace may be null (then allocated stream-ordered).
+void knn_backward_structured_gpu(const void* queries, const void* ref, int64_t nq, int32_t n, int dims, int dtype,
+                                 int k, const int32_t* sidx, const int32_t* shuffled, const void* grad_dist,
+                                 void* grad_q, void* grad_ref, hipStream_t stream, void* workspace,
+                                 size_t workspace_bytes);
+size_t knn_backward_structured_workspace(int64_t nq, int32_t n, int k, int dims, int dtype);
 
 // Exclusive prefix sum of n int32 values (n <= 2^24) into out; sums needs scan_sums_count(n)
 // int32 of scratch.  Kernels only (no memset, no host synchronisation).

@@ -109,6 +109,22 @@ struct KDTree {
         return py::make_tuple(out[0], out[1]);
     }
 
+    // The backward for this GPU tree's query, neighbours by structured index (tk_knn_backward_tree_gpu).
+    void backward(size_t q_ptr, size_t ref_ptr, int64_t n_q, int k, size_t sidx_ptr, size_t grad_ptr,
+                  size_t grad_q_ptr, size_t grad_ref_ptr, size_t stream) const {
+        int rc;
+        {
+            py::gil_scoped_release release;
+            rc = tk_knn_backward_tree_gpu(tree, reinterpret_cast<const void*>(q_ptr),
+                                          reinterpret_cast<const void*>(ref_ptr), n_q, k,
+                                          reinterpret_cast<const int32_t*>(sidx_ptr),
+                                          reinterpret_cast<const void*>(grad_ptr), reinterpret_cast<void*>(grad_q_ptr),
+                                          reinterpret_cast<void*>(grad_ref_ptr), nullptr, 0,
+                                          reinterpret_cast<void*>(stream));
+        }
+        check(rc);
+    }
+
     bool is_finite() const {
         int32_t info[6];
         check(tk_tree_info(tree, info));
@@ -326,6 +342,10 @@ void bind_tree(py::module_& m, const char* name) {
              "Returns the ordered points how they are used in the KD-Tree")
         .def("get_nodes", &C::get_nodes, "Returns (axis[int32], split[T], leaf_offsets[int32]) of the tree")
         .def("is_finite", &C::is_finite, "True when every coordinate and split value of the tree is finite")
+        .def("backward", &C::backward, py::arg("queries_ptr"), py::arg("ref_ptr"), py::arg("n_queries"), py::arg("k"),
+             py::arg("sidx_ptr"), py::arg("grad_dist_ptr"), py::arg("grad_queries_ptr"), py::arg("grad_ref_ptr"),
+             py::arg("stream") = 0,
+             "Backward of this GPU tree's squared distances, neighbours by structured index (tk_knn_backward_tree_gpu)")
         .def("set_variant", &C::set_variant, py::arg("variant"),
              "Launch variant of later GPU queries (see torch_knn.tree_variants())")
         .def("get_variant", &C::get_variant)

@@ -52,7 +52,8 @@ def _stream_handle(device: torch.device) -> int:
 
 
 class _KnnSquaredDistances(torch.autograd.Function):
-    """Squared KNN distances with a HIP backward (``tk_knn_backward_gpu``).
+    """Squared KNN distances with a HIP backward (``tk_knn_backward_tree_gpu`` given the tree and the
+    query's structured indices, else ``tk_knn_backward_gpu``; the two give the same bits).
 
     forward returns the distances the query kernel computed; backward recomputes the
     neighbour differences from the coordinates and returns
@@ -61,27 +62,34 @@ class _KnnSquaredDistances(torch.autograd.Function):
     """
 
     @staticmethod
-    def forward(ctx, points_query, points_ref, inds, dists):
-        ctx.save_for_backward(points_query, points_ref, inds)
+    def forward(ctx, points_query, points_ref, inds, dists, sidx=None, kdtree=None):
+        ctx.save_for_backward(points_query, points_ref, inds, sidx)
+        ctx.kdtree = kdtree
         return dists.clone()
 
     @staticmethod
     def backward(ctx, grad):
-        points_query, points_ref, inds = ctx.saved_tensors
+        points_query, points_ref, inds, sidx = ctx.saved_tensors
         need_q, need_r = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
         if not (need_q or need_r):
-            return None, None, None, None
+            return None, None, None, None, None, None
         q = points_query.detach().contiguous()
         r = points_ref.detach().contiguous()
         g = grad.contiguous()
         inds = inds.contiguous()
         dq = torch.empty_like(q) if need_q else None
         dr = torch.empty_like(r) if need_r else None
-        torch_knn.knn_backward(
-            q.data_ptr(), r.data_ptr(), q.shape[0], r.shape[0], q.shape[1], _dtype_name(q.dtype), inds.shape[1],
-            inds.data_ptr(), g.data_ptr(), dq.data_ptr() if need_q else 0, dr.data_ptr() if need_r else 0,
-            _stream_handle(q.device))
-        return dq, dr, None, None
+        if ctx.kdtree is not None and sidx is not None:
+            # the points gathered in the tree's structured order: spatially coherent reads
+            ctx.kdtree.backward(q.data_ptr(), r.data_ptr(), q.shape[0], inds.shape[1], sidx.contiguous().data_ptr(),
+                                g.data_ptr(), dq.data_ptr() if need_q else 0, dr.data_ptr() if need_r else 0,
+                                _stream_handle(q.device))
+        else:
+            torch_knn.knn_backward(
+                q.data_ptr(), r.data_ptr(), q.shape[0], r.shape[0], q.shape[1], _dtype_name(q.dtype), inds.shape[1],
+                inds.data_ptr(), g.data_ptr(), dq.data_ptr() if need_q else 0, dr.data_ptr() if need_r else 0,
+                _stream_handle(q.device))
+        return dq, dr, None, None, None, None
 
 
 class TorchKDTree:
@@ -233,7 +241,7 @@ class TorchKDTree:
         if (points_query.requires_grad or self.ref_requires_grad) and torch.is_grad_enabled():
             points_ref = self.points_ref_bak.to(self.device)
             if self.use_gpu:
-                dists = _KnnSquaredDistances.apply(points_query, points_ref, inds, result_dists)
+                dists = _KnnSquaredDistances.apply(points_query, points_ref, inds, result_dists, result_idx, self.kdtree)
             else:
                 dists = torch.sum((points_query[:, None] - points_ref[inds])**2, dim=-1)
 
