@@ -1,5 +1,5 @@
 """Full-size parity of the benchmark configurations: all 16,000,000 queries of the plain GPU workload
-against the oracle, all 256 clouds of the batching workload (forward against the oracle, captured
+against the oracle (at the auto depth and at the reference's default depth), all 256 clouds of the batching workload (forward against the oracle, captured
 fwd+bwd against the single-tree path and fp64 autograd), the d=8 k=32 backward, and the backward
 beyond 2^28 (query, neighbour) pairs."""
 import numpy as np
@@ -57,6 +57,20 @@ def test_plain_gpu_all_queries(cuda):
     k = 16, bit-exact against the oracle on the same tree."""
     w = workloads.plain_gpu()
     tree = tk.build_kd_tree(torch.from_numpy(w["ref"]), device=cuda, levels="auto")
+    gd, gi, go = gpu_query(tree, w["queries"], 16, cuda)
+    od, oi = oracle_for(tree).search(w["queries"], 16)
+    assert_same(gd, od, "distances")
+    assert_same(gi, oi, "structured indices")
+    del gd, od
+    assert_same(go, tree.kdtree.get_shuffled_inds()[oi].astype(np.int64), "original indices")
+
+
+def test_plain_gpu_all_queries_reference_default_depth(cuda):
+    """The same 16,000,000 queries on the tree a plain build_kd_tree(points) gives (depth 13, 488
+    points a leaf, record-group boxes), k = 16, bit-exact against the oracle on the same tree."""
+    w = workloads.plain_gpu()
+    tree = tk.build_kd_tree(torch.from_numpy(w["ref"]), device=cuda)
+    assert tree.levels == 13
     gd, gi, go = gpu_query(tree, w["queries"], 16, cuda)
     od, oi = oracle_for(tree).search(w["queries"], 16)
     assert_same(gd, od, "distances")
