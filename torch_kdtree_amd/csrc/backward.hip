@@ -19,6 +19,12 @@
 // are close in the canonical order, so a run's reads are close together.  The result is
 // bit-identical from run to run, and equal to the batched backward's on the same cloud.
 //
+// Structured form (knn_backward_structured_gpu, for the neighbours of a GPU tree query given by
+// structured index): the points are first gathered into the tree's structured order, the pairs are
+// keyed by structured position, and each position's sums are written to its original row.  The
+// runs hold the same pairs in the same canonical order, so the results are the same bits; the
+// gathers of spatially close queries then read close rows.
+//
 // Size: the pairs of one sort are at most 2^28 (the radix sort's limit).  With more, the canonical
 // order is cut into consecutive slices of at most 2^28 / k queries, and each slice's runs are
 // subtracted from the sums the previous slices left (the first slice starts from zero).  Every
