@@ -133,6 +133,28 @@ def test_unaligned_output_rows(cuda):
             assert_same(o.cpu().numpy(), o0, "%s k=%d original indices" % (policy, k))
 
 
+@pytest.mark.parametrize("d", [1, 2, 3, 8])
+@pytest.mark.parametrize("k", [5, 8, 12, 16])
+def test_lower_index_ties_among_distinct_distances(cuda, d, k):
+    """Lower-index policy, float lists of 8 and 16: a cloud where one point in ten has an exact
+    duplicate (every distance to the pair ties) and queries on and off the points, so that lanes
+    whose lists meet a tie (searched again with the 64-bit keyed list) share wavefronts with lanes
+    that meet none."""
+    rng = np.random.default_rng(900 + d * 13 + k)
+    base = rng.standard_normal((18000, d)).astype(np.float32)
+    dup = base[rng.choice(len(base), 1800, replace=False)]
+    ref = np.concatenate([base, dup])[rng.permutation(len(base) + len(dup))]
+    qry = np.concatenate([rng.standard_normal((3000, d)).astype(np.float32),
+                          ref[rng.choice(len(ref), 1000, replace=False)]])
+    for levels in (None, 10):
+        tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels=levels, tie_break="lower_index")
+        dist, sidx, oidx = gpu_query(tree, qry, k, cuda)
+        bd, bi = oracle.bruteforce(ref, qry, k)
+        assert_same(dist, bd, "distances")
+        assert_same(oidx, bi.astype(np.int64), "original indices")
+        assert_same(tree.kdtree.get_shuffled_inds()[sidx], bi, "structured -> original")
+
+
 @pytest.mark.parametrize("policy", ["reference", "lower_index"])
 def test_exact_ties_on_integer_grid(cuda, policy):
     """Points and queries on a small integer lattice: nearly every distance is tied."""

@@ -258,6 +258,8 @@ struct KBest {
         else
             return dist < d[K - 1];
     }
+    // The second acceptance test of a staged leaf scan (against the current list).
+    __device__ __forceinline__ bool admit(T dist, int32_t key) { return accepts(dist, key); }
 
     // The worst entry as of now, for acceptance tests against a stale list.
     struct Stale {
@@ -330,6 +332,7 @@ struct KBest<float, K, true> {
     __device__ __forceinline__ void set(int j, float dist, int32_t key) { k[j] = pack(dist, key); }
 
     __device__ __forceinline__ bool accepts(float dist, int32_t key) const { return pack(dist, key) < k[K - 1]; }
+    __device__ __forceinline__ bool admit(float dist, int32_t key) { return accepts(dist, key); }
 
     using Stale = uint64_t;
     __device__ __forceinline__ Stale stale() const { return k[K - 1]; }

@@ -78,6 +78,9 @@ constexpr int kGroupMinLeaf = 4 * kGroupRecs;
 #ifndef TK_STAGE_CHUNK_F32_LOWER
 #define TK_STAGE_CHUNK_F32_LOWER 4
 #endif
+#ifndef TK_STAGE_CHUNK_F32_TIE
+#define TK_STAGE_CHUNK_F32_TIE 8
+#endif
 #ifndef TK_STAGE_CHUNK_SMALLK
 #define TK_STAGE_CHUNK_SMALLK 8
 #endif
@@ -115,9 +118,11 @@ constexpr int kGroupMinLeaf = 4 * kGroupRecs;
 // with d <= 3 and lists of at most 16 (measured on MI355X, plain workload, k = 16):
 //   reference policy, TK_F32_WAVES = 7: 72 VGPRs, no spills, 8.48 ms (5 waves: the same 72 VGPRs
 //     scheduled differently, 8.77 ms; no target: 91 VGPRs, 9.14 ms; 8 waves: spills, 9.29 ms);
-//   lower-index policy (leaf chunks of 4), TK_F32_LOWER_WAVES = 7: 72 VGPRs and 16 bytes of
-//     spills, 9.75 ms (5 waves: 77 VGPRs, 10.08 ms).  With chunks of 8 the kernel needed 92 VGPRs
-//     at 5 waves (10.62 ms) and spilled 204 bytes at 7 (26.4 ms).
+//   lower-index policy with the KBestTie list (leaf chunks of 8): TK_F32_TIE_WAVES_K16 = 5 for
+//     lists of 16 (90 VGPRs, 9.61-9.64 ms with the presort; chunks of 4 at 7 waves: 10.38-10.45
+//     ms), TK_F32_LOWER_WAVES = 7 for lists of 8 (k = 8: 6.61-6.65 ms; chunks of 4: 6.86-6.88) and
+//     for lists of at most 4.  With the 64-bit keyed list, chunks of 4 at 7 waves had been best
+//     (72 VGPRs, 16 bytes of spills, 9.75 ms; with chunks of 8, 92 VGPRs at 5 waves, 10.62 ms).
 // Other kernels: see TK_F64_WAVES and TK_WIDE_WAVES below, no target for the rest.
 // TK_WAVES_PER_EU overrides every target (0 = no target anywhere).
 #ifndef TK_WAVES_PER_EU
@@ -128,6 +133,14 @@ constexpr int kGroupMinLeaf = 4 * kGroupRecs;
 #endif
 #ifndef TK_F32_LOWER_WAVES
 #define TK_F32_LOWER_WAVES 7
+#endif
+// Lower-index float searches with lists of 8 or 16 keep the reference policy's list plus tie
+// flags (KBestTie) instead of the 64-bit keyed list (0).
+#ifndef TK_LOWER_TIE_LIST
+#define TK_LOWER_TIE_LIST 1
+#endif
+#ifndef TK_F32_TIE_WAVES_K16
+#define TK_F32_TIE_WAVES_K16 5
 #endif
 // Double, d = 3, k = 16: no target (119 VGPRs, 4 waves; a 5-wave target spills 48 bytes, and a
 // 4-wave one measured neutral).  Float, d = 8, k = 32: 4 waves (128 VGPRs, 12 bytes of spills,
@@ -142,7 +155,8 @@ template <typename T, int D, int K, bool kLower>
 struct TreeOccupancy {
     static constexpr int kWaves =
         TK_WAVES_PER_EU >= 0 ? TK_WAVES_PER_EU
-        : (sizeof(T) == 4 && D <= 3 && K <= 16) ? (kLower ? TK_F32_LOWER_WAVES : TK_F32_WAVES)
+        : (sizeof(T) == 4 && D <= 3 && K <= 16)
+            ? (kLower ? (K == 16 && TK_LOWER_TIE_LIST ? TK_F32_TIE_WAVES_K16 : TK_F32_LOWER_WAVES) : TK_F32_WAVES)
         : (sizeof(T) == 8 && D <= 3 && K == 16 && !kLower) ? TK_F64_WAVES
         : (sizeof(T) == 4 && D == 8 && K == 32 && !kLower) ? TK_WIDE_WAVES
                                                             : 0;
@@ -268,6 +282,73 @@ struct KBestExact {
     }
 };
 
+// Lower-index policy on the reference policy's register list (one med3 per slot; the keys are
+// original indices).  The two policies' lists can differ only through exact distance ties: when
+// no point the search scans outside the list has a distance equal to the final worst W and the
+// list's k distances are distinct, both hold the k points of smallest distance in ascending
+// order.  `tie` records every event after which that may not hold:
+// * a candidate equal to the current worst in the second pass (the first pass passes every
+//   distance <= the stale worst, so a point at W is always a candidate, and the current worst it
+//   then meets is >= W);
+// * an insertion whose evicted entry equals the new worst (an evicted distance is >= W, and
+//   equal to W only if equal to that worst, which never falls below W; empty slots aside);
+// and unsure() adds equal distances among the k live entries (their order must follow the
+// original index).  The search prunes with the lower-index policy's tests, so every point with
+// distance <= W is scanned.  A lane that is unsure searches again with the lower-index list
+// (research_lower): rare on clouds without exact fp32 distance ties.
+template <typename T, int K>
+struct KBestTie : KBest<T, K, false> {
+    using Base = KBest<T, K, false>;
+    bool tie;
+
+    __device__ __forceinline__ void init(int k) {
+        Base::init(k);
+        tie = false;
+    }
+    static __device__ __forceinline__ bool accepts_stale(const typename Base::Stale& w, T dist, int32_t) {
+        return dist <= w.d;
+    }
+    __device__ __forceinline__ bool admit(T dist, int32_t) {
+        tie |= dist == Base::worst();
+        return dist < Base::worst();
+    }
+    // (an entry at +inf is an empty slot: a +inf distance is never inserted, so evicting one is no tie)
+    __device__ __forceinline__ void insert(T dist, int32_t key) {
+        const T old = Base::worst();
+        Base::insert(dist, key);
+        tie |= dist < old && Base::worst() == old && old < pos_inf<T>();
+    }
+    __device__ __forceinline__ bool unsure(int k) const {
+        bool t = tie;
+#pragma unroll
+        for (int j = 0; j + 1 < K; ++j) t |= j >= K - k && Base::d[j] == Base::d[j + 1];
+        return t;
+    }
+};
+
+// The list interface of scan_leaf / knn_search_one over a register list.
+template <typename T, typename L>
+struct RegListRef {
+    L& l;
+    __device__ __forceinline__ T worst() const { return l.worst(); }
+    __device__ __forceinline__ bool accepts(T dist, int32_t orig) const { return l.accepts(dist, orig); }
+    __device__ __forceinline__ void insert(T dist, int32_t, int32_t orig) { l.insert(dist, orig); }
+};
+
+// The list of a single-tree register search.  Lower-index float searches with lists of 8 or 16
+// take KBestTie (TK_LOWER_TIE_LIST 1): the 64-bit keyed list spends more issue slots per
+// insertion than the med3 list plus the tie flags.
+template <typename T, int K, bool kLower>
+struct TreeList {
+    using type = KBest<T, K, kLower>;
+    static constexpr bool kTie = false;
+};
+template <int K>
+struct TreeList<float, K, true> {
+    static constexpr bool kTie = TK_LOWER_TIE_LIST && (K == 8 || K == 16);
+    using type = typename std::conditional<kTie, KBestTie<float, K>, KBest<float, K, true>>::type;
+};
+
 // The original index of the record at structured position s: from the record where its layout
 // has room for it, else from the tree's `shuffled` map (indexed, like the records, by position).
 template <typename T, int D, bool kB>
@@ -385,9 +466,9 @@ constexpr int kRecPad = 16;
 // In pass 2 a candidate that the current list rejects is inserted as +inf, which leaves the list
 // unchanged: the insertion is then unconditional and updates the list in place.
 template <typename T, int D, int K, bool kLower, int CH, int LD, int SB = kTreeBlockStride, bool kGroups = false,
-          bool kB>
+          bool kB, typename List>
 __device__ __forceinline__ void scan_leaf_staged(const TreeView<T, kB>& tv, int32_t leaf, const Pt<T, D>& q,
-                                                 KBest<T, K, kLower>& kb, T* __restrict__ st_d,
+                                                 List& kb, T* __restrict__ st_d,
                                                  int32_t* __restrict__ st_o) {
     static_assert(CH % LD == 0 && CH <= 32, "a chunk is a whole number of load groups, at most 32 points");
     static_assert(!kGroups || kGroupRecs % CH == 0, "chunks nest in record groups");
@@ -437,7 +518,7 @@ __device__ __forceinline__ void scan_leaf_staged(const TreeView<T, kB>& tv, int3
             mask &= mask - 1u;
             const T dd = st_d[j * SB];
             const int32_t o = kLower ? st_o[j * SB] : 0;
-            const bool ok = kb.accepts(dd, o);
+            const bool ok = kb.admit(dd, o);
             kb.insert(ok ? dd : pos_inf<T>(), ok ? (kLower ? o : c0 + j) : 0x7fffffff);
         }
     }
@@ -517,8 +598,8 @@ __device__ __forceinline__ void descent_leaf(const TreeView<T, kB>& tv, const Pt
 // The list of a NaN query under the reference policy, from an initialised list: slot c (0 =
 // nearest) holds record end-1-c of the descent's leaf while that is in the leaf.  Under the
 // lower-index policy the initialised list is already the result.
-template <typename T, int D, int K, bool kLower, bool kB>
-__device__ __forceinline__ void nan_fill(const TreeView<T, kB>& tv, const Pt<T, D>& q, int k, KBest<T, K, kLower>& kb) {
+template <typename T, int D, int K, bool kLower, bool kB, typename List>
+__device__ __forceinline__ void nan_fill(const TreeView<T, kB>& tv, const Pt<T, D>& q, int k, List& kb) {
     if constexpr (!kLower) {
         int32_t beg = 0, end = 0;
         descent_leaf<T, D, kB>(tv, q, beg, end);
@@ -532,9 +613,9 @@ __device__ __forceinline__ void nan_fill(const TreeView<T, kB>& tv, const Pt<T, 
 }
 
 // Starts the search of the query in q.
-template <typename T, int D, int K, bool kLower, bool kB>
+template <typename T, int D, int K, bool kLower, bool kB, typename List>
 __device__ __forceinline__ void lane_start(const TreeView<T, kB>& tv, int k, const Pt<T, D>& q, Pt<T, D>& pp,
-                                           KBest<T, K, kLower>& kb, uint32_t& far_mask, int32_t& node,
+                                           List& kb, uint32_t& far_mask, int32_t& node,
                                            int& level) {
     pp = q;
     kb.init(k);
@@ -547,9 +628,9 @@ __device__ __forceinline__ void lane_start(const TreeView<T, kB>& tv, int k, con
 // Scans the leaf pair of the current leaf-parent, then climbs to the next far side to enter and
 // walks down to its leaf-parent.  Returns true when the search is complete.
 template <typename T, int D, int K, bool kLower, int CH, int LD, int SB = kTreeBlockStride, bool kGroups = false,
-          bool kB>
+          bool kB, typename List>
 __device__ __forceinline__ bool lane_step(const TreeView<T, kB>& tv, const Pt<T, D>& q, Pt<T, D>& pp,
-                                          KBest<T, K, kLower>& kb, uint32_t& far_mask, int32_t& node, int& level,
+                                          List& kb, uint32_t& far_mask, int32_t& node, int& level,
                                           T* __restrict__ st_d, int32_t* __restrict__ st_o) {
     const int L = tv.levels;
     const int a_lp = axis_of_level<D>(L - 1);  // split axis of every leaf-parent
@@ -766,8 +847,8 @@ __device__ __forceinline__ void store_slots(V* __restrict__ p, bool vec, int pad
 // stores, so their latencies overlap instead of following one another (a 32-entry list goes out
 // in two halves, which keeps the live registers below those of its search); when k == K and the
 // three arrays are 16-byte aligned, the rows go out in 16-byte stores.
-template <typename T, int K, bool kLower, int J0 = 0>
-__device__ __forceinline__ void store_list(const KBest<T, K, kLower>& kb, int k, const TreeView<T>& tv, int64_t qi,
+template <typename T, int K, bool kLower, int J0 = 0, typename List>
+__device__ __forceinline__ void store_list(const List& kb, int k, const TreeView<T>& tv, int64_t qi,
                                            T* __restrict__ out_d, int32_t* __restrict__ out_i,
                                            int64_t* __restrict__ out_o) {
     constexpr int N = K - J0 < 16 ? K - J0 : 16;
@@ -854,8 +935,8 @@ constexpr int wave_slab_bytes() {
 // store_list for a full wavefront whose rows are k == K entries long and 16-byte aligned: the same
 // values, through wave_store_part.  The distances go out before the index gathers are issued, so
 // the two are not live at once.
-template <typename T, int K, bool kLower, int BLK, int SW, int J0 = 0>
-__device__ __forceinline__ void store_list_wave(const KBest<T, K, kLower>& kb, const TreeView<T>& tv,
+template <typename T, int K, bool kLower, int BLK, int SW, int J0 = 0, typename List>
+__device__ __forceinline__ void store_list_wave(const List& kb, const TreeView<T>& tv,
                                                 const WaveStage<T, BLK>& st, int lane,
                                                 const int32_t (&qr)[SW / 16], T* __restrict__ out_d,
                                                 int32_t* __restrict__ out_i, int64_t* __restrict__ out_o) {
@@ -910,17 +991,19 @@ __device__ __forceinline__ void load_query(const T* __restrict__ queries, int64_
 // and lists of 8 or more: TK_STAGE_CHUNK_F32_K16 (16) for the single-tree reference-policy
 // search with a 16-entry list (plain workload 8.32 ms against 8.44 at 8), TK_STAGE_CHUNK_F32 (8)
 // for the other reference-policy ones (k = 8: 6.14 ms against 6.20 at 16; batched replay 21.76 ms
-// against 21.93), TK_STAGE_CHUNK_F32_LOWER (4) for the lower-index policy (10.14 ms against 10.56
-// at 8).  For lists of at most 4 at d <= 3, TK_STAGE_CHUNK_SMALLK (8) in the single-tree search
+// against 21.93), TK_STAGE_CHUNK_F32_TIE (8) for the lower-index policy with the KBestTie list
+// (see TK_F32_TIE_WAVES_K16), TK_STAGE_CHUNK_F32_LOWER (4) for its 64-bit keyed lists (10.14 ms
+// against 10.56 at 8).  For lists of at most 4 at d <= 3, TK_STAGE_CHUNK_SMALLK (8) in the single-tree search
 // (k = 4: 4.78 ms against 4.91 at 4; k = 1: 3.75 against 3.80) and TK_STAGE_CHUNK_WIDE (4) in the
 // batched one (replay 20.43 ms against 20.74 at 8).  TK_STAGE_CHUNK_WIDE (4) also for d = 8, where a
 // larger chunk's extra stale-worst pops cost more than its fewer exits; TK_STAGE_CHUNK_F64 for double.
 // kB: the batched search.
 template <typename T, int D, int K, bool kLower, bool kB = false>
 struct StageBytes {
+    static constexpr bool kTieList = !kB && TreeList<T, K, kLower>::kTie;
     static constexpr int kChunk =
         sizeof(T) == 4
-            ? ((D <= 3 && K >= 8) ? (kLower ? TK_STAGE_CHUNK_F32_LOWER
+            ? ((D <= 3 && K >= 8) ? (kLower ? (kTieList ? TK_STAGE_CHUNK_F32_TIE : TK_STAGE_CHUNK_F32_LOWER)
                                             : (K == 16 && !kB ? TK_STAGE_CHUNK_F32_K16 : TK_STAGE_CHUNK_F32))
                                   : (D <= 3 && !kB ? TK_STAGE_CHUNK_SMALLK : TK_STAGE_CHUNK_WIDE))
             : TK_STAGE_CHUNK_F64;
@@ -928,6 +1011,19 @@ struct StageBytes {
                                                  : (TK_SCAN_LOADS_F64 < kChunk ? TK_SCAN_LOADS_F64 : kChunk);
     static constexpr int kBytes = kChunk * kTreeBlock * ((int)sizeof(T) + (kLower ? 4 : 0));
 };
+
+// The lower-index search of q again, node by node with the 64-bit keyed list, into kb (for a lane
+// whose KBestTie list is unsure).
+template <typename T, int D, int K, bool kB>
+__device__ __forceinline__ void research_lower(const TreeView<T, kB>& tv, const Pt<T, D>& q, int k,
+                                               KBestTie<T, K>& kb) {
+    KBest<T, K, true> ex;
+    ex.init(k);
+    RegListRef<T, KBest<T, K, true>> ref{ex};
+    knn_search_one<T, D, true>(tv, q, ref);
+#pragma unroll
+    for (int j = 0; j < K; ++j) kb.set(j, ex.dist(j), ex.key(j));
+}
 
 // One lane per query.  The lanes of a wavefront take 64 consecutive presorted queries and run
 // their searches to completion together: neighbouring queries walk the tree nearly in step, which
@@ -948,7 +1044,7 @@ __device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T
     const int64_t wave0 = xcd_tile(blockIdx.x, gridDim.x) * BLK + wave_off;
     if (wave0 + (int)(threadIdx.x & 63u) >= nq) return;
     Pt<T, D> q, pp;
-    KBest<T, K, kLower> kb;
+    typename TreeList<T, K, kLower>::type kb;
     uint32_t far_mask;
     int32_t node;
     int level;
@@ -957,12 +1053,15 @@ __device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T
         load_query<T, D>(queries, order ? (int64_t)order[t] : t, q);
     }
     lane_start<T, D, K, kLower>(tv, k, q, pp, kb, far_mask, node, level);
-    if (has_nan<T, D>(q))
+    if (has_nan<T, D>(q)) {
         nan_fill<T, D, K, kLower>(tv, q, k, kb);
-    else
+    } else {
         while (!lane_step<T, D, K, kLower, CH, LD, BLK, kGroups>(tv, q, pp, kb, far_mask, node, level, stage_d + threadIdx.x,
                                                       stage_o + (kLower ? threadIdx.x : 0))) {
         }
+        if constexpr (TreeList<T, K, kLower>::kTie)
+            if (kb.unsure(k)) research_lower<T, D, K>(tv, q, k, kb);
+    }
     // (the lane index from mbcnt in a volatile asm, and the row by an atomic load, so that the
     // compiler recomputes both here rather than carrying the first values through the search)
     int lane;
