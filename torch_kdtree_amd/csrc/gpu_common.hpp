@@ -233,6 +233,7 @@ template <typename T, int K, bool kLower>
 struct KBest {
     T d[K];
     int32_t x[K];
+    static constexpr bool kOrigKeys = kLower;  // keys are original (else structured) indices
 
     __device__ __forceinline__ void init(int k) {
         const int pad = K - k;
@@ -315,6 +316,7 @@ struct KBest {
 template <int K>
 struct KBest<float, K, true> {
     uint64_t k[K];
+    static constexpr bool kOrigKeys = true;
 
     static __device__ __forceinline__ uint64_t pack(float d, int32_t key) {
         return ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)key;

@@ -78,9 +78,6 @@ constexpr int kGroupMinLeaf = 4 * kGroupRecs;
 #ifndef TK_STAGE_CHUNK_F32_LOWER
 #define TK_STAGE_CHUNK_F32_LOWER 4
 #endif
-#ifndef TK_STAGE_CHUNK_F32_TIE
-#define TK_STAGE_CHUNK_F32_TIE 8
-#endif
 #ifndef TK_STAGE_CHUNK_SMALLK
 #define TK_STAGE_CHUNK_SMALLK 8
 #endif
@@ -118,11 +115,11 @@ constexpr int kGroupMinLeaf = 4 * kGroupRecs;
 // with d <= 3 and lists of at most 16 (measured on MI355X, plain workload, k = 16):
 //   reference policy, TK_F32_WAVES = 7: 72 VGPRs, no spills, 8.48 ms (5 waves: the same 72 VGPRs
 //     scheduled differently, 8.77 ms; no target: 91 VGPRs, 9.14 ms; 8 waves: spills, 9.29 ms);
-//   lower-index policy with the KBestTie list (leaf chunks of 8): TK_F32_TIE_WAVES_K16 = 5 for
-//     lists of 16 (90 VGPRs, 9.61-9.64 ms with the presort; chunks of 4 at 7 waves: 10.38-10.45
-//     ms), TK_F32_LOWER_WAVES = 7 for lists of 8 (k = 8: 6.61-6.65 ms; chunks of 4: 6.86-6.88) and
-//     for lists of at most 4.  With the 64-bit keyed list, chunks of 4 at 7 waves had been best
-//     (72 VGPRs, 16 bytes of spills, 9.75 ms; with chunks of 8, 92 VGPRs at 5 waves, 10.62 ms).
+//   lower-index policy: lists of 8 and 16 (KBestTie, keyed like the reference policy's) take the
+//     reference policy's shape (k = 16: 72 VGPRs, 8.29-8.63 ms); TK_F32_LOWER_WAVES = 7 for its
+//     64-bit keyed lists (lists of at most 4; with that list at k = 16, chunks of 4 at 7 waves had
+//     been best: 72 VGPRs and 16 bytes of spills, 9.75 ms; with chunks of 8, 92 VGPRs at 5 waves,
+//     10.62 ms).
 // Other kernels: see TK_F64_WAVES and TK_WIDE_WAVES below, no target for the rest.
 // TK_WAVES_PER_EU overrides every target (0 = no target anywhere).
 #ifndef TK_WAVES_PER_EU
@@ -134,13 +131,10 @@ constexpr int kGroupMinLeaf = 4 * kGroupRecs;
 #ifndef TK_F32_LOWER_WAVES
 #define TK_F32_LOWER_WAVES 7
 #endif
-// Lower-index float searches with lists of 8 or 16 keep the reference policy's list plus tie
+// Lower-index float searches with lists of 8, 16 or 32 keep the reference policy's list plus tie
 // flags (KBestTie) instead of the 64-bit keyed list (0).
 #ifndef TK_LOWER_TIE_LIST
 #define TK_LOWER_TIE_LIST 1
-#endif
-#ifndef TK_F32_TIE_WAVES_K16
-#define TK_F32_TIE_WAVES_K16 5
 #endif
 // Double, d = 3, k = 16: no target (119 VGPRs, 4 waves; a 5-wave target spills 48 bytes, and a
 // 4-wave one measured neutral).  Float, d = 8, k = 32: 4 waves (128 VGPRs, 12 bytes of spills,
@@ -156,9 +150,9 @@ struct TreeOccupancy {
     static constexpr int kWaves =
         TK_WAVES_PER_EU >= 0 ? TK_WAVES_PER_EU
         : (sizeof(T) == 4 && D <= 3 && K <= 16)
-            ? (kLower ? (K == 16 && TK_LOWER_TIE_LIST ? TK_F32_TIE_WAVES_K16 : TK_F32_LOWER_WAVES) : TK_F32_WAVES)
+            ? (kLower && !(TK_LOWER_TIE_LIST && (K == 8 || K == 16)) ? TK_F32_LOWER_WAVES : TK_F32_WAVES)
         : (sizeof(T) == 8 && D <= 3 && K == 16 && !kLower) ? TK_F64_WAVES
-        : (sizeof(T) == 4 && D == 8 && K == 32 && !kLower) ? TK_WIDE_WAVES
+        : (sizeof(T) == 4 && D == 8 && K == 32 && (!kLower || TK_LOWER_TIE_LIST)) ? TK_WIDE_WAVES
                                                             : 0;
 };
 #define TK_TREE_OCCUPANCY(T, D, K, L) __attribute__((amdgpu_waves_per_eu(TreeOccupancy<T, D, K, L>::kWaves > 0 ? TreeOccupancy<T, D, K, L>::kWaves : 1)))
@@ -283,7 +277,9 @@ struct KBestExact {
 };
 
 // Lower-index policy on the reference policy's register list (one med3 per slot; the keys are
-// original indices).  The two policies' lists can differ only through exact distance ties: when
+// structured indices, as under the reference policy, since the key order no longer decides
+// anything: ties are settled by the flag below).  The two policies' lists can differ only through
+// exact distance ties: when
 // no point the search scans outside the list has a distance equal to the final worst W and the
 // list's k distances are distinct, both hold the k points of smallest distance in ascending
 // order.  `tie` records every event after which that may not hold:
@@ -335,9 +331,12 @@ struct RegListRef {
     __device__ __forceinline__ void insert(T dist, int32_t, int32_t orig) { l.insert(dist, orig); }
 };
 
-// The list of a single-tree register search.  Lower-index float searches with lists of 8 or 16
-// take KBestTie (TK_LOWER_TIE_LIST 1): the 64-bit keyed list spends more issue slots per
-// insertion than the med3 list plus the tie flags.
+// The list of a single-tree register search.  Lower-index float searches with lists of 8, 16 or
+// 32 take KBestTie (TK_LOWER_TIE_LIST 1): the 64-bit keyed list spends more issue slots per
+// insertion than the med3 list plus the tie flags, and needs the original indices staged with the
+// distances (plain workload, k = 16: 9.96-9.99 ms against 8.29-8.63; k = 32: 22.1-22.2 against
+// 16.9-17.1; d8_mixture, k = 32, 500K queries: 87.1-90.4 against 78.1-79.1).  Lists of at most 4
+// keep it: their insertions are short.
 template <typename T, int K, bool kLower>
 struct TreeList {
     using type = KBest<T, K, kLower>;
@@ -345,7 +344,7 @@ struct TreeList {
 };
 template <int K>
 struct TreeList<float, K, true> {
-    static constexpr bool kTie = TK_LOWER_TIE_LIST && (K == 8 || K == 16);
+    static constexpr bool kTie = TK_LOWER_TIE_LIST && (K == 8 || K == 16 || K == 32);
     using type = typename std::conditional<kTie, KBestTie<float, K>, KBest<float, K, true>>::type;
 };
 
@@ -504,11 +503,11 @@ __device__ __forceinline__ void scan_leaf_staged(const TreeView<T, kB>& tv, int3
                 const int j = g + jj;
                 const Rec<T, D>& r = rg[jj];
                 const T dd = sq_dist<T, D>(q, r.c);
-                const int32_t o = kLower ? rec_orig(tv, c0 + j, r) : 0;
+                const int32_t o = List::kOrigKeys ? rec_orig(tv, c0 + j, r) : 0;
                 const bool acc = kb.accepts_stale(w, dd, o);
                 mask |= acc ? (1u << j) : 0u;
                 st_d[j * SB] = dd;
-                if constexpr (kLower) st_o[j * SB] = o;
+                if constexpr (List::kOrigKeys) st_o[j * SB] = o;
             }
         }
         mask &= n >= 32 ? ~0u : ((1u << n) - 1u);  // only the leaf's points
@@ -517,9 +516,9 @@ __device__ __forceinline__ void scan_leaf_staged(const TreeView<T, kB>& tv, int3
             const int j = __builtin_ctz(mask);
             mask &= mask - 1u;
             const T dd = st_d[j * SB];
-            const int32_t o = kLower ? st_o[j * SB] : 0;
+            const int32_t o = List::kOrigKeys ? st_o[j * SB] : 0;
             const bool ok = kb.admit(dd, o);
-            kb.insert(ok ? dd : pos_inf<T>(), ok ? (kLower ? o : c0 + j) : 0x7fffffff);
+            kb.insert(ok ? dd : pos_inf<T>(), ok ? (List::kOrigKeys ? o : c0 + j) : 0x7fffffff);
         }
     }
 }
@@ -597,7 +596,8 @@ __device__ __forceinline__ void descent_leaf(const TreeView<T, kB>& tv, const Pt
 
 // The list of a NaN query under the reference policy, from an initialised list: slot c (0 =
 // nearest) holds record end-1-c of the descent's leaf while that is in the leaf.  Under the
-// lower-index policy the initialised list is already the result.
+// lower-index policy the initialised list (+inf, original index 0) is already the result; a list
+// keyed by structured index then takes the position of original index 0 as its key.
 template <typename T, int D, int K, bool kLower, bool kB, typename List>
 __device__ __forceinline__ void nan_fill(const TreeView<T, kB>& tv, const Pt<T, D>& q, int k, List& kb) {
     if constexpr (!kLower) {
@@ -609,6 +609,10 @@ __device__ __forceinline__ void nan_fill(const TreeView<T, kB>& tv, const Pt<T, 
             const int32_t rec = end - 1 - (j - pad);
             if (j >= pad && rec >= beg) kb.set(j, sq_dist<T, D>(q, load_rec<T, D>(tv.recs, rec).c), rec);
         }
+    } else if constexpr (!List::kOrigKeys) {
+        const int32_t s0 = load_i32_at(tv.inv_shuffled, 0);
+#pragma unroll
+        for (int j = 0; j < K; ++j) kb.set(j, kb.dist(j), s0);
     }
 }
 
@@ -841,13 +845,13 @@ __device__ __forceinline__ void store_slots(V* __restrict__ p, bool vec, int pad
 }
 
 // Writes one query's list to row qi of the outputs: distances, structured indices and (with
-// out_o) original indices.  The reference policy keeps structured indices and gathers the
-// original ones through `shuffled`; the lower-index policy keeps original indices and gathers the
-// structured ones through `inv_shuffled`.  The gathers of up to 16 slots are issued before their
+// out_o) original indices.  A list keyed by structured index (kOrig false) gathers the original
+// ones through `shuffled`; one keyed by original index (the lower-index policy's 64-bit keyed
+// list) gathers the structured ones through `inv_shuffled`.  The gathers of up to 16 slots are issued before their
 // stores, so their latencies overlap instead of following one another (a 32-entry list goes out
 // in two halves, which keeps the live registers below those of its search); when k == K and the
 // three arrays are 16-byte aligned, the rows go out in 16-byte stores.
-template <typename T, int K, bool kLower, int J0 = 0, typename List>
+template <typename T, int K, bool kOrig, int J0 = 0, typename List>
 __device__ __forceinline__ void store_list(const List& kb, int k, const TreeView<T>& tv, int64_t qi,
                                            T* __restrict__ out_d, int32_t* __restrict__ out_i,
                                            int64_t* __restrict__ out_o) {
@@ -855,19 +859,19 @@ __device__ __forceinline__ void store_list(const List& kb, int k, const TreeView
     const int pad = K - k;
     const bool vec = pad == 0 && ((((uintptr_t)out_d | (uintptr_t)out_i | (uintptr_t)out_o) & 15u) == 0);
     int32_t m[N] = {};
-    if (kLower || out_o) {
-        const int32_t* map = kLower ? tv.inv_shuffled : tv.shuffled;
+    if (kOrig || out_o) {
+        const int32_t* map = kOrig ? tv.inv_shuffled : tv.shuffled;
 #pragma unroll
         for (int u = 0; u < N; ++u) m[u] = load_i32_at(map, J0 + u >= pad ? kb.key(J0 + u) : 0);
     }
     T* od = out_d + qi * k;
     int32_t* oi = out_i + qi * k;
     store_slots<T, J0, N>(od, vec, pad, [&](int j) { return kb.dist(j); });
-    store_slots<int32_t, J0, N>(oi, vec, pad, [&](int j) { return kLower ? m[j - J0] : kb.key(j); });
+    store_slots<int32_t, J0, N>(oi, vec, pad, [&](int j) { return kOrig ? m[j - J0] : kb.key(j); });
     if (out_o)
         store_slots<int64_t, J0, N>(out_o + qi * k, vec, pad,
-                                    [&](int j) { return (int64_t)(kLower ? kb.key(j) : m[j - J0]); });
-    if constexpr (J0 + N < K) store_list<T, K, kLower, J0 + N>(kb, k, tv, qi, out_d, out_i, out_o);
+                                    [&](int j) { return (int64_t)(kOrig ? kb.key(j) : m[j - J0]); });
+    if constexpr (J0 + N < K) store_list<T, K, kOrig, J0 + N>(kb, k, tv, qi, out_d, out_i, out_o);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -935,7 +939,7 @@ constexpr int wave_slab_bytes() {
 // store_list for a full wavefront whose rows are k == K entries long and 16-byte aligned: the same
 // values, through wave_store_part.  The distances go out before the index gathers are issued, so
 // the two are not live at once.
-template <typename T, int K, bool kLower, int BLK, int SW, int J0 = 0, typename List>
+template <typename T, int K, bool kOrig, int BLK, int SW, int J0 = 0, typename List>
 __device__ __forceinline__ void store_list_wave(const List& kb, const TreeView<T>& tv,
                                                 const WaveStage<T, BLK>& st, int lane,
                                                 const int32_t (&qr)[SW / 16], T* __restrict__ out_d,
@@ -951,15 +955,15 @@ __device__ __forceinline__ void store_list_wave(const List& kb, const TreeView<T
                                     J0 * (int)sizeof(T), v);
     }
     int32_t m[N] = {};
-    if (kLower || out_o) {
-        const int32_t* map = kLower ? tv.inv_shuffled : tv.shuffled;
+    if (kOrig || out_o) {
+        const int32_t* map = kOrig ? tv.inv_shuffled : tv.shuffled;
 #pragma unroll
         for (int u = 0; u < N; ++u) m[u] = load_i32_at(map, kb.key(J0 + u));
     }
     {
         int32_t tmp[N];
 #pragma unroll
-        for (int u = 0; u < N; ++u) tmp[u] = kLower ? m[u] : kb.key(J0 + u);
+        for (int u = 0; u < N; ++u) tmp[u] = kOrig ? m[u] : kb.key(J0 + u);
         uint4 v[N * 4 / 16];
         __builtin_memcpy(v, tmp, sizeof(tmp));
         wave_store_part<T, BLK, SW>(st, lane, qr, reinterpret_cast<unsigned char*>(out_i), (int64_t)K * 4, J0 * 4, v);
@@ -967,13 +971,13 @@ __device__ __forceinline__ void store_list_wave(const List& kb, const TreeView<T
     if (out_o) {
         int64_t tmp[N];
 #pragma unroll
-        for (int u = 0; u < N; ++u) tmp[u] = (int64_t)(kLower ? kb.key(J0 + u) : m[u]);
+        for (int u = 0; u < N; ++u) tmp[u] = (int64_t)(kOrig ? kb.key(J0 + u) : m[u]);
         uint4 v[N * 8 / 16];
         __builtin_memcpy(v, tmp, sizeof(tmp));
         wave_store_part<T, BLK, SW>(st, lane, qr, reinterpret_cast<unsigned char*>(out_o), (int64_t)K * 8, J0 * 8, v);
     }
     if constexpr (J0 + N < K)
-        store_list_wave<T, K, kLower, BLK, SW, J0 + N>(kb, tv, st, lane, qr, out_d, out_i, out_o);
+        store_list_wave<T, K, kOrig, BLK, SW, J0 + N>(kb, tv, st, lane, qr, out_d, out_i, out_o);
 }
 
 template <typename T, int D>
@@ -991,9 +995,8 @@ __device__ __forceinline__ void load_query(const T* __restrict__ queries, int64_
 // and lists of 8 or more: TK_STAGE_CHUNK_F32_K16 (16) for the single-tree reference-policy
 // search with a 16-entry list (plain workload 8.32 ms against 8.44 at 8), TK_STAGE_CHUNK_F32 (8)
 // for the other reference-policy ones (k = 8: 6.14 ms against 6.20 at 16; batched replay 21.76 ms
-// against 21.93), TK_STAGE_CHUNK_F32_TIE (8) for the lower-index policy with the KBestTie list
-// (see TK_F32_TIE_WAVES_K16), TK_STAGE_CHUNK_F32_LOWER (4) for its 64-bit keyed lists (10.14 ms
-// against 10.56 at 8).  For lists of at most 4 at d <= 3, TK_STAGE_CHUNK_SMALLK (8) in the single-tree search
+// against 21.93); the lower-index policy's KBestTie lists take the same, its 64-bit keyed lists
+// TK_STAGE_CHUNK_F32_LOWER (4; 10.14 ms against 10.56 at 8).  For lists of at most 4 at d <= 3, TK_STAGE_CHUNK_SMALLK (8) in the single-tree search
 // (k = 4: 4.78 ms against 4.91 at 4; k = 1: 3.75 against 3.80) and TK_STAGE_CHUNK_WIDE (4) in the
 // batched one (replay 20.43 ms against 20.74 at 8).  TK_STAGE_CHUNK_WIDE (4) also for d = 8, where a
 // larger chunk's extra stale-worst pops cost more than its fewer exits; TK_STAGE_CHUNK_F64 for double.
@@ -1003,8 +1006,8 @@ struct StageBytes {
     static constexpr bool kTieList = !kB && TreeList<T, K, kLower>::kTie;
     static constexpr int kChunk =
         sizeof(T) == 4
-            ? ((D <= 3 && K >= 8) ? (kLower ? (kTieList ? TK_STAGE_CHUNK_F32_TIE : TK_STAGE_CHUNK_F32_LOWER)
-                                            : (K == 16 && !kB ? TK_STAGE_CHUNK_F32_K16 : TK_STAGE_CHUNK_F32))
+            ? ((D <= 3 && K >= 8) ? (kLower && !kTieList ? TK_STAGE_CHUNK_F32_LOWER
+                                                         : (K == 16 && !kB ? TK_STAGE_CHUNK_F32_K16 : TK_STAGE_CHUNK_F32))
                                   : (D <= 3 && !kB ? TK_STAGE_CHUNK_SMALLK : TK_STAGE_CHUNK_WIDE))
             : TK_STAGE_CHUNK_F64;
     static constexpr int kLoads = sizeof(T) == 4 ? (TK_SCAN_LOADS_F32 < kChunk ? TK_SCAN_LOADS_F32 : kChunk)
@@ -1013,7 +1016,7 @@ struct StageBytes {
 };
 
 // The lower-index search of q again, node by node with the 64-bit keyed list, into kb (for a lane
-// whose KBestTie list is unsure).
+// whose KBestTie list is unsure), with the original indices turned into structured ones.
 template <typename T, int D, int K, bool kB>
 __device__ __forceinline__ void research_lower(const TreeView<T, kB>& tv, const Pt<T, D>& q, int k,
                                                KBestTie<T, K>& kb) {
@@ -1022,7 +1025,7 @@ __device__ __forceinline__ void research_lower(const TreeView<T, kB>& tv, const 
     RegListRef<T, KBest<T, K, true>> ref{ex};
     knn_search_one<T, D, true>(tv, q, ref);
 #pragma unroll
-    for (int j = 0; j < K; ++j) kb.set(j, ex.dist(j), ex.key(j));
+    for (int j = 0; j < K; ++j) kb.set(j, ex.dist(j), load_i32_at(tv.inv_shuffled, ex.key(j)));
 }
 
 // One lane per query.  The lanes of a wavefront take 64 consecutive presorted queries and run
@@ -1036,15 +1039,17 @@ __device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T
                                                   int32_t* __restrict__ out_i, int64_t* __restrict__ out_o) {
     static_assert(CH <= kRecPad, "a chunk may read at most kRecPad records past a leaf's end");
     static_assert(BLK % 64 == 0, "whole wavefronts per block");
+    using List = typename TreeList<T, K, kLower>::type;
+    constexpr bool kO = List::kOrigKeys;  // list keys are original indices (staged with the distances)
     __shared__ __align__(16) T stage_d[CH * BLK];
-    __shared__ int32_t stage_o[kLower ? CH * BLK : 1];
+    __shared__ int32_t stage_o[kO ? CH * BLK : 1];
     // t = wave0 + lane with wave0 in scalars; the lane index and the row are found again after the
     // search, so that no per-lane index has to stay live through it
     const int wave_off = __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
     const int64_t wave0 = xcd_tile(blockIdx.x, gridDim.x) * BLK + wave_off;
     if (wave0 + (int)(threadIdx.x & 63u) >= nq) return;
     Pt<T, D> q, pp;
-    typename TreeList<T, K, kLower>::type kb;
+    List kb;
     uint32_t far_mask;
     int32_t node;
     int level;
@@ -1057,7 +1062,7 @@ __device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T
         nan_fill<T, D, K, kLower>(tv, q, k, kb);
     } else {
         while (!lane_step<T, D, K, kLower, CH, LD, BLK, kGroups>(tv, q, pp, kb, far_mask, node, level, stage_d + threadIdx.x,
-                                                      stage_o + (kLower ? threadIdx.x : 0))) {
+                                                      stage_o + (kO ? threadIdx.x : 0))) {
         }
         if constexpr (TreeList<T, K, kLower>::kTie)
             if (kb.unsure(k)) research_lower<T, D, K>(tv, q, k, kb);
@@ -1083,11 +1088,11 @@ __device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T
 #pragma unroll
             for (int p = 0; p < C; ++p) qr[p] = __shfl(qi, p * (64 / C) + lane / C, 64);
             const WaveStage<T, BLK> st{reinterpret_cast<unsigned char*>(stage_d + wave_off)};
-            store_list_wave<T, K, kLower, BLK, SW>(kb, tv, st, lane, qr, out_d, out_i, out_o);
+            store_list_wave<T, K, kO, BLK, SW>(kb, tv, st, lane, qr, out_d, out_i, out_o);
             return;
         }
     }
-    store_list<T, K, kLower>(kb, k, tv, qi, out_d, out_i, out_o);
+    store_list<T, K, kO>(kb, k, tv, qi, out_d, out_i, out_o);
 }
 
 // kGroups: the tree has record-group boxes (tv.group_boxes), and leaf scans test them.
