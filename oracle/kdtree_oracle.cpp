@@ -293,8 +293,9 @@ template <typename T, int D>
 void search_all(const OTree<T, D>& t, const T* qflat, int64_t nq, int k, T* dist, int32_t* idx,
                 int nthreads, bool recursive) {
     const auto* qs = reinterpret_cast<const std::array<T, D>*>(qflat);
-    if (nthreads > 0) omp_set_num_threads(nthreads);
-#pragma omp parallel for
+    // a num_threads clause, not omp_set_num_threads: the caller's default team size stays as it was
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for num_threads(nt)
     for (int64_t i = 0; i < nq; ++i) {
         const std::array<T, D> q = qs[i];
         std::vector<T> bd(k, std::numeric_limits<T>::infinity());
@@ -315,8 +316,8 @@ void bruteforce(const T* rflat, int32_t n, const T* qflat, int64_t nq, int k, T*
                 int nthreads) {
     const auto* rs = reinterpret_cast<const std::array<T, D>*>(rflat);
     const auto* qs = reinterpret_cast<const std::array<T, D>*>(qflat);
-    if (nthreads > 0) omp_set_num_threads(nthreads);
-#pragma omp parallel for schedule(dynamic, 16)
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
     for (int64_t i = 0; i < nq; ++i) {
         std::vector<T> bd(k, std::numeric_limits<T>::infinity());
         std::vector<int32_t> bi(k, 0);

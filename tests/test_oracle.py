@@ -81,6 +81,21 @@ def _numpy_sq_dist(q, r):
     return s
 
 
+def test_thread_count_argument_leaves_the_process_default():
+    """A search or brute force with an explicit thread count runs its own team of that size and
+    leaves the process's OpenMP default as it was (the library's multi-tree host build takes that
+    default, and ran on one thread after a one-thread oracle run when it did not)."""
+    import ctypes
+    gomp = ctypes.CDLL("libgomp.so.1")
+    before = gomp.omp_get_max_threads()
+    rng = np.random.default_rng(5)
+    ref = rng.random((2000, 3)).astype(np.float32)
+    t = oracle.OracleTree(ref, 6)
+    t.search(ref[:100], 4, nthreads=1)
+    oracle.bruteforce(ref, ref[:100], 4, nthreads=1)
+    assert gomp.omp_get_max_threads() == before
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("d", [1, 2, 3, 8])
 def test_search_against_bruteforce_and_recursive(dtype, d):
