@@ -132,6 +132,17 @@ int tk_tree_variant_info(int variant, int32_t out[3], int* count);
 int tk_tree_set_variant(tk_tree* tree, int variant);
 int tk_tree_get_variant(const tk_tree* tree, int* variant);
 
+/* The list of the tree's lower-index float GPU searches with k in 5..32 (no counterpart in the
+ * reference, whose GPU kernel keeps one LDS list, src/kdtree_g.cu:285-407): mode 0 = chosen at
+ * upload from a sample of the cloud, 1 = the reference policy's register list with tie flags
+ * (each query that meets an exact distance tie is searched again with the (distance, original
+ * index) list), 2 = the (distance, original index) list from the start.  Mode 0 takes 2 for a
+ * cloud whose sample shows duplicated points or coordinates on few distinct values, and for a tree
+ * with a non-finite value; 1 otherwise.  Results never depend on the mode.
+ * tk_tree_get_lower_list: out[0] = mode, out[1] = 1 when mode 0 takes list 2 for this tree. */
+int tk_tree_set_lower_list(tk_tree* tree, int mode);
+int tk_tree_get_lower_list(const tk_tree* tree, int32_t out[2]);
+
 /* Replaces KDTree::get_shuffled_inds (src/interface.cpp:45-50): out[n_points] receives the
  * original index of each structured position. */
 int tk_tree_get_shuffled_inds(const tk_tree* tree, int32_t* out);

@@ -67,18 +67,22 @@ def test_tune_lower_index_policy_kept_apart(cuda, tmp_path, monkeypatch):
     assert refp.kdtree.get_variant() == 2
 
 
-@pytest.mark.parametrize("tie", ["reference", "lower_index"])
+@pytest.mark.parametrize("tie", ["reference", "lower_index", "lower_index_mode2"])
 @pytest.mark.parametrize("k", [6, 16])
 def test_every_variant_gives_the_same_results(cuda, tie, k):
     """Each prebuilt launch variant (chunk, occupancy target, block) returns variant 0's results, bit
-    for bit, which are the oracle's (reference policy) or brute force's (lower-index policy)."""
+    for bit, which are the oracle's (reference policy) or brute force's (lower-index policy, with
+    the list chosen from the cloud, or the (distance, index) list forced)."""
     import oracle
     rng = np.random.default_rng(31 + k)
     ref = rng.random((120_000, 3)).astype(np.float32)
     qry = (rng.random((40_000, 3)) * 1.2 - 0.1).astype(np.float32)
+    mode = 2 if tie.endswith("mode2") else 0
+    tie = "lower_index" if tie.startswith("lower_index") else tie
     base = None
     for v in range(len(torch_knn.tree_variants())):
         tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels=12, tie_break=tie, variant=v)
+        tree.kdtree.set_lower_list(mode)
         out = gpu_query(tree, qry, k, cuda)
         assert tree.kdtree.get_variant() == v
         if base is None:

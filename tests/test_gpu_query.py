@@ -133,13 +133,15 @@ def test_unaligned_output_rows(cuda):
             assert_same(o.cpu().numpy(), o0, "%s k=%d original indices" % (policy, k))
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("d", [1, 2, 3, 8])
-@pytest.mark.parametrize("k", [5, 8, 12, 16])
-def test_lower_index_ties_among_distinct_distances(cuda, d, k):
-    """Lower-index policy, float lists of 8 and 16: a cloud where one point in ten has an exact
-    duplicate (every distance to the pair ties) and queries on and off the points, so that lanes
-    whose lists meet a tie (searched again with the 64-bit keyed list) share wavefronts with lanes
-    that meet none."""
+@pytest.mark.parametrize("k", [5, 8, 12, 16, 32])
+def test_lower_index_ties_among_distinct_distances(cuda, d, k, mode):
+    """Lower-index policy, float lists of 8 to 32, under each list mode (0: chosen from the cloud,
+    1: tie-flagged list, each unsure lane searched again, 2: (distance, index) list): a cloud where
+    one point in ten has an exact duplicate (every distance to the pair ties) and queries on and off
+    the points, so that under mode 1 lanes whose lists meet a tie share wavefronts with lanes that
+    meet none."""
     rng = np.random.default_rng(900 + d * 13 + k)
     base = rng.standard_normal((18000, d)).astype(np.float32)
     dup = base[rng.choice(len(base), 1800, replace=False)]
@@ -148,6 +150,8 @@ def test_lower_index_ties_among_distinct_distances(cuda, d, k):
                           ref[rng.choice(len(ref), 1000, replace=False)]])
     for levels in (None, 10):
         tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels=levels, tie_break="lower_index")
+        tree.kdtree.set_lower_list(mode)
+        assert tree.kdtree.get_lower_list() == (mode, True)  # the sample sees the duplicates
         dist, sidx, oidx = gpu_query(tree, qry, k, cuda)
         bd, bi = oracle.bruteforce(ref, qry, k)
         assert_same(dist, bd, "distances")
@@ -155,13 +159,17 @@ def test_lower_index_ties_among_distinct_distances(cuda, d, k):
         assert_same(tree.kdtree.get_shuffled_inds()[sidx], bi, "structured -> original")
 
 
-@pytest.mark.parametrize("policy", ["reference", "lower_index"])
+@pytest.mark.parametrize("policy", ["reference", "lower_index", "lower_index_mode1", "lower_index_mode2"])
 def test_exact_ties_on_integer_grid(cuda, policy):
-    """Points and queries on a small integer lattice: nearly every distance is tied."""
+    """Points and queries on a small integer lattice: nearly every distance is tied.  Under the
+    lower-index policy also with each list mode forced (the automatic choice takes mode 2 here)."""
     rng = np.random.default_rng(7)
     ref = rng.integers(0, 6, (5000, 3)).astype(np.float32)
     qry = (rng.integers(0, 6, (2000, 3)) + 0.5 * rng.integers(0, 2, (2000, 3))).astype(np.float32)
+    mode = int(policy[-1]) if policy.endswith(("mode1", "mode2")) else 0
+    policy = "lower_index" if policy.startswith("lower_index") else policy
     tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels=8, tie_break=policy)
+    tree.kdtree.set_lower_list(mode)
     for k in (1, 7, 20, 64):
         dist, sidx, oidx = gpu_query(tree, qry, k, cuda)
         if policy == "reference":
@@ -172,6 +180,27 @@ def test_exact_ties_on_integer_grid(cuda, policy):
             bd, bi = oracle.bruteforce(ref, qry, k)
             assert_same(dist, bd, "distances")
             assert_same(oidx, bi.astype(np.int64), "original indices")
+
+
+def test_lower_list_choice_from_the_cloud(cuda):
+    """The upload's sample test: continuous clouds take the tie-flagged list (mode 0 resolves to 1),
+    lattices, rounded coordinates, duplicated points and non-finite trees the (distance, index) list;
+    an unknown mode is refused."""
+    rng = np.random.default_rng(17)
+    cont = rng.random((50000, 3)).astype(np.float32)
+    cases = {
+        "uniform": (cont, False),
+        "normal": ((rng.standard_normal((50000, 3)) * 1e3).astype(np.float32), False),
+        "lattice": (rng.integers(0, 64, (50000, 3)).astype(np.float32), True),
+        "rounded": (np.round(cont, 3), True),
+        "duplicated": (np.concatenate([cont, cont[:5000]]), True),
+        "nan": (np.where(np.arange(50000)[:, None] == 7, np.nan, cont).astype(np.float32), True),
+    }
+    for name, (ref, tie_prone) in cases.items():
+        tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, tie_break="lower_index")
+        assert tree.kdtree.get_lower_list() == (0, tie_prone), name
+    with pytest.raises(RuntimeError):
+        tree.kdtree.set_lower_list(3)
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])

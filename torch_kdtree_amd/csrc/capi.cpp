@@ -229,6 +229,22 @@ int tk_tree_get_variant(const tk_tree* tree, int* variant) {
     return guarded([&] { *variant = tree_of(tree).variant; });
 }
 
+int tk_tree_set_lower_list(tk_tree* tree, int mode) {
+    return guarded([&] {
+        if (!tree || !tree->t) throw std::runtime_error("null tree handle");
+        if (mode < 0 || mode > 2) throw std::runtime_error("lower-index list mode must be 0, 1 or 2");
+        tree->t->lower_list = mode;
+    });
+}
+
+int tk_tree_get_lower_list(const tk_tree* tree, int32_t out[2]) {
+    return guarded([&] {
+        const tk::Tree& t = tree_of(tree);
+        out[0] = t.lower_list;
+        out[1] = t.tie_prone ? 1 : 0;
+    });
+}
+
 int tk_tree_get_shuffled_inds(const tk_tree* tree, int32_t* out) {
     return guarded([&] {
         const tk::Tree& t = tree_of(tree);

@@ -143,8 +143,55 @@ void free_device_tree(Tree& t) noexcept {
     t.dev = nullptr;
 }
 
+namespace {
+
+// Whether exact distance ties look common among the cloud's neighbours, from up to 256 leaves
+// spread over the tree and at most 64 points of each: more than one point in 256 with an exact
+// duplicate among its leaf's sampled points, or, on some axis, fewer distinct values than half
+// the sampled points (lattices, rounded coordinates).  Finite trees only (NaN has no order).
+template <typename TT>
+bool cloud_ties_often(const Tree& t) {
+    const int D = t.dims;
+    const TT* pts = reinterpret_cast<const TT*>(t.points.data());
+    const int32_t nl = t.n_leaves;
+    const int32_t S = std::min<int32_t>(nl, 256);
+    std::vector<std::vector<TT>> vals((size_t)D);
+    std::vector<int32_t> idx;
+    int64_t sampled = 0, dups = 0;
+    for (int32_t i = 0; i < S; ++i) {
+        const int32_t l = (int32_t)((int64_t)i * nl / S);
+        const int32_t b0 = t.leaf_off[(size_t)l];
+        const int32_t b1 = std::min(t.leaf_off[(size_t)l + 1], b0 + 64);
+        idx.resize((size_t)(b1 - b0));
+        for (int32_t s = b0; s < b1; ++s) idx[(size_t)(s - b0)] = s;
+        auto less = [&](int32_t a, int32_t b) {
+            for (int c = 0; c < D; ++c) {
+                const TT x = pts[(size_t)a * D + c], y = pts[(size_t)b * D + c];
+                if (x != y) return x < y;
+            }
+            return false;
+        };
+        std::sort(idx.begin(), idx.end(), less);
+        for (size_t j = 0; j + 1 < idx.size(); ++j) dups += !less(idx[j], idx[j + 1]) ? 1 : 0;
+        for (int32_t s = b0; s < b1; ++s)
+            for (int c = 0; c < D; ++c) vals[(size_t)c].push_back(pts[(size_t)s * D + c]);
+        sampled += b1 - b0;
+    }
+    if (sampled < 2) return false;
+    if (dups * 256 > sampled) return true;
+    for (auto& v : vals) {
+        std::sort(v.begin(), v.end());
+        const int64_t distinct = std::unique(v.begin(), v.end()) - v.begin();
+        if (distinct * 2 < sampled) return true;
+    }
+    return false;
+}
+
+}  // namespace
+
 void upload_tree(Tree& t, int device) {
     free_device_tree(t);
+    t.tie_prone = !t.finite || (t.dtype == kF32 ? cloud_ties_often<float>(t) : cloud_ties_often<double>(t));
     if (device < 0) TK_HIP_CHECK(hipGetDevice(&device));
     DeviceGuard g(device);
     std::unique_ptr<DeviceTree> d(new DeviceTree());

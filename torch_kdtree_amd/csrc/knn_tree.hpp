@@ -44,6 +44,10 @@ struct TreeView {
     // is +-inf, so the lower-index policy then prunes with point boxes only (they hold the actual
     // extents of every subtree whatever the order).
     bool split_bounds = true;
+    // Lower-index searches launch the 64-bit keyed list's kernel (kTieOK false) rather than the
+    // KBestTie one: a cloud whose neighbours tie often, where nearly every lane would be searched
+    // twice.  Read by the launcher.
+    bool lower_direct = false;
 
     __device__ __forceinline__ T split_at(int32_t node) const { return split[kBatched ? node_base + node : node]; }
     __device__ __forceinline__ int32_t leaf_begin(int32_t leaf) const {
@@ -145,17 +149,17 @@ constexpr int kGroupMinLeaf = 4 * kGroupRecs;
 #ifndef TK_WIDE_WAVES
 #define TK_WIDE_WAVES 4
 #endif
-template <typename T, int D, int K, bool kLower>
+// kTie: the lower-index search keeps a KBestTie list (TreeList).
+template <typename T, int D, int K, bool kLower, bool kTie = false>
 struct TreeOccupancy {
     static constexpr int kWaves =
         TK_WAVES_PER_EU >= 0 ? TK_WAVES_PER_EU
-        : (sizeof(T) == 4 && D <= 3 && K <= 16)
-            ? (kLower && !(TK_LOWER_TIE_LIST && (K == 8 || K == 16)) ? TK_F32_LOWER_WAVES : TK_F32_WAVES)
+        : (sizeof(T) == 4 && D <= 3 && K <= 16) ? (kLower && !kTie ? TK_F32_LOWER_WAVES : TK_F32_WAVES)
         : (sizeof(T) == 8 && D <= 3 && K == 16 && !kLower) ? TK_F64_WAVES
-        : (sizeof(T) == 4 && D == 8 && K == 32 && (!kLower || TK_LOWER_TIE_LIST)) ? TK_WIDE_WAVES
-                                                            : 0;
+        : (sizeof(T) == 4 && D == 8 && K == 32 && (!kLower || kTie)) ? TK_WIDE_WAVES
+                                                                     : 0;
 };
-#define TK_TREE_OCCUPANCY(T, D, K, L) __attribute__((amdgpu_waves_per_eu(TreeOccupancy<T, D, K, L>::kWaves > 0 ? TreeOccupancy<T, D, K, L>::kWaves : 1)))
+#define TK_TREE_OCCUPANCY(T, D, K, L, TIE) __attribute__((amdgpu_waves_per_eu(TreeOccupancy<T, D, K, L, TIE>::kWaves > 0 ? TreeOccupancy<T, D, K, L, TIE>::kWaves : 1)))
 // The same for the batched search kernels (more live state per lane: the cloud's offsets).
 #ifndef TK_BATCH_WAVES_K16
 #define TK_BATCH_WAVES_K16 4
@@ -291,7 +295,10 @@ struct KBestExact {
 // and unsure() adds equal distances among the k live entries (their order must follow the
 // original index).  The search prunes with the lower-index policy's tests, so every point with
 // distance <= W is scanned.  A lane that is unsure searches again with the lower-index list
-// (research_lower): rare on clouds without exact fp32 distance ties.
+// (research_lower), and the first search stops after the leaf pair where the lane became unsure:
+// rare on clouds without exact fp32 distance ties, and on clouds full of them (lattices, rounded
+// coordinates, duplicated points) the cost of the lower-index list's own search plus the part of
+// the first one up to its first tie.
 template <typename T, int K>
 struct KBestTie : KBest<T, K, false> {
     using Base = KBest<T, K, false>;
@@ -320,15 +327,16 @@ struct KBestTie : KBest<T, K, false> {
         for (int j = 0; j + 1 < K; ++j) t |= j >= K - k && Base::d[j] == Base::d[j + 1];
         return t;
     }
-};
-
-// The list interface of scan_leaf / knn_search_one over a register list.
-template <typename T, typename L>
-struct RegListRef {
-    L& l;
-    __device__ __forceinline__ T worst() const { return l.worst(); }
-    __device__ __forceinline__ bool accepts(T dist, int32_t orig) const { return l.accepts(dist, orig); }
-    __device__ __forceinline__ void insert(T dist, int32_t, int32_t orig) { l.insert(dist, orig); }
+    // unsure() during the search: equal entries are looked for only in a list without padding slots
+    // (k == K, a wave-uniform test), where every adjacent pair is live
+    __device__ __forceinline__ bool unsure_early(int k) const {
+        bool t = tie;
+        if (k == K) {
+#pragma unroll
+            for (int j = 0; j + 1 < K; ++j) t |= Base::d[j] == Base::d[j + 1];
+        }
+        return t;
+    }
 };
 
 // The list of a single-tree register search.  Lower-index float searches with lists of 8, 16 or
@@ -337,14 +345,15 @@ struct RegListRef {
 // distances (plain workload, k = 16: 9.96-9.99 ms against 8.29-8.63; k = 32: 22.1-22.2 against
 // 16.9-17.1; d8_mixture, k = 32, 500K queries: 87.1-90.4 against 78.1-79.1).  Lists of at most 4
 // keep it: their insertions are short.
-template <typename T, int K, bool kLower>
+// kTieOK false: the 64-bit keyed list in any case (trees whose clouds tie often, Tree::tie_prone).
+template <typename T, int K, bool kLower, bool kTieOK = true>
 struct TreeList {
     using type = KBest<T, K, kLower>;
     static constexpr bool kTie = false;
 };
-template <int K>
-struct TreeList<float, K, true> {
-    static constexpr bool kTie = TK_LOWER_TIE_LIST && (K == 8 || K == 16 || K == 32);
+template <int K, bool kTieOK>
+struct TreeList<float, K, true, kTieOK> {
+    static constexpr bool kTie = kTieOK && TK_LOWER_TIE_LIST && (K == 8 || K == 16 || K == 32);
     using type = typename std::conditional<kTie, KBestTie<float, K>, KBest<float, K, true>>::type;
 };
 
@@ -1001,9 +1010,9 @@ __device__ __forceinline__ void load_query(const T* __restrict__ queries, int64_
 // batched one (replay 20.43 ms against 20.74 at 8).  TK_STAGE_CHUNK_WIDE (4) also for d = 8, where a
 // larger chunk's extra stale-worst pops cost more than its fewer exits; TK_STAGE_CHUNK_F64 for double.
 // kB: the batched search.
-template <typename T, int D, int K, bool kLower, bool kB = false>
+template <typename T, int D, int K, bool kLower, bool kB = false, bool kTieOK = true>
 struct StageBytes {
-    static constexpr bool kTieList = !kB && TreeList<T, K, kLower>::kTie;
+    static constexpr bool kTieList = !kB && TreeList<T, K, kLower, kTieOK>::kTie;
     static constexpr int kChunk =
         sizeof(T) == 4
             ? ((D <= 3 && K >= 8) ? (kLower && !kTieList ? TK_STAGE_CHUNK_F32_LOWER
@@ -1015,8 +1024,18 @@ struct StageBytes {
     static constexpr int kBytes = kChunk * kTreeBlock * ((int)sizeof(T) + (kLower ? 4 : 0));
 };
 
-// The lower-index search of q again, node by node with the 64-bit keyed list, into kb (for a lane
-// whose KBestTie list is unsure), with the original indices turned into structured ones.
+// The list interface of scan_leaf / knn_search_one over a register list.
+template <typename T, typename L>
+struct RegListRef {
+    L& l;
+    __device__ __forceinline__ T worst() const { return l.worst(); }
+    __device__ __forceinline__ bool accepts(T dist, int32_t orig) const { return l.accepts(dist, orig); }
+    __device__ __forceinline__ void insert(T dist, int32_t, int32_t orig) { l.insert(dist, orig); }
+};
+
+// The lower-index search of q again, node by node with the 64-bit keyed list (little code beside
+// the staged search, for the rare unsure lane), into kb, with the original indices turned into
+// structured ones.
 template <typename T, int D, int K, bool kB>
 __device__ __forceinline__ void research_lower(const TreeView<T, kB>& tv, const Pt<T, D>& q, int k,
                                                KBestTie<T, K>& kb) {
@@ -1033,14 +1052,15 @@ __device__ __forceinline__ void research_lower(const TreeView<T, kB>& tv, const 
 // keeps the wavefront's control flow and loads coherent.  (Handing a finished lane the next query
 // was measured to be slower: it breaks that lockstep.)  BLK threads per block, leaf chunks of CH
 // records with LD loads in flight.
-template <typename T, int D, int K, bool kLower, int CH, int LD, int BLK, bool kGroups = false>
+template <typename T, int D, int K, bool kLower, int CH, int LD, int BLK, bool kGroups = false, bool kTieOK = true>
 __device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T* __restrict__ queries, int64_t nq,
                                                   const int32_t* __restrict__ order, int k, T* __restrict__ out_d,
                                                   int32_t* __restrict__ out_i, int64_t* __restrict__ out_o) {
     static_assert(CH <= kRecPad, "a chunk may read at most kRecPad records past a leaf's end");
     static_assert(BLK % 64 == 0, "whole wavefronts per block");
-    using List = typename TreeList<T, K, kLower>::type;
+    using List = typename TreeList<T, K, kLower, kTieOK>::type;
     constexpr bool kO = List::kOrigKeys;  // list keys are original indices (staged with the distances)
+    constexpr bool kTie = TreeList<T, K, kLower, kTieOK>::kTie;
     __shared__ __align__(16) T stage_d[CH * BLK];
     __shared__ int32_t stage_o[kO ? CH * BLK : 1];
     // t = wave0 + lane with wave0 in scalars; the lane index and the row are found again after the
@@ -1061,10 +1081,14 @@ __device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T
     if (has_nan<T, D>(q)) {
         nan_fill<T, D, K, kLower>(tv, q, k, kb);
     } else {
-        while (!lane_step<T, D, K, kLower, CH, LD, BLK, kGroups>(tv, q, pp, kb, far_mask, node, level, stage_d + threadIdx.x,
-                                                      stage_o + (kO ? threadIdx.x : 0))) {
+        // (a KBestTie lane stops after the leaf pair where it becomes unsure: it is searched again
+        // in any case)
+        while (!lane_step<T, D, K, kLower, CH, LD, BLK, kGroups>(tv, q, pp, kb, far_mask, node, level,
+                                                              stage_d + threadIdx.x, stage_o + (kO ? threadIdx.x : 0))) {
+            if constexpr (kTie)
+                if (kb.unsure_early(k)) break;
         }
-        if constexpr (TreeList<T, K, kLower>::kTie)
+        if constexpr (kTie)
             if (kb.unsure(k)) research_lower<T, D, K>(tv, q, k, kb);
     }
     // (the lane index from mbcnt in a volatile asm, and the row by an atomic load, so that the
@@ -1095,14 +1119,16 @@ __device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T
     store_list<T, K, kO>(kb, k, tv, qi, out_d, out_i, out_o);
 }
 
-// kGroups: the tree has record-group boxes (tv.group_boxes), and leaf scans test them.
-template <typename T, int D, int K, bool kLower, bool kGroups = false>
-__global__ __launch_bounds__(kTreeBlock) TK_TREE_OCCUPANCY(T, D, K, kLower) void knn_tree_kernel(TreeView<T> tv, const T* __restrict__ queries,
+// kGroups: the tree has record-group boxes (tv.group_boxes), and leaf scans test them.  kTieOK
+// false: a lower-index search keeps the 64-bit keyed list (a tree whose cloud ties often).
+template <typename T, int D, int K, bool kLower, bool kGroups = false, bool kTieOK = true>
+__global__ __launch_bounds__(kTreeBlock) TK_TREE_OCCUPANCY(T, D, K, kLower, (TreeList<T, K, kLower, kTieOK>::kTie)) void knn_tree_kernel(TreeView<T> tv, const T* __restrict__ queries,
                                                                int64_t nq, const int32_t* __restrict__ order, int k,
                                                                T* __restrict__ out_d, int32_t* __restrict__ out_i,
                                                                int64_t* __restrict__ out_o) {
-    tree_search_lanes<T, D, K, kLower, StageBytes<T, D, K, kLower>::kChunk, StageBytes<T, D, K, kLower>::kLoads,
-                      kTreeBlock, kGroups>(tv, queries, nq, order, k, out_d, out_i, out_o);
+    tree_search_lanes<T, D, K, kLower, StageBytes<T, D, K, kLower, false, kTieOK>::kChunk,
+                      StageBytes<T, D, K, kLower, false, kTieOK>::kLoads, kTreeBlock, kGroups, kTieOK>(
+        tv, queries, nq, order, k, out_d, out_i, out_o);
 }
 
 // The same search with its launch shape as template parameters: the prebuilt variants an autotuner

@@ -12,9 +12,26 @@ TreeView<T> tree_view(const Tree& t) {
     TreeView<T> v{static_cast<const T*>(t.dev->split), t.dev->recs, t.dev->leaf_off, t.dev->shuffled,
                   t.dev->inv_shuffled, t.dev->boxes, t.levels, t.n_nodes};
     v.split_bounds = t.finite;
+    v.lower_direct = t.lower_list == 2 || (t.lower_list == 0 && t.tie_prone);
     v.group_boxes = t.dev->group_boxes;
     v.group_base = t.dev->group_base;
     return v;
+}
+
+// One register-list kernel; a lower-index search on a tree whose cloud ties often
+// (tv.lower_direct) takes the 64-bit keyed list's instantiation.
+template <typename T, int D, int KK, bool kLower, bool kGroups>
+void launch_tree_kernel(const TreeView<T>& tv, dim3 grid, dim3 block, hipStream_t s, const T* q, int64_t nq,
+                        const int32_t* order, int k, T* dist, int32_t* idx, int64_t* orig) {
+    if constexpr (kLower && TreeList<T, KK, true>::kTie) {
+        if (tv.lower_direct) {
+            hipLaunchKernelGGL((knn_tree_kernel<T, D, KK, true, kGroups, false>), grid, block, 0, s, tv, q, nq, order, k,
+                               dist, idx, orig);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((knn_tree_kernel<T, D, KK, kLower, kGroups>), grid, block, 0, s, tv, q, nq, order, k, dist, idx,
+                       orig);
 }
 
 template <typename T, int D, bool kLower>
@@ -28,11 +45,9 @@ void launch_tree_td(const TreeView<T>& tv, const T* q, int64_t nq, const int32_t
 #define TK_LAUNCH_K(KK)                                                                                         \
     case KK:                                                                                                    \
         if (tv.group_boxes)                                                                                     \
-            hipLaunchKernelGGL((knn_tree_kernel<T, D, KK, kLower, true>), grid, block, 0, s, tv, q, nq, order, \
-                               k, dist, idx, orig);                                                             \
+            launch_tree_kernel<T, D, KK, kLower, true>(tv, grid, block, s, q, nq, order, k, dist, idx, orig);   \
         else                                                                                                    \
-            hipLaunchKernelGGL((knn_tree_kernel<T, D, KK, kLower, false>), grid, block, 0, s, tv, q, nq, order, \
-                               k, dist, idx, orig);                                                             \
+            launch_tree_kernel<T, D, KK, kLower, false>(tv, grid, block, s, q, nq, order, k, dist, idx, orig);  \
         break;
     switch (kb) {
         TK_LAUNCH_K(1)

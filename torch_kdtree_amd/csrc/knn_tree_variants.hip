@@ -39,6 +39,7 @@ bool launch_tree_variant_f32_d3(const Tree& t, const float* q, int64_t nq, const
                                 float* dist, int32_t* idx, int64_t* orig, bool lower, int variant, hipStream_t s) {
     if (t.dims != 3 || variant <= 0 || (kb != 8 && kb != 16)) return false;
     const TreeView<float> tv = tree_view<float>(t);
+    if (lower && tv.lower_direct) return false;  // the 64-bit keyed list's built-in kernel
     if (kb == 8)
         return lower ? launch_k<8, true>(tv, q, nq, order, k, dist, idx, orig, variant, s)
                      : launch_k<8, false>(tv, q, nq, order, k, dist, idx, orig, variant, s);

@@ -91,6 +91,12 @@ struct KDTree {
     // Every coordinate and split value finite (otherwise GPU queries under the reference tie
     // policy run the reference's own steps, without point-box pruning).
     void set_variant(int v) { check(tk_tree_set_variant(tree, v)); }
+    void set_lower_list(int mode) { check(tk_tree_set_lower_list(tree, mode)); }
+    py::tuple get_lower_list() const {
+        int32_t out[2] = {0, 0};
+        check(tk_tree_get_lower_list(tree, out));
+        return py::make_tuple(out[0], out[1] != 0);
+    }
     int get_variant() const {
         int v = 0;
         check(tk_tree_get_variant(tree, &v));
@@ -349,6 +355,9 @@ void bind_tree(py::module_& m, const char* name) {
         .def("set_variant", &C::set_variant, py::arg("variant"),
              "Launch variant of later GPU queries (see torch_knn.tree_variants())")
         .def("get_variant", &C::get_variant)
+        .def("set_lower_list", &C::set_lower_list, py::arg("mode"),
+             "List of later lower-index GPU queries: 0 = chosen from the cloud, 1 = tie-flagged, 2 = (distance, index)")
+        .def("get_lower_list", &C::get_lower_list, "(mode, whether mode 0 takes the (distance, index) list)")
         .def("search_stats", &C::search_stats, py::arg("queries"), py::arg("k"), py::arg("n_threads") = 0,
              "(leaves, points) scanned by the search of these host queries, summed")
         .def("query", &C::query, py::arg("points_query_ptr"), py::arg("nr_query_points"), py::arg("nr_nns_searches"),

@@ -47,6 +47,13 @@ struct Tree {
     bool finite = true;
     // Launch variant of the GPU traversal (kTreeVariants in knn_gpu.hpp; 0 = the built-in choice).
     int variant = 0;
+    // List of the GPU's lower-index float searches with 8 to 32 entries: 0 = chosen from the cloud
+    // (tie_prone), 1 = the reference policy's list with tie flags, each unsure lane searched again,
+    // 2 = the list keyed by (distance, original index) from the start.  Results never depend on it.
+    int lower_list = 0;
+    // Set at upload: exact distance ties look common among this cloud's neighbours (duplicated
+    // points, coordinates on few distinct values), or the tree holds a non-finite value.
+    bool tie_prone = false;
     DeviceTree* dev = nullptr;          // GPU copy, owned
 
     template <typename T> T* split_ptr() { return reinterpret_cast<T*>(split.data()); }
