@@ -283,10 +283,9 @@ struct KBestExact {
 // Lower-index policy on the reference policy's register list (one med3 per slot; the keys are
 // structured indices, as under the reference policy, since the key order no longer decides
 // anything: ties are settled by the flag below).  The two policies' lists can differ only through
-// exact distance ties: when
-// no point the search scans outside the list has a distance equal to the final worst W and the
-// list's k distances are distinct, both hold the k points of smallest distance in ascending
-// order.  `tie` records every event after which that may not hold:
+// exact distance ties: when no point the search scans outside the list has a distance equal to the
+// final worst W and the list's k distances are distinct, both hold the k points of smallest
+// distance in ascending order.  `tie` records every event after which that may not hold:
 // * a candidate equal to the current worst in the second pass (the first pass passes every
 //   distance <= the stale worst, so a point at W is always a candidate, and the current worst it
 //   then meets is >= W);
@@ -294,11 +293,10 @@ struct KBestExact {
 //   equal to W only if equal to that worst, which never falls below W; empty slots aside);
 // and unsure() adds equal distances among the k live entries (their order must follow the
 // original index).  The search prunes with the lower-index policy's tests, so every point with
-// distance <= W is scanned.  A lane that is unsure searches again with the lower-index list
-// (research_lower), and the first search stops after the leaf pair where the lane became unsure:
-// rare on clouds without exact fp32 distance ties, and on clouds full of them (lattices, rounded
-// coordinates, duplicated points) the cost of the lower-index list's own search plus the part of
-// the first one up to its first tie.
+// distance <= W is scanned.  A lane that becomes unsure stops after that leaf pair and searches
+// again, node by node, with the lower-index list (research_lower): rare on clouds without exact
+// fp32 distance ties.  Trees whose clouds are full of them (lattices, rounded coordinates,
+// duplicated points; Tree::tie_prone) launch the lower-index list's kernel instead.
 template <typename T, int K>
 struct KBestTie : KBest<T, K, false> {
     using Base = KBest<T, K, false>;
