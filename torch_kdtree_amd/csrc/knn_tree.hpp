@@ -854,10 +854,11 @@ __device__ __forceinline__ void store_slots(V* __restrict__ p, bool vec, int pad
 // Writes one query's list to row qi of the outputs: distances, structured indices and (with
 // out_o) original indices.  A list keyed by structured index (kOrig false) gathers the original
 // ones through `shuffled`; one keyed by original index (the lower-index policy's 64-bit keyed
-// list) gathers the structured ones through `inv_shuffled`.  The gathers of up to 16 slots are issued before their
-// stores, so their latencies overlap instead of following one another (a 32-entry list goes out
-// in two halves, which keeps the live registers below those of its search); when k == K and the
-// three arrays are 16-byte aligned, the rows go out in 16-byte stores.
+// list) gathers the structured ones through `inv_shuffled`.  The gathers of up to 16 slots are
+// issued before their stores, so their latencies overlap instead of following one another (a
+// 32-entry list goes out in two halves, which keeps the live registers below those of its
+// search); when k == K and the three arrays are 16-byte aligned, the rows go out in 16-byte
+// stores.
 template <typename T, int K, bool kOrig, int J0 = 0, typename List>
 __device__ __forceinline__ void store_list(const List& kb, int k, const TreeView<T>& tv, int64_t qi,
                                            T* __restrict__ out_d, int32_t* __restrict__ out_i,
@@ -1120,10 +1121,9 @@ __device__ __forceinline__ void tree_search_lanes(const TreeView<T>& tv, const T
 // kGroups: the tree has record-group boxes (tv.group_boxes), and leaf scans test them.  kTieOK
 // false: a lower-index search keeps the 64-bit keyed list (a tree whose cloud ties often).
 template <typename T, int D, int K, bool kLower, bool kGroups = false, bool kTieOK = true>
-__global__ __launch_bounds__(kTreeBlock) TK_TREE_OCCUPANCY(T, D, K, kLower, (TreeList<T, K, kLower, kTieOK>::kTie)) void knn_tree_kernel(TreeView<T> tv, const T* __restrict__ queries,
-                                                               int64_t nq, const int32_t* __restrict__ order, int k,
-                                                               T* __restrict__ out_d, int32_t* __restrict__ out_i,
-                                                               int64_t* __restrict__ out_o) {
+__global__ __launch_bounds__(kTreeBlock) TK_TREE_OCCUPANCY(T, D, K, kLower, (TreeList<T, K, kLower, kTieOK>::kTie))
+void knn_tree_kernel(TreeView<T> tv, const T* __restrict__ queries, int64_t nq, const int32_t* __restrict__ order,
+                     int k, T* __restrict__ out_d, int32_t* __restrict__ out_i, int64_t* __restrict__ out_o) {
     tree_search_lanes<T, D, K, kLower, StageBytes<T, D, K, kLower, false, kTieOK>::kChunk,
                       StageBytes<T, D, K, kLower, false, kTieOK>::kLoads, kTreeBlock, kGroups, kTieOK>(
         tv, queries, nq, order, k, out_d, out_i, out_o);
