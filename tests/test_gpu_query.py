@@ -208,22 +208,33 @@ def test_lower_list_choice_from_the_cloud(cuda):
 def test_record_group_boxes_big_leaves(cuda, dtype, d):
     """Trees whose leaves hold 64 or more points on average carry a box per run of 16 records of a
     leaf, and a leaf scan skips the runs whose box lies at least the worst distance away.  Random
-    points with queries on and off them, and a tie-heavy integer lattice with duplicate points, at
-    3,000 and 375 points a leaf: every result equals the oracle's (reference policy) and brute
-    force's (lower-index policy)."""
+    points with queries on and off them, a tie-heavy integer lattice with duplicate points, and
+    random points with a few duplicates, at 3,000 and 375 points a leaf, under every lower-index
+    list mode: every result equals the oracle's (reference policy) and brute force's (lower-index
+    policy)."""
     rng = np.random.default_rng(40 + d)
     rnd_ref, rnd_q = _data(rng, 12000, 1500, d, dtype)
     lat_ref = rng.integers(0, 5, (12000, d)).astype(dtype)
     lat_q = (rng.integers(0, 5, (1500, d)) + 0.5 * rng.integers(0, 2, (1500, d))).astype(dtype)
+    # a few exact duplicates, too few for the upload's sample test: lower-index float searches keep
+    # the tie-flagged list, and the lanes at the duplicates are searched again
+    dup = rnd_ref[rng.choice(len(rnd_ref), 24, replace=False)]
+    sd_ref = np.concatenate([rnd_ref, dup])[rng.permutation(len(rnd_ref) + len(dup))]
+    sd_q = np.ascontiguousarray(np.concatenate([rnd_q, dup, dup + dtype(0.5)]))
     cases = (("random", rnd_ref, np.ascontiguousarray(np.concatenate([rnd_q, rnd_ref[:300]]))),
-             ("lattice", lat_ref, lat_q))
+             ("lattice", lat_ref, lat_q), ("sparse duplicates", sd_ref, sd_q))
     for name, ref, qry in cases:
         for levels in (2, 5):
-            for policy in ("reference", "lower_index"):
+            for policy in ("reference", "lower_index", "lower_index_mode1", "lower_index_mode2"):
+                mode = int(policy[-1]) if policy.endswith(("mode1", "mode2")) else 0
+                policy = "lower_index" if policy.startswith("lower_index") else policy
                 tree = tk.build_kd_tree(torch.from_numpy(ref), device=cuda, levels=levels, tie_break=policy)
+                tree.kdtree.set_lower_list(mode)
+                if name != "lattice":
+                    assert not tree.kdtree.get_lower_list()[1], name
                 ot = oracle_for(tree) if policy == "reference" else None
                 for k in (1, 5, 16, 32, 40):
-                    what = "%s levels %d %s k %d" % (name, levels, policy, k)
+                    what = "%s levels %d %s mode %d k %d" % (name, levels, policy, mode, k)
                     dist, sidx, oidx = gpu_query(tree, qry, k, cuda)
                     if policy == "reference":
                         od, oi = ot.search(qry, k)
